@@ -1,0 +1,232 @@
+/*
+ * qsm_hip.h -- C ABI of libqsm_hip.so, the MI355X (gfx950) implementation of
+ * the QSM.m primal-dual TV/TGV dipole-inversion path.
+ *
+ * The entry points replace the reference's C-MEX compute routines one for
+ * one; the MEX wrappers in qsm.m_amd/mex/ keep the reference's mexFunction
+ * argument layouts and call these.  Signatures use plain pointers and sizes
+ * only.  Conventions shared by every entry point:
+ *
+ *   dtype    QSM_F64 (MATLAB double) or QSM_F32 (MATLAB single); the MEX
+ *            layer picks it from mxIsSingle(prhs[0]) like the reference.
+ *   sz[3]    nx, ny, nz; column-major, x fastest, l = i + nx*j + nx*ny*k.
+ *   h[3]     voxel size (always double, as the reference reads prhs[k]).
+ *   masks    uint8 (MATLAB logical), non-zero = inside.
+ *   vector fields (w, p, q) are component-major: component c of voxel l is
+ *            at c*N + l.
+ *   return   QSM_OK or a QSM_E* code; qsm_last_error() describes the last
+ *            failure of the calling thread.
+ *
+ * Host-pointer entry points are synchronous from the caller's view (they
+ * return after the outputs are in host memory), like a MEX call.  The
+ * *_dev entry points take device pointers and a hipStream_t passed as
+ * void* (NULL = the null stream) and are asynchronous on that stream.
+ */
+#ifndef QSM_HIP_H
+#define QSM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QSM_OK       0
+#define QSM_EARG     1   /* invalid argument (sizes, dtype, null pointer) */
+#define QSM_EHIP     2   /* HIP runtime error */
+#define QSM_ENOMEM   3   /* device allocation failed */
+#define QSM_ENODEV   4   /* no usable HIP device */
+#define QSM_EFULL    5   /* admission refused: over the HBM budget */
+
+#define QSM_F64 0
+#define QSM_F32 1
+
+#define QSM_TV  0
+#define QSM_TGV 1
+
+/* Receives the progress lines the reference prints with mexPrintf
+ * (pd_tgv_mex.c:292-293, 313-315).  NULL prints to stdout. */
+typedef void (*qsm_print_fn)(const char *line, void *ctx);
+
+/* Library version (major*10000 + minor*100 + patch). */
+int qsm_version(void);
+
+/* Text of the calling thread's last error (never NULL). */
+const char *qsm_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Primal-dual dipole inversion (drop-in, host pointers, synchronous)
+ * ------------------------------------------------------------------------ */
+
+/* Replaces pd_tgv_mex.c:104-135 (mexFunction -> mx_init -> mx_pdd/mx_pdf ->
+ * mxDuplicateArray(chi)).  d2f: solver input (the MEX's prhs[0], i.e.
+ * -lapw(phase)); mask: prhs[1]; lam, alpha, tol, maxiter, verbose: prhs[3..7]
+ * (alpha0 = alpha*lam, alpha1 = lam).  chi: N outputs.  maxiter == 0 is
+ * rejected with QSM_EARG (the reference divides by zero there). */
+int qsm_pd_tgv(int dtype, const void *d2f, const uint8_t *mask,
+               const size_t sz[3], const double h[3],
+               double lam, double alpha, double tol,
+               uint32_t maxiter, uint32_t verbose,
+               void *chi, qsm_print_fn print, void *print_ctx);
+
+/* Replaces pd_tv_mex.c:100-130 (prhs: d2f, mask, vsz, lam, tol, maxiter,
+ * verbose). */
+int qsm_pd_tv(int dtype, const void *d2f, const uint8_t *mask,
+              const size_t sz[3], const double h[3],
+              double lam, double tol, uint32_t maxiter, uint32_t verbose,
+              void *chi, qsm_print_fn print, void *print_ctx);
+
+/* ------------------------------------------------------------------------
+ * Resident solver handle: the state of one inversion kept in HBM between
+ * calls (used by the drop-in entry points above, by the benchmark, and by the
+ * batching/serving runtime).  Replaces the mx_init/mx_pdd/mx_cleanup life
+ * cycle (pd_tgv_mex.c:1124-1231) with an explicit one.
+ * ------------------------------------------------------------------------ */
+typedef struct qsm_pd qsm_pd;
+
+/* kind: QSM_TV or QSM_TGV.  alpha is ignored for TV.  stream: hipStream_t
+ * or NULL.  Allocates the state (TGV: 31 words/voxel, TV: 17 words/voxel,
+ * plus one mask byte). */
+int qsm_pd_create(qsm_pd **out, int kind, int dtype, const size_t sz[3],
+                  const double h[3], double lam, double alpha, void *stream);
+
+/* Bytes of device memory qsm_pd_create would allocate for this problem. */
+size_t qsm_pd_footprint(int kind, int dtype, const size_t sz[3]);
+
+/* Loads the input (N values of dtype) and the mask (N bytes), zeroes the
+ * iterates and scales f by sigma (pd_tgv_mex.c:280-283).  on_device != 0:
+ * d2f and mask are device pointers (device-to-device copies on the stream). */
+int qsm_pd_load(qsm_pd *s, const void *d2f, const uint8_t *mask,
+                int on_device);
+
+/* Enqueues niter further iterations (the reference's loop body,
+ * pd_tgv_mex.c:296-320, including the every-10th-iteration convergence
+ * check against tol; tol <= 0 never stops).  Asynchronous: nothing is
+ * waited for.  Iterations after a stop are no-ops on the device. */
+int qsm_pd_iterate(qsm_pd *s, uint32_t niter, double tol);
+
+/* Waits for the stream and reports: iterations done (the reference's
+ * iteration count at exit), whether the tolerance test stopped the loop,
+ * and the last convergence norms nr[2] (any pointer may be NULL). */
+int qsm_pd_status(qsm_pd *s, uint32_t *iters, int *stopped, double nr[2]);
+
+/* Convergence norms of every check so far: nr[2*c], nr[2*c+1] for check c
+ * (iteration 10*(c+1)).  Returns the number of checks written (<= cap). */
+int qsm_pd_norm_history(qsm_pd *s, double *nr, int cap);
+
+/* Copies the current chi (N values) out; on_device != 0: chi is a device
+ * pointer (copy enqueued on the stream, not waited for). */
+int qsm_pd_get_chi(qsm_pd *s, void *chi, int on_device);
+
+/* Frees the handle and its device memory. */
+void qsm_pd_destroy(qsm_pd *s);
+
+/* Selects the sweep schedule of a handle: 0 = fused single pass per
+ * iteration (default), 1 = the reference's two sweeps per iteration (dual
+ * kernel then primal kernel).  Both give identical results. */
+int qsm_pd_set_schedule(qsm_pd *s, int schedule);
+
+/* ------------------------------------------------------------------------
+ * Primal-dual single sweeps on device arrays (per-operator access; the
+ * layouts are the reference's, asynchronous on stream)
+ * ------------------------------------------------------------------------ */
+
+/* pd_tgv_mex.c:493-646 / :331-485 (update_duald / update_dualf). */
+int qsm_tgv_dual_dev(int dtype, void *nu, void *p, void *q,
+                     const void *chi_, const void *eta_, const void *w_,
+                     const void *f, const uint8_t *mi,
+                     const size_t sz[3], const double h[3],
+                     double alpha0, double alpha1, double sigma, void *stream);
+
+/* pd_tgv_mex.c:824-977 / :658-812 (update_primald / update_primalf). */
+int qsm_tgv_primal_dev(int dtype, void *chi, void *eta, void *w,
+                       void *chi_, void *eta_, void *w_,
+                       const void *nu, const void *p, const void *q,
+                       const uint8_t *m, const size_t sz[3], const double h[3],
+                       double tau, void *stream);
+
+/* pd_tv_mex.c:417-513 / :313-410. */
+int qsm_tv_dual_dev(int dtype, void *nu, void *p,
+                    const void *chi_, const void *eta_,
+                    const void *f, const uint8_t *mi,
+                    const size_t sz[3], const double h[3],
+                    double lam, double sigma, void *stream);
+
+/* pd_tv_mex.c:630-727 / :523-620. */
+int qsm_tv_primal_dev(int dtype, void *chi, void *eta,
+                      void *chi_, void *eta_,
+                      const void *nu, const void *p,
+                      const uint8_t *m, const size_t sz[3], const double h[3],
+                      double tau, void *stream);
+
+/* pd_tgv_mex.c:1049-1121 (init_bitmask): mi from m. */
+int qsm_init_mask_dev(uint8_t *mi, const uint8_t *m, const size_t sz[3],
+                      void *stream);
+
+/* pd_tgv_mex.c:980-1045 (convergence_checkd/f): writes out[0..1] = n1, n2
+ * (host pointer; waits for the stream).  Returns the stop flag in *flag. */
+int qsm_convergence_dev(int dtype, const void *chi, const void *chi0,
+                        const void *eta, const void *eta0, size_t N,
+                        double tol, double out[2], int *flag, void *stream);
+
+/* pd_tgv_mex.c:272-278: tau and sigma for voxel size h. */
+void qsm_step_sizes(const double h[3], double *tau, double *sigma);
+
+/* ------------------------------------------------------------------------
+ * Finite-difference operators (drop-in, host pointers, synchronous).
+ * Outputs are overwritten in place, as the reference MEX overwrite their
+ * caller-allocated prhs arrays.
+ * ------------------------------------------------------------------------ */
+
+/* gradfp_mex.c:13-47 -> gradfpd/gradfpf (:147-242 / :50-144). */
+int qsm_gradfp(int dtype, void *dx, void *dy, void *dz, const void *u,
+               const size_t sz[3], const double h[3]);
+/* gradfp_adj_mex.c:15-49 -> gradfp_adjd/f (:156-257 / :52-153). */
+int qsm_gradfp_adj(int dtype, void *du, const void *x, const void *y,
+                   const void *z, const size_t sz[3], const double h[3]);
+/* gradf_mex.c:13-47 -> gradfd/gradff (:149-245 / :50-146).  Needs every
+ * dimension >= 2 (the reference reads out of bounds otherwise). */
+int qsm_gradf(int dtype, void *dx, void *dy, void *dz, const void *u,
+              const size_t sz[3], const double h[3]);
+/* gradfm_mex.c:15-50 -> gradfmd/gradfmf (:105-154 / :53-102). */
+int qsm_gradfm(int dtype, void *dx, void *dy, void *dz, const void *u,
+               const uint8_t *G, const size_t sz[3], const double h[3]);
+/* gradf_adj_mex.c:15-49 -> gradf_adjd/f (:152-249 / :52-149).  Needs every
+ * dimension >= 2. */
+int qsm_gradf_adj(int dtype, void *du, const void *x, const void *y,
+                  const void *z, const size_t sz[3], const double h[3]);
+/* gradfm_adj_mex.c:15-50 -> gradfm_adjd/f (:109-162 / :53-106). */
+int qsm_gradfm_adj(int dtype, void *du, const void *x, const void *y,
+                   const void *z, const uint8_t *G,
+                   const size_t sz[3], const double h[3]);
+/* lapw_mex.c:21-51 -> lapwd/lapwf (:112-187 / :54-109).  Interior voxels
+ * are written; the faces of d2u are left as the caller passed them. */
+int qsm_lapw(int dtype, void *d2u, const void *u,
+             const size_t sz[3], const double h[3]);
+
+/* Device-pointer forms of the operators above (asynchronous on stream). */
+int qsm_gradfp_dev(int dtype, void *dx, void *dy, void *dz, const void *u,
+                   const size_t sz[3], const double h[3], void *stream);
+int qsm_gradfp_adj_dev(int dtype, void *du, const void *x, const void *y,
+                       const void *z, const size_t sz[3], const double h[3],
+                       void *stream);
+int qsm_gradf_dev(int dtype, void *dx, void *dy, void *dz, const void *u,
+                  const size_t sz[3], const double h[3], void *stream);
+int qsm_gradfm_dev(int dtype, void *dx, void *dy, void *dz, const void *u,
+                   const uint8_t *G, const size_t sz[3], const double h[3],
+                   void *stream);
+int qsm_gradf_adj_dev(int dtype, void *du, const void *x, const void *y,
+                      const void *z, const size_t sz[3], const double h[3],
+                      void *stream);
+int qsm_gradfm_adj_dev(int dtype, void *du, const void *x, const void *y,
+                       const void *z, const uint8_t *G, const size_t sz[3],
+                       const double h[3], void *stream);
+int qsm_lapw_dev(int dtype, void *d2u, const void *u, const size_t sz[3],
+                 const double h[3], void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QSM_HIP_H */

@@ -1,0 +1,60 @@
+// common.h -- shared host/device helpers of libqsm_hip.so (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "qsm_hip.h"
+
+namespace qsm {
+
+// Records the calling thread's last error text (printf-style).
+void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+
+// Maps a HIP error to a status code, recording where it happened.
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define QSM_HIP(...)                                                         \
+    do {                                                                     \
+        hipError_t qsm_e_ = (__VA_ARGS__);                                   \
+        if (qsm_e_ != hipSuccess)                                            \
+            return ::qsm::hip_fail(qsm_e_, #__VA_ARGS__, __FILE__, __LINE__);\
+    } while (0)
+
+#define QSM_ARG(cond, ...)                                                   \
+    do {                                                                     \
+        if (!(cond)) {                                                       \
+            ::qsm::set_error(__VA_ARGS__);                                   \
+            return QSM_EARG;                                                 \
+        }                                                                    \
+    } while (0)
+
+// Checks sz (every dimension >= 1, N*6 addressable with 32-bit voxel
+// indices) and h (finite, > 0).  min_dim: smallest allowed dimension.
+int check_grid(const size_t sz[3], const double h[3], size_t min_dim);
+
+// Ensures a HIP device is present and current; QSM_ENODEV otherwise.
+int ensure_device();
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline size_t dtype_size(int dtype) { return dtype == QSM_F32 ? 4 : 8; }
+
+// Scoped device buffer used by the synchronous host-pointer entry points.
+struct DevBuf {
+    void *p = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    int alloc(size_t bytes);
+};
+
+// Correctly rounded square root for both storage types (the reference calls
+// sqrt on the double path and sqrtf on the single path).
+__device__ __forceinline__ double rsqrt_cr(double x) { return sqrt(x); }
+__device__ __forceinline__ float rsqrt_cr(float x) { return sqrtf(x); }
+
+}  // namespace qsm

@@ -1,0 +1,386 @@
+// fd.hip -- finite-difference operators of QSM.m (src/inversion/mex/gradfp*,
+// src/utils/fd/mex/grad{f,fm}{,_adj}_mex.c) and the wrapped-phase Laplacian
+// (src/unwrap/mex/lapw_mex.c) for gfx950.
+//
+// One thread per voxel, x fastest, so every wavefront reads and writes 64
+// consecutive voxels; the +/-1 neighbours come from the same or adjacent
+// cache lines.  These operators are HBM-bound elementwise stencils: 4 words
+// per voxel for the gradients (1 read, 3 written) and their adjoints (3 read,
+// 1 written), 2 for lapw.  The per-voxel formulas (including the order of the
+// three axis terms of the adjoints) are the reference's, built with
+// -ffp-contract=off, so results are bitwise the reference's.
+#include <cmath>
+
+#include "common.h"
+
+using namespace qsm;
+
+namespace {
+
+constexpr int FX = 64, FY = 4;
+
+struct G3 {
+    int nx, ny, nz;
+};
+
+__device__ __forceinline__ bool vox(G3 g, int &i, int &j, int &k, int &l)
+{
+    i = blockIdx.x * FX + threadIdx.x;
+    j = blockIdx.y * FY + threadIdx.y;
+    k = blockIdx.z;
+    l = i + g.nx * (j + g.ny * k);
+    return i < g.nx && j < g.ny;
+}
+
+// gradfp_mex.c:163-241: periodic forward differences.
+template <typename T>
+__global__ __launch_bounds__(FX *FY) void gradfp_kernel(T *dx, T *dy, T *dz,
+                                                        const T *u, G3 g,
+                                                        T hx, T hy, T hz)
+{
+    int i, j, k, l;
+    if (!vox(g, i, j, k, l)) return;
+    const int sy = g.nx, sk = g.nx * g.ny;
+    const int lx = (i + 1 < g.nx) ? l + 1 : l - (g.nx - 1);
+    const int ly = (j + 1 < g.ny) ? l + sy : l - sy * (g.ny - 1);
+    const int lz = (k + 1 < g.nz) ? l + sk : l - sk * (g.nz - 1);
+    const T c = u[l];
+    dx[l] = hx * (u[lx] - c);
+    dy[l] = hy * (u[ly] - c);
+    dz[l] = hz * (u[lz] - c);
+}
+
+// gradfp_adj_mex.c:174-254: -(periodic backward divergence), with the
+// reference's per-region term order: k == 0 -> (X+Y)+Z; k > 0, j == 0 ->
+// (Z+X)+Y; otherwise (Z+Y)+X.
+template <typename T>
+__global__ __launch_bounds__(FX *FY) void gradfp_adj_kernel(T *du, const T *x,
+                                                            const T *y, const T *z,
+                                                            G3 g, T hx, T hy, T hz)
+{
+    int i, j, k, l;
+    if (!vox(g, i, j, k, l)) return;
+    const int sy = g.nx, sk = g.nx * g.ny;
+    const int lx = (i > 0) ? l - 1 : l + (g.nx - 1);
+    const int ly = (j > 0) ? l - sy : l + sy * (g.ny - 1);
+    const int lz = (k > 0) ? l - sk : l + sk * (g.nz - 1);
+    const T X = hx * (x[l] - x[lx]);
+    const T Y = hy * (y[l] - y[ly]);
+    const T Z = hz * (z[l] - z[lz]);
+    du[l] = (k == 0) ? (X + Y) + Z : (j == 0) ? (Z + X) + Y : (Z + Y) + X;
+}
+
+// gradf_mex.c:167-243: forward differences, backward on the last face.
+template <typename T>
+__global__ __launch_bounds__(FX *FY) void gradf_kernel(T *dx, T *dy, T *dz,
+                                                       const T *u, G3 g,
+                                                       T hx, T hy, T hz)
+{
+    int i, j, k, l;
+    if (!vox(g, i, j, k, l)) return;
+    const int sy = g.nx, sk = g.nx * g.ny;
+    const T c = u[l];
+    dx[l] = (i + 1 < g.nx) ? hx * (u[l + 1] - c) : hx * (c - u[l - 1]);
+    dy[l] = (j + 1 < g.ny) ? hy * (u[l + sy] - c) : hy * (c - u[l - sy]);
+    dz[l] = (k + 1 < g.nz) ? hz * (u[l + sk] - c) : hz * (c - u[l - sk]);
+}
+
+// gradfm_mex.c:123-150: inside G, forward if the +1 neighbour is in G, else
+// backward if the -1 neighbour is, else 0; outside G untouched.
+template <typename T>
+__global__ __launch_bounds__(FX *FY) void gradfm_kernel(T *dx, T *dy, T *dz,
+                                                        const T *u,
+                                                        const uint8_t *G, G3 g,
+                                                        T hx, T hy, T hz)
+{
+    int i, j, k, l;
+    if (!vox(g, i, j, k, l)) return;
+    if (!G[l]) return;
+    const int sy = g.nx, sk = g.nx * g.ny;
+    const T c = u[l];
+    dx[l] = (i + 1 < g.nx && G[l + 1]) ? hx * (u[l + 1] - c)
+          : (i > 0 && G[l - 1]) ? hx * (c - u[l - 1]) : T(0);
+    dy[l] = (j + 1 < g.ny && G[l + sy]) ? hy * (u[l + sy] - c)
+          : (j > 0 && G[l - sy]) ? hy * (c - u[l - sy]) : T(0);
+    dz[l] = (k + 1 < g.nz && G[l + sk]) ? hz * (u[l + sk] - c)
+          : (k > 0 && G[l - sk]) ? hz * (c - u[l - sk]) : T(0);
+}
+
+// gradf_adj_mex.c:168-245: (X+Y)+Z, backward differences scaled by -1/h,
+// forward on the first face.
+template <typename T>
+__global__ __launch_bounds__(FX *FY) void gradf_adj_kernel(T *du, const T *x,
+                                                           const T *y, const T *z,
+                                                           G3 g, T hx, T hy, T hz)
+{
+    int i, j, k, l;
+    if (!vox(g, i, j, k, l)) return;
+    const int sy = g.nx, sk = g.nx * g.ny;
+    const T X = (i > 0) ? hx * (x[l] - x[l - 1]) : hx * (x[l + 1] - x[l]);
+    const T Y = (j > 0) ? hy * (y[l] - y[l - sy]) : hy * (y[l + sy] - y[l]);
+    const T Z = (k > 0) ? hz * (z[l] - z[l - sk]) : hz * (z[l + sk] - z[l]);
+    du[l] = (X + Y) + Z;
+}
+
+// gradfm_adj_mex.c:131-157: inside G, backward if the -1 neighbour is in G,
+// else forward if the +1 neighbour is, else 0; du = (dx+dy)+dz.
+template <typename T>
+__global__ __launch_bounds__(FX *FY) void gradfm_adj_kernel(T *du, const T *x,
+                                                            const T *y, const T *z,
+                                                            const uint8_t *G, G3 g,
+                                                            T hx, T hy, T hz)
+{
+    int i, j, k, l;
+    if (!vox(g, i, j, k, l)) return;
+    if (!G[l]) return;
+    const int sy = g.nx, sk = g.nx * g.ny;
+    const T DZ = (k > 0 && G[l - sk]) ? hz * (z[l] - z[l - sk])
+               : (k + 1 < g.nz && G[l + sk]) ? hz * (z[l + sk] - z[l]) : T(0);
+    const T DY = (j > 0 && G[l - sy]) ? hy * (y[l] - y[l - sy])
+               : (j + 1 < g.ny && G[l + sy]) ? hy * (y[l + sy] - y[l]) : T(0);
+    const T DX = (i > 0 && G[l - 1]) ? hx * (x[l] - x[l - 1])
+               : (i + 1 < g.nx && G[l + 1]) ? hx * (x[l + 1] - x[l]) : T(0);
+    du[l] = (DX + DY) + DZ;
+}
+
+__device__ __forceinline__ double fmod_t(double a, double b) { return fmod(a, b); }
+__device__ __forceinline__ float fmod_t(float a, float b) { return fmodf(a, b); }
+
+// lapw_mex.c:170-187: wrap to [-pi, pi] by fmod (exact in both precisions).
+template <typename T>
+__device__ __forceinline__ T wraptopi(T x)
+{
+    const T pi = T(3.14159265358979323846);
+    const T tau = T(2 * 3.14159265358979323846);
+    x += pi;
+    return x < T(0) ? fmod_t(x, tau) + pi : fmod_t(x, tau) - pi;
+}
+
+// lapw_mex.c:133-164: interior voxels only.
+template <typename T>
+__global__ __launch_bounds__(FX *FY) void lapw_kernel(T *du, const T *u, G3 g,
+                                                      T hx, T hy, T hz)
+{
+    int i, j, k, l;
+    if (!vox(g, i, j, k, l)) return;
+    if (i < 1 || i + 1 >= g.nx || j < 1 || j + 1 >= g.ny || k < 1 || k + 1 >= g.nz)
+        return;
+    const T pi = T(3.14159265358979323846);
+    const int sy = g.nx, sk = g.nx * g.ny;
+    const T c = u[l];
+    T ub = c - u[l - 1], uf = u[l + 1] - c;
+    ub = (ub < -pi || ub > pi) ? wraptopi(ub) : ub;
+    uf = (uf < -pi || uf > pi) ? wraptopi(uf) : uf;
+    T acc = hx * (uf - ub);
+    ub = c - u[l - sy];
+    uf = u[l + sy] - c;
+    ub = (ub < -pi || ub > pi) ? wraptopi(ub) : ub;
+    uf = (uf < -pi || uf > pi) ? wraptopi(uf) : uf;
+    acc = acc + hy * (uf - ub);
+    ub = c - u[l - sk];
+    uf = u[l + sk] - c;
+    ub = (ub < -pi || ub > pi) ? wraptopi(ub) : ub;
+    uf = (uf < -pi || uf > pi) ? wraptopi(uf) : uf;
+    du[l] = acc + hz * (uf - ub);
+}
+
+dim3 fd_grid(G3 g) { return dim3((g.nx + FX - 1) / FX, (g.ny + FY - 1) / FY, g.nz); }
+
+enum Op { GRADFP, GRADFP_ADJ, GRADF, GRADFM, GRADF_ADJ, GRADFM_ADJ, LAPW };
+
+// Coefficients as the reference forms them: 1/h (gradients), -1/h (adjoints),
+// 1/h^2 (lapw), in double and cast on the single path.
+template <typename T>
+void coefs(Op op, const double h[3], T c[3])
+{
+    for (int a = 0; a < 3; ++a) {
+        if (op == LAPW) c[a] = T(1.0 / (h[a] * h[a]));
+        else if (op == GRADFP_ADJ || op == GRADF_ADJ || op == GRADFM_ADJ) c[a] = T(-1.0 / h[a]);
+        else c[a] = T(1.0 / h[a]);
+    }
+}
+
+// Launches op on device arrays: a3 = the three vector arrays (outputs of the
+// gradients, inputs of the adjoints), s = the scalar array.
+template <typename T>
+hipError_t launch_op(Op op, T *s, T *a0, T *a1, T *a2, const uint8_t *G,
+                     G3 g, const double h[3], hipStream_t st)
+{
+    T c[3];
+    coefs<T>(op, h, c);
+    const dim3 grid = fd_grid(g), blk(FX, FY);
+    switch (op) {
+    case GRADFP: gradfp_kernel<T><<<grid, blk, 0, st>>>(a0, a1, a2, s, g, c[0], c[1], c[2]); break;
+    case GRADFP_ADJ: gradfp_adj_kernel<T><<<grid, blk, 0, st>>>(s, a0, a1, a2, g, c[0], c[1], c[2]); break;
+    case GRADF: gradf_kernel<T><<<grid, blk, 0, st>>>(a0, a1, a2, s, g, c[0], c[1], c[2]); break;
+    case GRADFM: gradfm_kernel<T><<<grid, blk, 0, st>>>(a0, a1, a2, s, G, g, c[0], c[1], c[2]); break;
+    case GRADF_ADJ: gradf_adj_kernel<T><<<grid, blk, 0, st>>>(s, a0, a1, a2, g, c[0], c[1], c[2]); break;
+    case GRADFM_ADJ: gradfm_adj_kernel<T><<<grid, blk, 0, st>>>(s, a0, a1, a2, G, g, c[0], c[1], c[2]); break;
+    case LAPW: lapw_kernel<T><<<grid, blk, 0, st>>>(a0, s, g, c[0], c[1], c[2]); break;
+    }
+    return hipGetLastError();
+}
+
+size_t min_dim(Op op) { return (op == GRADF || op == GRADF_ADJ) ? 2 : 1; }
+
+int run_dev(Op op, int dtype, void *s, void *a0, void *a1, void *a2,
+            const uint8_t *G, const size_t sz[3], const double h[3], void *stream)
+{
+    int rc = check_grid(sz, h, min_dim(op));
+    if (rc) return rc;
+    QSM_ARG(dtype == QSM_F64 || dtype == QSM_F32, "bad dtype");
+    QSM_ARG(s && a0 && (op == LAPW || (a1 && a2)), "null array pointer");
+    QSM_ARG(!(op == GRADFM || op == GRADFM_ADJ) || G, "null mask pointer");
+    const G3 g{int(sz[0]), int(sz[1]), int(sz[2])};
+    if (dtype == QSM_F64)
+        QSM_HIP(launch_op<double>(op, (double *)s, (double *)a0, (double *)a1, (double *)a2, G, g, h, as_stream(stream)));
+    else
+        QSM_HIP(launch_op<float>(op, (float *)s, (float *)a0, (float *)a1, (float *)a2, G, g, h, as_stream(stream)));
+    return QSM_OK;
+}
+
+// Host-pointer form: stage to the device, run, copy the outputs back.
+// in_s/in_v: whether the scalar / vector arrays are inputs; outputs that the
+// operator leaves partly untouched (masked variants, lapw's faces) are staged
+// in too so the untouched voxels keep the caller's values.
+int run_host(Op op, int dtype, void *s, void *a0, void *a1, void *a2,
+             const uint8_t *G, const size_t sz[3], const double h[3])
+{
+    int rc = check_grid(sz, h, min_dim(op));
+    if (rc) return rc;
+    QSM_ARG(dtype == QSM_F64 || dtype == QSM_F32, "bad dtype");
+    const bool lap = op == LAPW;
+    QSM_ARG(s && a0 && (lap || (a1 && a2)), "null array pointer");
+    const bool masked = op == GRADFM || op == GRADFM_ADJ;
+    QSM_ARG(!masked || G, "null mask pointer");
+    if ((rc = ensure_device())) return rc;
+    const size_t N = sz[0] * sz[1] * sz[2], bytes = N * dtype_size(dtype);
+    const bool vec_out = op == GRADFP || op == GRADF || op == GRADFM;
+    const int nv = lap ? 1 : 3;
+    DevBuf ds, dv[3], dg;
+    if ((rc = ds.alloc(bytes))) return rc;
+    for (int v = 0; v < nv; ++v)
+        if ((rc = dv[v].alloc(bytes))) return rc;
+    void *hv[3] = {a0, a1, a2};
+    // scalar array: input of gradients; output of adjoints (staged when masked)
+    if (vec_out || masked || lap)
+        QSM_HIP(hipMemcpy(ds.p, s, bytes, hipMemcpyHostToDevice));
+    for (int v = 0; v < nv; ++v)
+        if (!vec_out || masked || lap)
+            QSM_HIP(hipMemcpy(dv[v].p, hv[v], bytes, hipMemcpyHostToDevice));
+    if (masked) {
+        if ((rc = dg.alloc(N))) return rc;
+        QSM_HIP(hipMemcpy(dg.p, G, N, hipMemcpyHostToDevice));
+    }
+    if ((rc = run_dev(op, dtype, ds.p, dv[0].p, lap ? dv[0].p : dv[1].p,
+                      lap ? dv[0].p : dv[2].p, (const uint8_t *)dg.p, sz, h, nullptr)))
+        return rc;
+    if (vec_out || lap) {
+        for (int v = 0; v < nv; ++v)
+            QSM_HIP(hipMemcpy(hv[v], dv[v].p, bytes, hipMemcpyDeviceToHost));
+    } else {
+        QSM_HIP(hipMemcpy(s, ds.p, bytes, hipMemcpyDeviceToHost));
+    }
+    return QSM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qsm_gradfp(int dtype, void *dx, void *dy, void *dz, const void *u,
+               const size_t sz[3], const double h[3])
+{
+    return run_host(GRADFP, dtype, const_cast<void *>(u), dx, dy, dz, nullptr, sz, h);
+}
+
+int qsm_gradfp_adj(int dtype, void *du, const void *x, const void *y,
+                   const void *z, const size_t sz[3], const double h[3])
+{
+    return run_host(GRADFP_ADJ, dtype, du, const_cast<void *>(x), const_cast<void *>(y),
+                    const_cast<void *>(z), nullptr, sz, h);
+}
+
+int qsm_gradf(int dtype, void *dx, void *dy, void *dz, const void *u,
+              const size_t sz[3], const double h[3])
+{
+    return run_host(GRADF, dtype, const_cast<void *>(u), dx, dy, dz, nullptr, sz, h);
+}
+
+int qsm_gradfm(int dtype, void *dx, void *dy, void *dz, const void *u,
+               const uint8_t *G, const size_t sz[3], const double h[3])
+{
+    return run_host(GRADFM, dtype, const_cast<void *>(u), dx, dy, dz, G, sz, h);
+}
+
+int qsm_gradf_adj(int dtype, void *du, const void *x, const void *y,
+                  const void *z, const size_t sz[3], const double h[3])
+{
+    return run_host(GRADF_ADJ, dtype, du, const_cast<void *>(x), const_cast<void *>(y),
+                    const_cast<void *>(z), nullptr, sz, h);
+}
+
+int qsm_gradfm_adj(int dtype, void *du, const void *x, const void *y,
+                   const void *z, const uint8_t *G, const size_t sz[3],
+                   const double h[3])
+{
+    return run_host(GRADFM_ADJ, dtype, du, const_cast<void *>(x), const_cast<void *>(y),
+                    const_cast<void *>(z), G, sz, h);
+}
+
+int qsm_lapw(int dtype, void *d2u, const void *u, const size_t sz[3],
+             const double h[3])
+{
+    return run_host(LAPW, dtype, const_cast<void *>(u), d2u, nullptr, nullptr, nullptr, sz, h);
+}
+
+int qsm_gradfp_dev(int dtype, void *dx, void *dy, void *dz, const void *u,
+                   const size_t sz[3], const double h[3], void *stream)
+{
+    return run_dev(GRADFP, dtype, const_cast<void *>(u), dx, dy, dz, nullptr, sz, h, stream);
+}
+
+int qsm_gradfp_adj_dev(int dtype, void *du, const void *x, const void *y,
+                       const void *z, const size_t sz[3], const double h[3],
+                       void *stream)
+{
+    return run_dev(GRADFP_ADJ, dtype, du, const_cast<void *>(x), const_cast<void *>(y),
+                   const_cast<void *>(z), nullptr, sz, h, stream);
+}
+
+int qsm_gradf_dev(int dtype, void *dx, void *dy, void *dz, const void *u,
+                  const size_t sz[3], const double h[3], void *stream)
+{
+    return run_dev(GRADF, dtype, const_cast<void *>(u), dx, dy, dz, nullptr, sz, h, stream);
+}
+
+int qsm_gradfm_dev(int dtype, void *dx, void *dy, void *dz, const void *u,
+                   const uint8_t *G, const size_t sz[3], const double h[3],
+                   void *stream)
+{
+    return run_dev(GRADFM, dtype, const_cast<void *>(u), dx, dy, dz, G, sz, h, stream);
+}
+
+int qsm_gradf_adj_dev(int dtype, void *du, const void *x, const void *y,
+                      const void *z, const size_t sz[3], const double h[3],
+                      void *stream)
+{
+    return run_dev(GRADF_ADJ, dtype, du, const_cast<void *>(x), const_cast<void *>(y),
+                   const_cast<void *>(z), nullptr, sz, h, stream);
+}
+
+int qsm_gradfm_adj_dev(int dtype, void *du, const void *x, const void *y,
+                       const void *z, const uint8_t *G, const size_t sz[3],
+                       const double h[3], void *stream)
+{
+    return run_dev(GRADFM_ADJ, dtype, du, const_cast<void *>(x), const_cast<void *>(y),
+                   const_cast<void *>(z), G, sz, h, stream);
+}
+
+int qsm_lapw_dev(int dtype, void *d2u, const void *u, const size_t sz[3],
+                 const double h[3], void *stream)
+{
+    return run_dev(LAPW, dtype, const_cast<void *>(u), d2u, nullptr, nullptr, nullptr, sz, h, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,613 @@
+// pd_api.hip -- C ABI of the primal-dual solver: resident handles, the
+// drop-in whole-solve entry points and the single-sweep device entry points.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "pd_kernels.h"
+
+using namespace qsm;
+
+namespace {
+
+constexpr int NB_TGV = 15;  // chi, eta, w[3], nu, p[3], q[6]
+constexpr int NB_TV = 6;    // chi, eta, nu, p[3]
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+struct Layout {
+    size_t bank_bytes, f_off, mask_off, ctl_off, part_off, hist_off, total;
+};
+
+Layout layout(int kind, int dtype, const size_t sz[3], int npart, int hist_cap)
+{
+    const size_t N = sz[0] * sz[1] * sz[2];
+    const size_t s = dtype_size(dtype);
+    const int nb = kind == QSM_TGV ? NB_TGV : NB_TV;
+    Layout L;
+    L.bank_bytes = align_up(size_t(nb) * N * s, 256);
+    L.f_off = 2 * L.bank_bytes;
+    L.mask_off = L.f_off + align_up(N * s, 256);
+    L.ctl_off = L.mask_off + align_up(N, 256);
+    L.part_off = L.ctl_off + 256;
+    L.hist_off = L.part_off + align_up(size_t(npart) * 4 * sizeof(double), 256);
+    L.total = L.hist_off + align_up(size_t(hist_cap) * 2 * sizeof(double), 256);
+    return L;
+}
+
+// Default fused tile shape per precision/model (tuned on MI355X; the
+// batching runtime's autotuner may pick others).
+FusedShape default_shape(int kind, int dtype)
+{
+    if (dtype == QSM_F64 && kind == QSM_TGV)
+        return FusedShape{64, 8, 16};
+    return FusedShape{64, 8, 16};
+}
+
+}  // namespace
+
+struct qsm_pd {
+    int kind, dtype, schedule;
+    size_t sz[3], N;
+    double h[3], lam, alpha, tau, sigma;
+    hipStream_t stream;
+    char *mem;
+    Layout L;
+    int npart, hist_cap;
+    uint32_t n;       // iterations enqueued since the last load
+    int dual_ahead;   // fused schedule: dual of iteration n+1 applied
+    int loaded;
+    FusedShape shape;
+
+    void *bank(int b) const { return mem + size_t(b) * L.bank_bytes; }
+    void *f() const { return mem + L.f_off; }
+    uint8_t *mask() const { return reinterpret_cast<uint8_t *>(mem + L.mask_off); }
+    Ctl *ctl() const { return reinterpret_cast<Ctl *>(mem + L.ctl_off); }
+    double *partials() const { return reinterpret_cast<double *>(mem + L.part_off); }
+    double *hist() const { return reinterpret_cast<double *>(mem + L.hist_off); }
+    Geom geom() const { return Geom{int(sz[0]), int(sz[1]), int(sz[2])}; }
+};
+
+namespace {
+
+template <typename T>
+Bank<T> make_bank(void *base, int kind, size_t N)
+{
+    T *a = static_cast<T *>(base);
+    Bank<T> b{};
+    if (kind == QSM_TGV) {
+        b.chi = a; b.eta = a + N;
+        b.w1 = a + 2 * N; b.w2 = a + 3 * N; b.w3 = a + 4 * N;
+        b.nu = a + 5 * N;
+        b.p1 = a + 6 * N; b.p2 = a + 7 * N; b.p3 = a + 8 * N;
+        b.q1 = a + 9 * N; b.q2 = a + 10 * N; b.q3 = a + 11 * N;
+        b.q4 = a + 12 * N; b.q5 = a + 13 * N; b.q6 = a + 14 * N;
+    } else {
+        b.chi = a; b.eta = a + N;
+        b.nu = a + 2 * N;
+        b.p1 = a + 3 * N; b.p2 = a + 4 * N; b.p3 = a + 5 * N;
+    }
+    return b;
+}
+
+// The two-sweep schedule keeps its extrapolated arrays in bank 1.
+template <typename T>
+Bars<T> bars_of(const Bank<T> &b1)
+{
+    return Bars<T>{b1.chi, b1.eta, b1.w1, b1.w2, b1.w3};
+}
+
+template <typename T>
+int iterate_impl(qsm_pd *s, uint32_t niter, double tol)
+{
+    const Geom g = s->geom();
+    const DualCoef<T> dc = make_dual_coef<T>(
+        s->h, s->sigma, s->kind == QSM_TGV ? s->alpha * s->lam : s->lam, s->lam);
+    const PrimalCoef<T> pc = make_primal_coef<T>(s->h, s->tau);
+    const T *f = static_cast<const T *>(s->f());
+    const Bank<T> b0 = make_bank<T>(s->bank(0), s->kind, s->N);
+    const Bank<T> b1 = make_bank<T>(s->bank(1), s->kind, s->N);
+    const int *stop = &s->ctl()->stop;
+    const bool tgv = s->kind == QSM_TGV;
+    QSM_ARG(s->schedule != 0 || fused_blocks(g, s->shape) <= s->npart,
+            "fused tile shape needs %d partial-sum slots, %d allocated",
+            fused_blocks(g, s->shape), s->npart);
+
+    for (uint32_t t = 0; t < niter; ++t) {
+        const uint32_t n1 = s->n + 1;
+        const bool check = (n1 % 10) == 0;  // pd_tgv_mex.c:298,309
+        double *part = check ? s->partials() : nullptr;
+        int nblk = 0;
+        if (s->schedule == 0) {
+            if (!s->dual_ahead) {
+                // Dual of iteration 1 from the zero primal iterates, which
+                // bank 1 holds until the first fused pass writes it.
+                const hipError_t e = tgv
+                    ? launch_dual_sweep<T, true>(b0, bars_of(b1), f, s->mask(), 2, g, dc, stop, s->stream)
+                    : launch_dual_sweep<T, false>(b0, bars_of(b1), f, s->mask(), 2, g, dc, stop, s->stream);
+                QSM_HIP(e);
+                s->dual_ahead = 1;
+            }
+            const Bank<T> &in = (n1 & 1) ? b0 : b1;
+            const Bank<T> &out = (n1 & 1) ? b1 : b0;
+            const hipError_t e = tgv
+                ? launch_fused<T, true>(in, out, f, s->mask(), g, s->shape, dc, pc, stop, part, s->stream)
+                : launch_fused<T, false>(in, out, f, s->mask(), g, s->shape, dc, pc, stop, part, s->stream);
+            QSM_HIP(e);
+            nblk = fused_blocks(g, s->shape);
+        } else {
+            hipError_t e = tgv
+                ? launch_dual_sweep<T, true>(b0, bars_of(b1), f, s->mask(), 2, g, dc, stop, s->stream)
+                : launch_dual_sweep<T, false>(b0, bars_of(b1), f, s->mask(), 2, g, dc, stop, s->stream);
+            QSM_HIP(e);
+            e = tgv
+                ? launch_primal_sweep<T, true>(b0, bars_of(b1), s->mask(), 1, g, pc, stop, part, s->stream)
+                : launch_primal_sweep<T, false>(b0, bars_of(b1), s->mask(), 1, g, pc, stop, part, s->stream);
+            QSM_HIP(e);
+            nblk = primal_sweep_blocks(g);
+        }
+        if (check) {
+            // An empty interior gives nblk == 0: all sums zero and 0/0 norms,
+            // as the reference computes them.
+            QSM_HIP(launch_conv_finalize(s->partials(), nblk, s->ctl(), s->hist(),
+                                         s->hist_cap, tol, n1, s->stream));
+        }
+        s->n = n1;
+    }
+    return QSM_OK;
+}
+
+template <typename T>
+int get_chi_impl(qsm_pd *s, void *chi, int on_device)
+{
+    const Bank<T> b0 = make_bank<T>(s->bank(0), s->kind, s->N);
+    const Bank<T> b1 = make_bank<T>(s->bank(1), s->kind, s->N);
+    if (on_device) {
+        if (s->schedule == 0) {
+            QSM_HIP(launch_select_copy<T>(static_cast<T *>(chi), b0.chi, b1.chi,
+                                          s->ctl(), s->n, s->N, s->stream));
+        } else {
+            QSM_HIP(hipMemcpyAsync(chi, b0.chi, s->N * sizeof(T),
+                                   hipMemcpyDeviceToDevice, s->stream));
+        }
+        return QSM_OK;
+    }
+    const T *src = b0.chi;
+    if (s->schedule == 0) {
+        Ctl c;
+        QSM_HIP(hipMemcpyAsync(&c, s->ctl(), sizeof c, hipMemcpyDeviceToHost, s->stream));
+        QSM_HIP(hipStreamSynchronize(s->stream));
+        const uint32_t it = c.stop ? c.stop_iter : s->n;
+        src = (it & 1u) ? b1.chi : b0.chi;
+    }
+    QSM_HIP(hipMemcpyAsync(chi, src, s->N * sizeof(T), hipMemcpyDeviceToHost,
+                           s->stream));
+    QSM_HIP(hipStreamSynchronize(s->stream));
+    return QSM_OK;
+}
+
+int create_impl(qsm_pd **out, int kind, int dtype, const size_t sz[3],
+                const double h[3], double lam, double alpha, void *stream,
+                int hist_cap)
+{
+    QSM_ARG(out != nullptr, "null handle pointer");
+    *out = nullptr;
+    QSM_ARG(kind == QSM_TV || kind == QSM_TGV, "kind must be QSM_TV or QSM_TGV");
+    QSM_ARG(dtype == QSM_F64 || dtype == QSM_F32, "dtype must be QSM_F64 or QSM_F32");
+    int rc = check_grid(sz, h, 1);
+    if (rc) return rc;
+    if ((rc = ensure_device())) return rc;
+
+    qsm_pd *s = new (std::nothrow) qsm_pd();
+    QSM_ARG(s != nullptr, "host allocation failed");
+    s->kind = kind;
+    s->dtype = dtype;
+    s->schedule = 0;
+    for (int a = 0; a < 3; ++a) { s->sz[a] = sz[a]; s->h[a] = h[a]; }
+    s->N = sz[0] * sz[1] * sz[2];
+    s->lam = lam;
+    s->alpha = alpha;
+    s->stream = as_stream(stream);
+    s->shape = default_shape(kind, dtype);
+    qsm_step_sizes(h, &s->tau, &s->sigma);
+    const Geom g = s->geom();
+    s->npart = std::max({primal_sweep_blocks(g), fused_blocks(g, FusedShape{32, 8, 1}), 1});
+    s->hist_cap = hist_cap;
+    s->L = layout(kind, dtype, sz, s->npart, hist_cap);
+    const hipError_t e = hipMalloc(reinterpret_cast<void **>(&s->mem), s->L.total);
+    if (e != hipSuccess) {
+        delete s;
+        return hip_fail(e, "hipMalloc(solver state)", __FILE__, __LINE__);
+    }
+    *out = s;
+    return QSM_OK;
+}
+
+void print_line(qsm_print_fn print, void *ctx, const char *line)
+{
+    if (print)
+        print(line, ctx);
+    else
+        fputs(line, stdout);
+}
+
+// The reference's progress table (pd_tgv_mex.c:290-315), reproduced after
+// the solve from the device's norm history.
+void print_progress(qsm_pd *s, uint32_t maxiter, uint32_t verbose,
+                    qsm_print_fn print, void *ctx)
+{
+    if (!verbose)
+        return;
+    uint32_t iters = 0;
+    int stopped = 0;
+    double nr[2];
+    qsm_pd_status(s, &iters, &stopped, nr);
+    std::vector<double> hist(size_t(s->hist_cap) * 2);
+    const int nh = qsm_pd_norm_history(s, hist.data(), s->hist_cap);
+    const uint32_t vb = verbose > 4 ? verbose : 4;
+    const uint32_t pmod = maxiter / (vb < maxiter ? vb : maxiter);
+    char line[128];
+    print_line(print, ctx, "Iter\t\t||delta chi||\t||delta eta||\n");
+    double n1 = 1.0, n2 = 1.0;
+    for (uint32_t i = 0; i < iters; ++i) {
+        const uint32_t it = i + 1;
+        const bool flag = stopped && it == iters;
+        if (it % 10 == 0) {
+            const int c = int(it / 10) - 1;
+            if (c < nh) { n1 = hist[2 * c]; n2 = hist[2 * c + 1]; }
+        }
+        if (i == 0 || i == maxiter - 1 || (it % pmod) == 0 || flag) {
+            snprintf(line, sizeof line, "%5d/%d\t%.3e\t%.3e\n", it, maxiter, n1, n2);
+            print_line(print, ctx, line);
+        }
+    }
+}
+
+int pd_solve(int kind, int dtype, const void *d2f, const uint8_t *mask,
+             const size_t sz[3], const double h[3], double lam, double alpha,
+             double tol, uint32_t maxiter, uint32_t verbose, void *chi,
+             qsm_print_fn print, void *print_ctx)
+{
+    QSM_ARG(d2f && mask && chi, "null input or output pointer");
+    QSM_ARG(maxiter > 0, "maxiter must be > 0");
+    qsm_pd *s = nullptr;
+    int rc = create_impl(&s, kind, dtype, sz, h, lam, alpha, nullptr,
+                         int(maxiter / 10) + 1);
+    if (rc) return rc;
+    if (!(rc = qsm_pd_load(s, d2f, mask, 0))
+        && !(rc = qsm_pd_iterate(s, maxiter, tol))
+        && !(rc = qsm_pd_get_chi(s, chi, 0)))
+        print_progress(s, maxiter, verbose, print, print_ctx);
+    qsm_pd_destroy(s);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+void qsm_step_sizes(const double h[3], double *tau, double *sigma)
+{
+    // pd_tgv_mex.c:272-278
+    const double gnrm = 4.0 * (1.0 / (h[0] * h[0]) + 1.0 / (h[1] * h[1])
+                               + 1.0 / (h[2] * h[2]));
+    const double wnrm = 6.0 * (1.0 / (h[0] * h[0]) + 1.0 / (h[1] * h[1])
+                               + 2.0 / (h[2] * h[2])) / 3.0;
+    const double K2 = gnrm * gnrm + wnrm * wnrm;
+    const double t = 1 / std::sqrt(K2);
+    *tau = t;
+    *sigma = 1 / (t * K2);
+}
+
+size_t qsm_pd_footprint(int kind, int dtype, const size_t sz[3])
+{
+    if (!sz) return 0;
+    const Geom g{int(sz[0]), int(sz[1]), int(sz[2])};
+    const int npart = std::max({primal_sweep_blocks(g), fused_blocks(g, FusedShape{32, 8, 1}), 1});
+    return layout(kind, dtype, sz, npart, 1024).total;
+}
+
+int qsm_pd_create(qsm_pd **out, int kind, int dtype, const size_t sz[3],
+                  const double h[3], double lam, double alpha, void *stream)
+{
+    return create_impl(out, kind, dtype, sz, h, lam, alpha, stream, 1024);
+}
+
+int qsm_pd_load(qsm_pd *s, const void *d2f, const uint8_t *mask, int on_device)
+{
+    QSM_ARG(s && d2f && mask, "null handle or input pointer");
+    const size_t es = dtype_size(s->dtype);
+    const hipMemcpyKind kd = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    uint8_t *raw = static_cast<uint8_t *>(s->bank(1));  // staging, cleared below
+    QSM_HIP(hipMemcpyAsync(raw, mask, s->N, kd, s->stream));
+    QSM_HIP(launch_mask_init(s->mask(), raw, s->geom(), 0, s->stream));
+    QSM_HIP(hipMemsetAsync(s->bank(0), 0, 2 * s->L.bank_bytes, s->stream));
+    QSM_HIP(hipMemcpyAsync(s->f(), d2f, s->N * es, kd, s->stream));
+    if (s->dtype == QSM_F64)
+        QSM_HIP(launch_scale<double>(static_cast<double *>(s->f()),
+                                     static_cast<const double *>(s->f()),
+                                     s->sigma, s->N, s->stream));
+    else
+        QSM_HIP(launch_scale<float>(static_cast<float *>(s->f()),
+                                    static_cast<const float *>(s->f()),
+                                    float(s->sigma), s->N, s->stream));
+    QSM_HIP(launch_ctl_reset(s->ctl(), 1, s->stream));
+    if (!on_device)
+        QSM_HIP(hipStreamSynchronize(s->stream));  // host buffers may go away
+    s->n = 0;
+    s->dual_ahead = 0;
+    s->loaded = 1;
+    return QSM_OK;
+}
+
+int qsm_pd_iterate(qsm_pd *s, uint32_t niter, double tol)
+{
+    QSM_ARG(s && s->loaded, "handle not loaded");
+    if (s->dtype == QSM_F64)
+        return iterate_impl<double>(s, niter, tol);
+    return iterate_impl<float>(s, niter, tol);
+}
+
+int qsm_pd_status(qsm_pd *s, uint32_t *iters, int *stopped, double nr[2])
+{
+    QSM_ARG(s, "null handle");
+    Ctl c;
+    QSM_HIP(hipMemcpyAsync(&c, s->ctl(), sizeof c, hipMemcpyDeviceToHost, s->stream));
+    QSM_HIP(hipStreamSynchronize(s->stream));
+    if (iters) *iters = c.stop ? c.stop_iter : s->n;
+    if (stopped) *stopped = c.stop;
+    if (nr) { nr[0] = c.nr[0]; nr[1] = c.nr[1]; }
+    return QSM_OK;
+}
+
+int qsm_pd_norm_history(qsm_pd *s, double *nr, int cap)
+{
+    if (!s || !nr || cap <= 0) return 0;
+    Ctl c;
+    if (hipMemcpyAsync(&c, s->ctl(), sizeof c, hipMemcpyDeviceToHost, s->stream) != hipSuccess
+        || hipStreamSynchronize(s->stream) != hipSuccess)
+        return 0;
+    int n = int(c.checks);
+    if (n > s->hist_cap) n = s->hist_cap;
+    if (n > cap) n = cap;
+    if (n > 0
+        && hipMemcpy(nr, s->hist(), size_t(n) * 2 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        return 0;
+    return n;
+}
+
+int qsm_pd_get_chi(qsm_pd *s, void *chi, int on_device)
+{
+    QSM_ARG(s && chi, "null handle or output pointer");
+    if (s->dtype == QSM_F64)
+        return get_chi_impl<double>(s, chi, on_device);
+    return get_chi_impl<float>(s, chi, on_device);
+}
+
+void qsm_pd_destroy(qsm_pd *s)
+{
+    if (!s) return;
+    if (s->mem) {
+        (void)hipStreamSynchronize(s->stream);
+        (void)hipFree(s->mem);
+    }
+    delete s;
+}
+
+int qsm_pd_set_schedule(qsm_pd *s, int schedule)
+{
+    QSM_ARG(s, "null handle");
+    QSM_ARG(schedule == 0 || schedule == 1, "schedule must be 0 or 1");
+    QSM_ARG(s->n == 0 && !s->dual_ahead, "schedule can only change before the first iteration");
+    s->schedule = schedule;
+    return QSM_OK;
+}
+
+int qsm_pd_tgv(int dtype, const void *d2f, const uint8_t *mask,
+               const size_t sz[3], const double h[3], double lam,
+               double alpha, double tol, uint32_t maxiter, uint32_t verbose,
+               void *chi, qsm_print_fn print, void *print_ctx)
+{
+    return pd_solve(QSM_TGV, dtype, d2f, mask, sz, h, lam, alpha, tol, maxiter,
+                    verbose, chi, print, print_ctx);
+}
+
+int qsm_pd_tv(int dtype, const void *d2f, const uint8_t *mask,
+              const size_t sz[3], const double h[3], double lam, double tol,
+              uint32_t maxiter, uint32_t verbose, void *chi,
+              qsm_print_fn print, void *print_ctx)
+{
+    return pd_solve(QSM_TV, dtype, d2f, mask, sz, h, lam, 0.0, tol, maxiter,
+                    verbose, chi, print, print_ctx);
+}
+
+// ---- single sweeps on caller device arrays ------------------------------
+
+int qsm_tgv_dual_dev(int dtype, void *nu, void *p, void *q, const void *chi_,
+                     const void *eta_, const void *w_, const void *f,
+                     const uint8_t *mi, const size_t sz[3], const double h[3],
+                     double alpha0, double alpha1, double sigma, void *stream)
+{
+    int rc = check_grid(sz, h, 1);
+    if (rc) return rc;
+    const size_t N = sz[0] * sz[1] * sz[2];
+    const Geom g{int(sz[0]), int(sz[1]), int(sz[2])};
+    if (dtype == QSM_F64) {
+        using T = double;
+        T *P = static_cast<T *>(p), *Q = static_cast<T *>(q);
+        const T *W = static_cast<const T *>(w_);
+        Bank<T> b{};
+        b.nu = static_cast<T *>(nu);
+        b.p1 = P; b.p2 = P + N; b.p3 = P + 2 * N;
+        b.q1 = Q; b.q2 = Q + N; b.q3 = Q + 2 * N; b.q4 = Q + 3 * N; b.q5 = Q + 4 * N; b.q6 = Q + 5 * N;
+        Bars<T> br{const_cast<T *>(static_cast<const T *>(chi_)),
+                   const_cast<T *>(static_cast<const T *>(eta_)),
+                   const_cast<T *>(W), const_cast<T *>(W + N), const_cast<T *>(W + 2 * N)};
+        QSM_HIP(launch_dual_sweep<T, true>(b, br, static_cast<const T *>(f), mi, 0xff, g,
+                                           make_dual_coef<T>(h, sigma, alpha0, alpha1),
+                                           nullptr, as_stream(stream)));
+    } else if (dtype == QSM_F32) {
+        using T = float;
+        T *P = static_cast<T *>(p), *Q = static_cast<T *>(q);
+        const T *W = static_cast<const T *>(w_);
+        Bank<T> b{};
+        b.nu = static_cast<T *>(nu);
+        b.p1 = P; b.p2 = P + N; b.p3 = P + 2 * N;
+        b.q1 = Q; b.q2 = Q + N; b.q3 = Q + 2 * N; b.q4 = Q + 3 * N; b.q5 = Q + 4 * N; b.q6 = Q + 5 * N;
+        Bars<T> br{const_cast<T *>(static_cast<const T *>(chi_)),
+                   const_cast<T *>(static_cast<const T *>(eta_)),
+                   const_cast<T *>(W), const_cast<T *>(W + N), const_cast<T *>(W + 2 * N)};
+        QSM_HIP(launch_dual_sweep<T, true>(b, br, static_cast<const T *>(f), mi, 0xff, g,
+                                           make_dual_coef<T>(h, sigma, alpha0, alpha1),
+                                           nullptr, as_stream(stream)));
+    } else {
+        QSM_ARG(false, "bad dtype");
+    }
+    return QSM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+int tgv_primal_dev_t(void *chi, void *eta, void *w, void *chi_, void *eta_,
+                     void *w_, const void *nu, const void *p, const void *q,
+                     const uint8_t *m, const size_t sz[3], const double h[3],
+                     double tau, void *stream, bool tgv)
+{
+    const size_t N = sz[0] * sz[1] * sz[2];
+    const Geom g{int(sz[0]), int(sz[1]), int(sz[2])};
+    T *P = const_cast<T *>(static_cast<const T *>(p));
+    Bank<T> b{};
+    b.chi = static_cast<T *>(chi);
+    b.eta = static_cast<T *>(eta);
+    b.nu = const_cast<T *>(static_cast<const T *>(nu));
+    b.p1 = P; b.p2 = P + N; b.p3 = P + 2 * N;
+    Bars<T> br{static_cast<T *>(chi_), static_cast<T *>(eta_), nullptr, nullptr, nullptr};
+    const PrimalCoef<T> c = make_primal_coef<T>(h, tau);
+    if (tgv) {
+        T *W = static_cast<T *>(w), *B = static_cast<T *>(w_);
+        T *Q = const_cast<T *>(static_cast<const T *>(q));
+        b.w1 = W; b.w2 = W + N; b.w3 = W + 2 * N;
+        b.q1 = Q; b.q2 = Q + N; b.q3 = Q + 2 * N; b.q4 = Q + 3 * N; b.q5 = Q + 4 * N; b.q6 = Q + 5 * N;
+        br.b1 = B; br.b2 = B + N; br.b3 = B + 2 * N;
+        QSM_HIP(launch_primal_sweep<T, true>(b, br, m, 0xff, g, c, nullptr, nullptr, as_stream(stream)));
+    } else {
+        QSM_HIP(launch_primal_sweep<T, false>(b, br, m, 0xff, g, c, nullptr, nullptr, as_stream(stream)));
+    }
+    return QSM_OK;
+}
+
+template <typename T>
+int tv_dual_dev_t(void *nu, void *p, const void *chi_, const void *eta_,
+                  const void *f, const uint8_t *mi, const size_t sz[3],
+                  const double h[3], double lam, double sigma, void *stream)
+{
+    const size_t N = sz[0] * sz[1] * sz[2];
+    const Geom g{int(sz[0]), int(sz[1]), int(sz[2])};
+    T *P = static_cast<T *>(p);
+    Bank<T> b{};
+    b.nu = static_cast<T *>(nu);
+    b.p1 = P; b.p2 = P + N; b.p3 = P + 2 * N;
+    Bars<T> br{const_cast<T *>(static_cast<const T *>(chi_)),
+               const_cast<T *>(static_cast<const T *>(eta_)), nullptr, nullptr, nullptr};
+    QSM_HIP(launch_dual_sweep<T, false>(b, br, static_cast<const T *>(f), mi, 0xff, g,
+                                        make_dual_coef<T>(h, sigma, lam, lam), nullptr,
+                                        as_stream(stream)));
+    return QSM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qsm_tgv_primal_dev(int dtype, void *chi, void *eta, void *w, void *chi_,
+                       void *eta_, void *w_, const void *nu, const void *p,
+                       const void *q, const uint8_t *m, const size_t sz[3],
+                       const double h[3], double tau, void *stream)
+{
+    int rc = check_grid(sz, h, 1);
+    if (rc) return rc;
+    QSM_ARG(dtype == QSM_F64 || dtype == QSM_F32, "bad dtype");
+    return dtype == QSM_F64
+        ? tgv_primal_dev_t<double>(chi, eta, w, chi_, eta_, w_, nu, p, q, m, sz, h, tau, stream, true)
+        : tgv_primal_dev_t<float>(chi, eta, w, chi_, eta_, w_, nu, p, q, m, sz, h, tau, stream, true);
+}
+
+int qsm_tv_primal_dev(int dtype, void *chi, void *eta, void *chi_, void *eta_,
+                      const void *nu, const void *p, const uint8_t *m,
+                      const size_t sz[3], const double h[3], double tau,
+                      void *stream)
+{
+    int rc = check_grid(sz, h, 1);
+    if (rc) return rc;
+    QSM_ARG(dtype == QSM_F64 || dtype == QSM_F32, "bad dtype");
+    return dtype == QSM_F64
+        ? tgv_primal_dev_t<double>(chi, eta, nullptr, chi_, eta_, nullptr, nu, p, nullptr, m, sz, h, tau, stream, false)
+        : tgv_primal_dev_t<float>(chi, eta, nullptr, chi_, eta_, nullptr, nu, p, nullptr, m, sz, h, tau, stream, false);
+}
+
+int qsm_tv_dual_dev(int dtype, void *nu, void *p, const void *chi_,
+                    const void *eta_, const void *f, const uint8_t *mi,
+                    const size_t sz[3], const double h[3], double lam,
+                    double sigma, void *stream)
+{
+    int rc = check_grid(sz, h, 1);
+    if (rc) return rc;
+    QSM_ARG(dtype == QSM_F64 || dtype == QSM_F32, "bad dtype");
+    return dtype == QSM_F64
+        ? tv_dual_dev_t<double>(nu, p, chi_, eta_, f, mi, sz, h, lam, sigma, stream)
+        : tv_dual_dev_t<float>(nu, p, chi_, eta_, f, mi, sz, h, lam, sigma, stream);
+}
+
+int qsm_init_mask_dev(uint8_t *mi, const uint8_t *m, const size_t sz[3],
+                      void *stream)
+{
+    const double h1[3] = {1.0, 1.0, 1.0};
+    int rc = check_grid(sz, h1, 1);
+    if (rc) return rc;
+    QSM_ARG(mi && m, "null mask pointer");
+    QSM_HIP(launch_mask_init(mi, m, Geom{int(sz[0]), int(sz[1]), int(sz[2])}, 1,
+                             as_stream(stream)));
+    return QSM_OK;
+}
+
+int qsm_convergence_dev(int dtype, const void *chi, const void *chi0,
+                        const void *eta, const void *eta0, size_t N,
+                        double tol, double out[2], int *flag, void *stream)
+{
+    QSM_ARG(chi && chi0 && eta && eta0 && out, "null pointer");
+    QSM_ARG(dtype == QSM_F64 || dtype == QSM_F32, "bad dtype");
+    const hipStream_t st = as_stream(stream);
+    const int nblk = 512;
+    DevBuf buf;
+    int rc = buf.alloc(256 + size_t(nblk) * 4 * sizeof(double));
+    if (rc) return rc;
+    Ctl *ctl = static_cast<Ctl *>(buf.p);
+    double *part = reinterpret_cast<double *>(static_cast<char *>(buf.p) + 256);
+    QSM_HIP(launch_ctl_reset(ctl, 1, st));
+    if (dtype == QSM_F64)
+        QSM_HIP(launch_conv_sums<double>(static_cast<const double *>(chi),
+                                         static_cast<const double *>(chi0),
+                                         static_cast<const double *>(eta),
+                                         static_cast<const double *>(eta0), N, part, nblk, st));
+    else
+        QSM_HIP(launch_conv_sums<float>(static_cast<const float *>(chi),
+                                        static_cast<const float *>(chi0),
+                                        static_cast<const float *>(eta),
+                                        static_cast<const float *>(eta0), N, part, nblk, st));
+    QSM_HIP(launch_conv_finalize(part, nblk, ctl, nullptr, 0, tol, 0, st));
+    Ctl c;
+    QSM_HIP(hipMemcpyAsync(&c, ctl, sizeof c, hipMemcpyDeviceToHost, st));
+    QSM_HIP(hipStreamSynchronize(st));
+    out[0] = c.nr[0];
+    out[1] = c.nr[1];
+    if (flag) *flag = c.stop;
+    return QSM_OK;
+}
+
+}  // extern "C"

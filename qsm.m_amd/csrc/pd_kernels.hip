@@ -1,0 +1,733 @@
+// pd_kernels.hip -- primal-dual TV/TGV kernels for gfx950.
+//
+// Built with -ffp-contract=off: every expression is evaluated exactly as
+// written (no fused multiply-add), which together with the association order
+// of pd_math.h reproduces the reference's CPU arithmetic bit for bit.
+#include "pd_kernels.h"
+
+namespace qsm {
+
+// ---------------------------------------------------------------------------
+// deterministic block reduction of four double sums
+// ---------------------------------------------------------------------------
+
+// Sums s[0..3] over the block in a fixed order (wavefront shuffles, then the
+// waves in index order) and writes the result from thread 0.
+template <int NT>
+__device__ __forceinline__ void block_sum4(double s[4], double *dst)
+{
+    static_assert(NT % 64 == 0, "block must be whole wavefronts");
+    __shared__ double red[NT / 64][4];
+    for (int off = 32; off > 0; off >>= 1)
+        for (int q = 0; q < 4; ++q)
+            s[q] += __shfl_down(s[q], off, 64);
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    if ((tid & 63) == 0)
+        for (int q = 0; q < 4; ++q)
+            red[tid >> 6][q] = s[q];
+    __syncthreads();
+    if (tid == 0) {
+        double t[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int w = 0; w < NT / 64; ++w)
+            for (int q = 0; q < 4; ++q)
+                t[q] += red[w][q];
+        for (int q = 0; q < 4; ++q)
+            dst[q] = t[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// two-sweep schedule (reference structure)
+// ---------------------------------------------------------------------------
+
+constexpr int SW_X = 64;
+constexpr int SW_Y = 4;
+
+template <typename T, bool TGV>
+__global__ __launch_bounds__(SW_X *SW_Y) void dual_sweep_kernel(
+    Bank<T> s, Bars<T> b, const T *__restrict__ f,
+    const uint8_t *__restrict__ mk, uint8_t bit, Geom g, DualCoef<T> c,
+    const int *stop)
+{
+    if (stop && *stop)
+        return;
+    const int i = blockIdx.x * SW_X + threadIdx.x + 1;
+    const int j = blockIdx.y * SW_Y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.nx - 2 || j > g.ny - 2)
+        return;
+    const int sy = g.nx, sk = g.nx * g.ny;
+    const int l = i + sy * j + sk * k;
+    if (!(mk[l] & bit))
+        return;
+
+    const Star<T> eb{b.eb[l], b.eb[l - 1], b.eb[l + 1], b.eb[l - sy],
+                     b.eb[l + sy], b.eb[l - sk], b.eb[l + sk]};
+    const Star<T> cb{b.cb[l], b.cb[l - 1], b.cb[l + 1], b.cb[l - sy],
+                     b.cb[l + sy], b.cb[l - sk], b.cb[l + sk]};
+    DualVox<T> d;
+    d.nu = dual_nu(c, s.nu[l], eb, cb, f[l]);
+    d.p1 = s.p1[l];
+    d.p2 = s.p2[l];
+    d.p3 = s.p3[l];
+    if constexpr (TGV) {
+        d.q1 = s.q1[l]; d.q2 = s.q2[l]; d.q3 = s.q3[l];
+        d.q4 = s.q4[l]; d.q5 = s.q5[l]; d.q6 = s.q6[l];
+        tgv_dual_pq(c, d, cb,
+                    b.b1[l], b.b1[l + 1], b.b1[l + sy], b.b1[l + sk],
+                    b.b2[l], b.b2[l + 1], b.b2[l + sy], b.b2[l + sk],
+                    b.b3[l], b.b3[l + 1], b.b3[l + sy], b.b3[l + sk]);
+        s.q1[l] = d.q1; s.q2[l] = d.q2; s.q3[l] = d.q3;
+        s.q4[l] = d.q4; s.q5[l] = d.q5; s.q6[l] = d.q6;
+    } else {
+        tv_dual_p(c, d, cb);
+    }
+    s.nu[l] = d.nu;
+    s.p1[l] = d.p1;
+    s.p2[l] = d.p2;
+    s.p3[l] = d.p3;
+}
+
+template <typename T, bool TGV, bool CHECK>
+__global__ __launch_bounds__(SW_X *SW_Y) void primal_sweep_kernel(
+    Bank<T> s, Bars<T> b, const uint8_t *__restrict__ mk, uint8_t bit,
+    Geom g, PrimalCoef<T> c, const int *stop, double *partials)
+{
+    if (stop && *stop)
+        return;
+    const int i = blockIdx.x * SW_X + threadIdx.x + 1;
+    const int j = blockIdx.y * SW_Y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    const int sy = g.nx, sk = g.nx * g.ny;
+    const int l = i + sy * j + sk * k;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+
+    bool act = (i <= g.nx - 2 && j <= g.ny - 2);
+    if (act)
+        act = (mk[l] & bit) != 0;
+    if (act) {
+        PrimalIn<T> in;
+        in.nu = Star<T>{s.nu[l], s.nu[l - 1], s.nu[l + 1], s.nu[l - sy],
+                        s.nu[l + sy], s.nu[l - sk], s.nu[l + sk]};
+        in.p1c = s.p1[l]; in.p1xm = s.p1[l - 1];
+        in.p2c = s.p2[l]; in.p2ym = s.p2[l - sy];
+        in.p3c = s.p3[l]; in.p3zm = s.p3[l - sk];
+        if constexpr (TGV) {
+            in.q1c = s.q1[l]; in.q1xm = s.q1[l - 1];
+            in.q2c = s.q2[l]; in.q2xm = s.q2[l - 1]; in.q2ym = s.q2[l - sy];
+            in.q3c = s.q3[l]; in.q3xm = s.q3[l - 1]; in.q3zm = s.q3[l - sk];
+            in.q4c = s.q4[l]; in.q4ym = s.q4[l - sy];
+            in.q5c = s.q5[l]; in.q5ym = s.q5[l - sy]; in.q5zm = s.q5[l - sk];
+            in.q6c = s.q6[l]; in.q6zm = s.q6[l - sk];
+        }
+        const T en = s.eta[l], cn = s.chi[l];
+        T e, x;
+        primal_eta_chi(c, in, en, cn, e, x);
+        s.eta[l] = e;
+        s.chi[l] = x;
+        b.eb[l] = extrap(e, en);
+        b.cb[l] = extrap(x, cn);
+        if constexpr (TGV) {
+            const T o1 = s.w1[l], o2 = s.w2[l], o3 = s.w3[l];
+            T n1 = o1, n2 = o2, n3 = o3;
+            primal_w(c, in, n1, n2, n3);
+            s.w1[l] = n1; s.w2[l] = n2; s.w3[l] = n3;
+            b.b1[l] = extrap(n1, o1);
+            b.b2[l] = extrap(n2, o2);
+            b.b3[l] = extrap(n3, o3);
+        }
+        if constexpr (CHECK)
+            conv_terms(x, cn, e, en, acc);
+    }
+    if constexpr (CHECK) {
+        const int blk = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+        block_sum4<SW_X * SW_Y>(acc, partials + 4 * blk);
+    }
+}
+
+static dim3 sweep_grid(Geom g)
+{
+    return dim3((g.nx - 2 + SW_X - 1) / SW_X, (g.ny - 2 + SW_Y - 1) / SW_Y,
+                g.nz - 2);
+}
+
+int primal_sweep_blocks(Geom g)
+{
+    if (g.nx < 3 || g.ny < 3 || g.nz < 3)
+        return 0;
+    const dim3 gr = sweep_grid(g);
+    return int(gr.x * gr.y * gr.z);
+}
+
+template <typename T, bool TGV>
+hipError_t launch_dual_sweep(const Bank<T> &s, const Bars<T> &b, const T *f,
+                             const uint8_t *mk, uint8_t bit, Geom g,
+                             const DualCoef<T> &c, const int *stop,
+                             hipStream_t st)
+{
+    if (g.nx < 3 || g.ny < 3 || g.nz < 3)
+        return hipSuccess;
+    dual_sweep_kernel<T, TGV><<<sweep_grid(g), dim3(SW_X, SW_Y), 0, st>>>(
+        s, b, f, mk, bit, g, c, stop);
+    return hipGetLastError();
+}
+
+template <typename T, bool TGV>
+hipError_t launch_primal_sweep(const Bank<T> &s, const Bars<T> &b,
+                               const uint8_t *mk, uint8_t bit, Geom g,
+                               const PrimalCoef<T> &c, const int *stop,
+                               double *partials, hipStream_t st)
+{
+    if (g.nx < 3 || g.ny < 3 || g.nz < 3)
+        return hipSuccess;
+    if (partials)
+        primal_sweep_kernel<T, TGV, true><<<sweep_grid(g), dim3(SW_X, SW_Y), 0, st>>>(
+            s, b, mk, bit, g, c, stop, partials);
+    else
+        primal_sweep_kernel<T, TGV, false><<<sweep_grid(g), dim3(SW_X, SW_Y), 0, st>>>(
+            s, b, mk, bit, g, c, stop, partials);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// fused schedule: primal of iteration n, then dual of iteration n+1
+// ---------------------------------------------------------------------------
+
+// z-queues of one (i, j) column while a block marches up a chunk of planes.
+template <typename T>
+struct ColQ {
+    int l0;             // i + nx*j, or -1 when the column is outside the grid
+    uint8_t mc;         // mask byte at the current plane kp
+    T nzm, nzc;         // nu at kp-1, kp
+    T p3m, q3m, q5m, q6m;  // z-backward neighbours (plane kp-1)
+};
+
+// Primal results at one voxel (zero where the voxel is not updated).
+template <typename T>
+struct POut {
+    T chi, eta, w1, w2, w3;   // new iterates
+    T cb, eb, b1, b2, b3;     // extrapolations
+    T chi_o, eta_o;           // old iterates (convergence terms)
+};
+
+// Primal update of column q at plane kp from bank `in`; also returns the
+// voxel's dual-side values (dv, zero where both mask bits are clear) and
+// advances the column's queues to plane kp+1.
+template <typename T, bool TGV>
+__device__ __forceinline__ void fused_primal_col(const Bank<T> &in,
+                                                 const uint8_t *__restrict__ mk,
+                                                 const PrimalCoef<T> &pc,
+                                                 int sy, int sk, int nz, int kp,
+                                                 ColQ<T> &q, DualVox<T> &dv,
+                                                 bool &pact, POut<T> &o)
+{
+    const uint8_t mc = q.mc;
+    uint8_t mn = 0;
+    T np = T(0);
+    dv = DualVox<T>{};
+    o = POut<T>{};
+    pact = false;
+    if (q.l0 >= 0) {
+        const int l = q.l0 + sk * kp;
+        if (kp + 1 < nz) {
+            mn = mk[l + sk];
+            if (mn & 2)
+                np = in.nu[l + sk];
+        }
+        if (mc & 3) {
+            dv.nu = q.nzc;
+            dv.p1 = in.p1[l];
+            dv.p2 = in.p2[l];
+            dv.p3 = in.p3[l];
+            if constexpr (TGV) {
+                dv.q1 = in.q1[l]; dv.q2 = in.q2[l]; dv.q3 = in.q3[l];
+                dv.q4 = in.q4[l]; dv.q5 = in.q5[l]; dv.q6 = in.q6[l];
+            }
+        }
+        pact = (mc & 1) != 0;
+        if (pact) {
+            PrimalIn<T> pi;
+            pi.nu = Star<T>{q.nzc, in.nu[l - 1], in.nu[l + 1], in.nu[l - sy],
+                            in.nu[l + sy], q.nzm, np};
+            pi.p1c = dv.p1; pi.p1xm = in.p1[l - 1];
+            pi.p2c = dv.p2; pi.p2ym = in.p2[l - sy];
+            pi.p3c = dv.p3; pi.p3zm = q.p3m;
+            if constexpr (TGV) {
+                pi.q1c = dv.q1; pi.q1xm = in.q1[l - 1];
+                pi.q2c = dv.q2; pi.q2xm = in.q2[l - 1]; pi.q2ym = in.q2[l - sy];
+                pi.q3c = dv.q3; pi.q3xm = in.q3[l - 1]; pi.q3zm = q.q3m;
+                pi.q4c = dv.q4; pi.q4ym = in.q4[l - sy];
+                pi.q5c = dv.q5; pi.q5ym = in.q5[l - sy]; pi.q5zm = q.q5m;
+                pi.q6c = dv.q6; pi.q6zm = q.q6m;
+            }
+            o.eta_o = in.eta[l];
+            o.chi_o = in.chi[l];
+            primal_eta_chi(pc, pi, o.eta_o, o.chi_o, o.eta, o.chi);
+            o.eb = extrap(o.eta, o.eta_o);
+            o.cb = extrap(o.chi, o.chi_o);
+            if constexpr (TGV) {
+                const T a1 = in.w1[l], a2 = in.w2[l], a3 = in.w3[l];
+                o.w1 = a1; o.w2 = a2; o.w3 = a3;
+                primal_w(pc, pi, o.w1, o.w2, o.w3);
+                o.b1 = extrap(o.w1, a1);
+                o.b2 = extrap(o.w2, a2);
+                o.b3 = extrap(o.w3, a3);
+            }
+        }
+    }
+    q.mc = mn;
+    q.nzm = q.nzc;
+    q.nzc = np;
+    q.p3m = dv.p3;
+    if constexpr (TGV) {
+        q.q3m = dv.q3;
+        q.q5m = dv.q5;
+        q.q6m = dv.q6;
+    }
+}
+
+// Loads the queues of a column for a march starting at primal plane kp0.
+template <typename T, bool TGV>
+__device__ __forceinline__ void fused_col_init(const Bank<T> &in,
+                                               const uint8_t *__restrict__ mk,
+                                               int sk, int kp0, ColQ<T> &q)
+{
+    q.mc = 0;
+    q.nzm = q.nzc = q.p3m = q.q3m = q.q5m = q.q6m = T(0);
+    if (q.l0 < 0)
+        return;
+    const int l = q.l0 + sk * kp0;
+    q.mc = mk[l];
+    if (q.mc & 2)
+        q.nzc = in.nu[l];
+    if (kp0 >= 1) {
+        const uint8_t mb = mk[l - sk];
+        if (mb & 2)
+            q.nzm = in.nu[l - sk];
+        if (mb & 3) {
+            q.p3m = in.p3[l - sk];
+            if constexpr (TGV) {
+                q.q3m = in.q3[l - sk];
+                q.q5m = in.q5[l - sk];
+                q.q6m = in.q6[l - sk];
+            }
+        }
+    }
+}
+
+// Ring cells (the x/y neighbours of the tile whose extrapolations the dual
+// needs): the -y and +y rows, then the -x and +x columns; no corners.
+template <int TX, int TY>
+__device__ __forceinline__ void ring_pos(int r, int &rx, int &ry)
+{
+    if (r < TX) { rx = r; ry = -1; }
+    else if (r < 2 * TX) { rx = r - TX; ry = TY; }
+    else if (r < 2 * TX + TY) { rx = -1; ry = r - 2 * TX; }
+    else { rx = TX; ry = r - 2 * TX - TY; }
+}
+
+template <typename T, bool TGV, int TX, int TY, bool CHECK>
+__global__ __launch_bounds__(TX *TY) void fused_kernel(
+    Bank<T> in, Bank<T> out, const T *__restrict__ f,
+    const uint8_t *__restrict__ mk, Geom g, int kz, int ntx, int nty,
+    int nchunk, DualCoef<T> dc, PrimalCoef<T> pc, const int *stop,
+    double *partials)
+{
+    constexpr int NT = TX * TY;
+    constexpr int NRING = 2 * TX + 2 * TY;
+    static_assert(NRING <= NT, "one ring cell per thread at most");
+    constexpr int LW = TX + 2;
+    constexpr int LP = (TX + 2) * (TY + 2);
+    constexpr int NB = TGV ? 5 : 2;
+    __shared__ T lds[2][NB][LP];
+
+    if (stop && *stop)
+        return;
+
+    // XCD-aware order: hardware block b runs on XCD b % 8; give each XCD a
+    // contiguous run of logical tiles so ring neighbours share its L2.
+    const int nblk = ntx * nty * nchunk;
+    const int per = nblk / 8, rem = nblk % 8;
+    const int xcd = int(blockIdx.x) % 8, idx = int(blockIdx.x) / 8;
+    const int lb = (xcd < rem) ? xcd * (per + 1) + idx
+                               : rem * (per + 1) + (xcd - rem) * per + idx;
+    const int bx = lb % ntx;
+    const int by = (lb / ntx) % nty;
+    const int bz = lb / (ntx * nty);
+
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int sy = nx, sk = nx * ny;
+    const int t = threadIdx.x;
+    const int tx = t % TX, ty = t / TX;
+    const int i0 = bx * TX, j0 = by * TY;
+    const int k0 = 1 + bz * kz;
+    const int k1 = min(k0 + kz, nz - 1);  // owned planes [k0, k1)
+
+    // own column
+    const int oi = i0 + tx, oj = j0 + ty;
+    ColQ<T> qo;
+    qo.l0 = (oi < nx && oj < ny) ? oi + sy * oj : -1;
+    // ring column
+    ColQ<T> qr;
+    int rx = 0, ry = 0;
+    qr.l0 = -1;
+    if (t < NRING) {
+        ring_pos<TX, TY>(t, rx, ry);
+        const int ri = i0 + rx, rj = j0 + ry;
+        if (ri >= 0 && ri < nx && rj >= 0 && rj < ny)
+            qr.l0 = ri + sy * rj;
+    }
+    const int lo = (ty + 1) * LW + (tx + 1);
+    const int lr = (ry + 1) * LW + (rx + 1);
+
+    fused_col_init<T, TGV>(in, mk, sk, k0 - 1, qo);
+    if (t < NRING)
+        fused_col_init<T, TGV>(in, mk, sk, k0 - 1, qr);
+
+    // own-column extrapolation queues and the dual-side values one plane back
+    T cbm = T(0), cbc = T(0), ebm = T(0), ebc = T(0);
+    T wc1 = T(0), wc2 = T(0), wc3 = T(0);
+    DualVox<T> dprev{};
+    uint8_t mprev = 0;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+
+    for (int kp = k0 - 1; kp <= k1; ++kp) {
+        const int slot = kp & 1;
+        // ---- primal at plane kp: own column and (some threads) a ring cell
+        const uint8_t mcur = qo.mc;
+        DualVox<T> dv;
+        bool pact;
+        POut<T> o;
+        fused_primal_col<T, TGV>(in, mk, pc, sy, sk, nz, kp, qo, dv, pact, o);
+        lds[slot][0][lo] = o.cb;
+        lds[slot][1][lo] = o.eb;
+        if constexpr (TGV) {
+            lds[slot][2][lo] = o.b1;
+            lds[slot][3][lo] = o.b2;
+            lds[slot][4][lo] = o.b3;
+        }
+        const bool owned_p = pact && kp >= k0 && kp < k1;
+        if (owned_p) {
+            const int l = qo.l0 + sk * kp;
+            out.chi[l] = o.chi;
+            out.eta[l] = o.eta;
+            if constexpr (TGV) {
+                out.w1[l] = o.w1;
+                out.w2[l] = o.w2;
+                out.w3[l] = o.w3;
+            }
+            if constexpr (CHECK)
+                conv_terms(o.chi, o.chi_o, o.eta, o.eta_o, acc);
+        }
+        if (t < NRING) {
+            DualVox<T> rdv;
+            bool ract;
+            POut<T> ro;
+            fused_primal_col<T, TGV>(in, mk, pc, sy, sk, nz, kp, qr, rdv, ract, ro);
+            lds[slot][0][lr] = ro.cb;
+            lds[slot][1][lr] = ro.eb;
+            if constexpr (TGV) {
+                lds[slot][2][lr] = ro.b1;
+                lds[slot][3][lr] = ro.b2;
+                lds[slot][4][lr] = ro.b3;
+            }
+        }
+
+        // ---- dual at plane kd = kp-1 (neighbour extrapolations were
+        // written to the other LDS slot in the previous step)
+        const int kd = kp - 1;
+        if (kd >= k0 && kd < k1 && (mprev & 2)) {
+            const int ps = slot ^ 1;
+            const int l = qo.l0 + sk * kd;
+            const Star<T> cs{cbc, lds[ps][0][lo - 1], lds[ps][0][lo + 1],
+                             lds[ps][0][lo - LW], lds[ps][0][lo + LW], cbm, o.cb};
+            const Star<T> es{ebc, lds[ps][1][lo - 1], lds[ps][1][lo + 1],
+                             lds[ps][1][lo - LW], lds[ps][1][lo + LW], ebm, o.eb};
+            DualVox<T> d = dprev;
+            d.nu = dual_nu(dc, d.nu, es, cs, f[l]);
+            if constexpr (TGV) {
+                tgv_dual_pq(dc, d, cs,
+                            wc1, lds[ps][2][lo + 1], lds[ps][2][lo + LW], o.b1,
+                            wc2, lds[ps][3][lo + 1], lds[ps][3][lo + LW], o.b2,
+                            wc3, lds[ps][4][lo + 1], lds[ps][4][lo + LW], o.b3);
+                out.q1[l] = d.q1; out.q2[l] = d.q2; out.q3[l] = d.q3;
+                out.q4[l] = d.q4; out.q5[l] = d.q5; out.q6[l] = d.q6;
+            } else {
+                tv_dual_p(dc, d, cs);
+            }
+            out.nu[l] = d.nu;
+            out.p1[l] = d.p1;
+            out.p2[l] = d.p2;
+            out.p3[l] = d.p3;
+        }
+        cbm = cbc; cbc = o.cb;
+        ebm = ebc; ebc = o.eb;
+        wc1 = o.b1; wc2 = o.b2; wc3 = o.b3;
+        dprev = dv;
+        mprev = mcur;
+        __syncthreads();
+    }
+
+    if constexpr (CHECK)
+        block_sum4<NT>(acc, partials + 4 * lb);
+}
+
+bool fused_shape_supported(int tx, int ty)
+{
+    return (tx == 64 && (ty == 4 || ty == 8 || ty == 16))
+           || (tx == 32 && (ty == 8 || ty == 16));
+}
+
+static void fused_tiles(Geom g, FusedShape sh, int &ntx, int &nty, int &nch)
+{
+    ntx = (g.nx + sh.tx - 1) / sh.tx;
+    nty = (g.ny + sh.ty - 1) / sh.ty;
+    nch = (g.nz - 2 + sh.kz - 1) / sh.kz;
+}
+
+int fused_blocks(Geom g, FusedShape sh)
+{
+    if (g.nx < 3 || g.ny < 3 || g.nz < 3)
+        return 0;
+    int ntx, nty, nch;
+    fused_tiles(g, sh, ntx, nty, nch);
+    return ntx * nty * nch;
+}
+
+template <typename T, bool TGV, int TX, int TY>
+static hipError_t launch_fused_shape(const Bank<T> &in, const Bank<T> &out,
+                                     const T *f, const uint8_t *mk, Geom g,
+                                     FusedShape sh, const DualCoef<T> &dc,
+                                     const PrimalCoef<T> &pc, const int *stop,
+                                     double *partials, hipStream_t st)
+{
+    int ntx, nty, nch;
+    fused_tiles(g, sh, ntx, nty, nch);
+    const int nblk = ntx * nty * nch;
+    if (partials)
+        fused_kernel<T, TGV, TX, TY, true><<<nblk, TX * TY, 0, st>>>(
+            in, out, f, mk, g, sh.kz, ntx, nty, nch, dc, pc, stop, partials);
+    else
+        fused_kernel<T, TGV, TX, TY, false><<<nblk, TX * TY, 0, st>>>(
+            in, out, f, mk, g, sh.kz, ntx, nty, nch, dc, pc, stop, partials);
+    return hipGetLastError();
+}
+
+template <typename T, bool TGV>
+hipError_t launch_fused(const Bank<T> &in, const Bank<T> &out, const T *f,
+                        const uint8_t *mk, Geom g, FusedShape sh,
+                        const DualCoef<T> &dc, const PrimalCoef<T> &pc,
+                        const int *stop, double *partials, hipStream_t st)
+{
+    if (g.nx < 3 || g.ny < 3 || g.nz < 3)
+        return hipSuccess;
+    if (sh.kz < 1 || !fused_shape_supported(sh.tx, sh.ty))
+        return hipErrorInvalidValue;
+#define QSM_FUSED_CASE(X, Y)                                                  \
+    if (sh.tx == X && sh.ty == Y)                                             \
+        return launch_fused_shape<T, TGV, X, Y>(in, out, f, mk, g, sh, dc, pc, \
+                                                stop, partials, st);
+    QSM_FUSED_CASE(64, 4)
+    QSM_FUSED_CASE(64, 8)
+    QSM_FUSED_CASE(64, 16)
+    QSM_FUSED_CASE(32, 8)
+    QSM_FUSED_CASE(32, 16)
+#undef QSM_FUSED_CASE
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+// convergence, masks, scaling
+// ---------------------------------------------------------------------------
+
+constexpr int FIN_T = 256;
+
+__global__ __launch_bounds__(FIN_T) void conv_finalize_kernel(
+    const double *partials, int nblk, Ctl *ctl, double *hist, int hist_cap,
+    double tol, uint32_t iter)
+{
+    if (ctl->stop)
+        return;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nblk; b += FIN_T)
+        for (int q = 0; q < 4; ++q)
+            s[q] += partials[4 * b + q];
+    __shared__ double tot[4];
+    block_sum4<FIN_T>(s, tot);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // pd_tgv_mex.c:1004-1010
+        const double n1 = sqrt(tot[2] / tot[0]);
+        const double n2 = sqrt(tot[3] / tot[1]);
+        const uint32_t c = ctl->checks;
+        if (hist && int(c) < hist_cap) {
+            hist[2 * c] = n1;
+            hist[2 * c + 1] = n2;
+        }
+        ctl->checks = c + 1;
+        ctl->nr[0] = n1;
+        ctl->nr[1] = n2;
+        if ((n1 < tol) && (n2 < 2.0 * tol)) {
+            ctl->stop = 1;
+            ctl->stop_iter = iter;
+        }
+    }
+}
+
+hipError_t launch_conv_finalize(const double *partials, int nblk, Ctl *ctl,
+                                double *hist, int hist_cap, double tol,
+                                uint32_t iter, hipStream_t st)
+{
+    conv_finalize_kernel<<<1, FIN_T, 0, st>>>(partials, nblk, ctl, hist,
+                                              hist_cap, tol, iter);
+    return hipGetLastError();
+}
+
+constexpr int RED_T = 256;
+
+template <typename T>
+__global__ __launch_bounds__(RED_T) void conv_sums_kernel(
+    const T *chi, const T *chi0, const T *eta, const T *eta0, size_t N,
+    double *partials)
+{
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    const size_t stride = size_t(gridDim.x) * RED_T;
+    for (size_t l = size_t(blockIdx.x) * RED_T + threadIdx.x; l < N; l += stride)
+        conv_terms(chi[l], chi0[l], eta[l], eta0[l], s);
+    block_sum4<RED_T>(s, partials + 4 * blockIdx.x);
+}
+
+template <typename T>
+hipError_t launch_conv_sums(const T *chi, const T *chi0, const T *eta,
+                            const T *eta0, size_t N, double *partials,
+                            int nblk, hipStream_t st)
+{
+    conv_sums_kernel<T><<<nblk, RED_T, 0, st>>>(chi, chi0, eta, eta0, N,
+                                                partials);
+    return hipGetLastError();
+}
+
+__global__ void mask_init_kernel(uint8_t *out, const uint8_t *m, Geom g,
+                                 int raw)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.nx)
+        return;
+    const int sy = g.nx, sk = g.nx * g.ny;
+    const int l = i + sy * j + sk * k;
+    // pd_tgv_mex.c:1075-1115: the "both neighbours present" code of all
+    // three axes, i.e. all six face neighbours exist and are in m.
+    const bool ix = (i > 0 && m[l - 1]) && (i + 1 < g.nx && m[l + 1]);
+    const bool iy = (j > 0 && m[l - sy]) && (j + 1 < g.ny && m[l + sy]);
+    const bool iz = (k > 0 && m[l - sk]) && (k + 1 < g.nz && m[l + sk]);
+    const bool mi = ix && iy && iz;
+    if (raw) {
+        out[l] = mi ? 1 : 0;
+        return;
+    }
+    const bool interior = i >= 1 && i + 1 < g.nx && j >= 1 && j + 1 < g.ny
+                          && k >= 1 && k + 1 < g.nz;
+    out[l] = uint8_t((interior && m[l] != 0 ? 1 : 0) | (interior && mi ? 2 : 0));
+}
+
+hipError_t launch_mask_init(uint8_t *out, const uint8_t *m, Geom g, int raw,
+                            hipStream_t st)
+{
+    const dim3 grid((g.nx + 255) / 256, g.ny, g.nz);
+    mask_init_kernel<<<grid, 256, 0, st>>>(out, m, g, raw);
+    return hipGetLastError();
+}
+
+__global__ void ctl_reset_kernel(Ctl *ctl, int n)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n)
+        return;
+    ctl[v].stop = 0;
+    ctl[v].stop_iter = 0;
+    ctl[v].checks = 0;
+    ctl[v].pad = 0;
+    ctl[v].nr[0] = 1.0;
+    ctl[v].nr[1] = 1.0;
+}
+
+hipError_t launch_ctl_reset(Ctl *ctl, int n, hipStream_t st)
+{
+    ctl_reset_kernel<<<(n + 63) / 64, 64, 0, st>>>(ctl, n);
+    return hipGetLastError();
+}
+
+template <typename T>
+__global__ void select_copy_kernel(T *dst, const T *chi0, const T *chi1,
+                                   const Ctl *ctl, uint32_t n_enq, size_t N)
+{
+    const uint32_t it = ctl->stop ? ctl->stop_iter : n_enq;
+    const T *src = (it & 1u) ? chi1 : chi0;
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t l = size_t(blockIdx.x) * blockDim.x + threadIdx.x; l < N; l += stride)
+        dst[l] = src[l];
+}
+
+template <typename T>
+hipError_t launch_select_copy(T *dst, const T *chi0, const T *chi1,
+                              const Ctl *ctl, uint32_t n_enq, size_t N,
+                              hipStream_t st)
+{
+    select_copy_kernel<T><<<1024, 256, 0, st>>>(dst, chi0, chi1, ctl, n_enq, N);
+    return hipGetLastError();
+}
+
+template <typename T>
+__global__ void scale_kernel(T *f, const T *src, T sg, size_t N)
+{
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t l = size_t(blockIdx.x) * blockDim.x + threadIdx.x; l < N; l += stride)
+        f[l] = sg * src[l];
+}
+
+template <typename T>
+hipError_t launch_scale(T *f, const T *src, T sg, size_t N, hipStream_t st)
+{
+    scale_kernel<T><<<2048, 256, 0, st>>>(f, src, sg, N);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// explicit instantiations
+// ---------------------------------------------------------------------------
+#define QSM_INST(T)                                                           \
+    template hipError_t launch_dual_sweep<T, true>(                           \
+        const Bank<T> &, const Bars<T> &, const T *, const uint8_t *, uint8_t, \
+        Geom, const DualCoef<T> &, const int *, hipStream_t);                 \
+    template hipError_t launch_dual_sweep<T, false>(                          \
+        const Bank<T> &, const Bars<T> &, const T *, const uint8_t *, uint8_t, \
+        Geom, const DualCoef<T> &, const int *, hipStream_t);                 \
+    template hipError_t launch_primal_sweep<T, true>(                         \
+        const Bank<T> &, const Bars<T> &, const uint8_t *, uint8_t, Geom,     \
+        const PrimalCoef<T> &, const int *, double *, hipStream_t);           \
+    template hipError_t launch_primal_sweep<T, false>(                        \
+        const Bank<T> &, const Bars<T> &, const uint8_t *, uint8_t, Geom,     \
+        const PrimalCoef<T> &, const int *, double *, hipStream_t);           \
+    template hipError_t launch_fused<T, true>(                                \
+        const Bank<T> &, const Bank<T> &, const T *, const uint8_t *, Geom,   \
+        FusedShape, const DualCoef<T> &, const PrimalCoef<T> &, const int *,  \
+        double *, hipStream_t);                                               \
+    template hipError_t launch_fused<T, false>(                               \
+        const Bank<T> &, const Bank<T> &, const T *, const uint8_t *, Geom,   \
+        FusedShape, const DualCoef<T> &, const PrimalCoef<T> &, const int *,  \
+        double *, hipStream_t);                                               \
+    template hipError_t launch_conv_sums<T>(const T *, const T *, const T *,  \
+                                            const T *, size_t, double *, int, \
+                                            hipStream_t);                     \
+    template hipError_t launch_scale<T>(T *, const T *, T, size_t, hipStream_t); \
+    template hipError_t launch_select_copy<T>(T *, const T *, const T *,      \
+                                              const Ctl *, uint32_t, size_t,  \
+                                              hipStream_t);
+
+QSM_INST(double)
+QSM_INST(float)
+#undef QSM_INST
+
+}  // namespace qsm

@@ -1,0 +1,138 @@
+"""Finite-difference operators on the GPU, mirroring the reference MEX
+(src/inversion/mex/gradfp*_mex.c, src/utils/fd/mex/grad{f,fm}{,_adj}_mex.c,
+src/unwrap/mex/lapw_mex.c): outputs are written IN PLACE into the arrays the
+caller passes first, which must be column-major (Fortran-ordered) and of the
+input's dtype, exactly as the MEX files overwrite their caller-allocated
+prhs[0..2].  The wrappers below also offer allocating forms (gradfp(u, h)
+etc.) like src/utils/fd/gradf.m.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from ._lib import check, dtype_code, fortran, h3, lib, ptr, sz3
+
+
+def _out_ok(a, ref):
+    if not (isinstance(a, np.ndarray) and a.flags.f_contiguous and a.dtype == ref.dtype
+            and a.shape == ref.shape):
+        raise ValueError("output arrays must be Fortran-contiguous, same shape/dtype as the input")
+
+
+def _vec_op(name, outs, u, h, G=None):
+    u = fortran(u)
+    if u.dtype not in (np.float32, np.float64):
+        raise TypeError("u must be single or double")
+    for o in outs:
+        _out_ok(o, u)
+    args = [dtype_code(u.dtype), *(ptr(o) for o in outs), ptr(u)]
+    if G is not None:
+        g = fortran(np.asarray(G) != 0, np.uint8)
+        args.append(ptr(g))
+    check(getattr(lib(), name)(*args, sz3(u.shape), h3(h)))
+
+
+def _adj_op(name, du, x, y, z, h, G=None):
+    x, y, z = (fortran(a, du.dtype) for a in (x, y, z))
+    _out_ok(du, x)
+    args = [dtype_code(du.dtype), ptr(du), ptr(x), ptr(y), ptr(z)]
+    if G is not None:
+        g = fortran(np.asarray(G) != 0, np.uint8)
+        args.append(ptr(g))
+    check(getattr(lib(), name)(*args, sz3(du.shape), h3(h)))
+
+
+# ---- MEX-shaped, in place ------------------------------------------------
+
+def gradfp_mex(dx, dy, dz, u, h):
+    """gradfp_mex(dx, dy, dz, u, h): periodic forward gradient, in place."""
+    _vec_op("qsm_gradfp", (dx, dy, dz), u, h)
+    return 1.0
+
+
+def gradfp_adj_mex(du, x, y, z, h):
+    """gradfp_adj_mex(du, x, y, z, h): -(periodic backward divergence)."""
+    _adj_op("qsm_gradfp_adj", du, x, y, z, h)
+    return 1.0
+
+
+def gradf_mex(dx, dy, dz, u, h):
+    """gradf_mex(dx, dy, dz, u, h): forward gradient, last face backward."""
+    _vec_op("qsm_gradf", (dx, dy, dz), u, h)
+    return 1.0
+
+
+def gradfm_mex(dx, dy, dz, u, G, h):
+    """gradfm_mex(dx, dy, dz, u, G, h): masked forward/backward gradient."""
+    _vec_op("qsm_gradfm", (dx, dy, dz), u, h, G)
+    return 1.0
+
+
+def gradf_adj_mex(du, x, y, z, h):
+    """gradf_adj_mex(du, x, y, z, h)."""
+    _adj_op("qsm_gradf_adj", du, x, y, z, h)
+    return 1.0
+
+
+def gradfm_adj_mex(du, x, y, z, G, h):
+    """gradfm_adj_mex(du, x, y, z, G, h)."""
+    _adj_op("qsm_gradfm_adj", du, x, y, z, h, G)
+    return 1.0
+
+
+def lapw_mex(d2u, u, h):
+    """lapw_mex(d2u, u, h): Laplacian of wrapped phase, interior, in place."""
+    u = fortran(u)
+    _out_ok(d2u, u)
+    check(lib().qsm_lapw(dtype_code(u.dtype), ptr(d2u), ptr(u), sz3(u.shape), h3(h)))
+    return 1.0
+
+
+# ---- allocating forms (src/utils/fd/gradf.m, gradfAdj.m, src/unwrap/lapw.m)
+
+def _zeros_like(u):
+    return np.zeros(u.shape, dtype=u.dtype, order="F")
+
+
+def gradfp(u, h=(1.0, 1.0, 1.0)):
+    u = fortran(u)
+    d = [_zeros_like(u) for _ in range(3)]
+    gradfp_mex(*d, u, h)
+    return tuple(d)
+
+
+def gradfp_adj(x, y, z, h=(1.0, 1.0, 1.0)):
+    x = fortran(x)
+    du = _zeros_like(x)
+    gradfp_adj_mex(du, x, y, z, h)
+    return du
+
+
+def gradf(u, mask=None, h=(1.0, 1.0, 1.0)):
+    """src/utils/fd/gradf.m:24-36: unmasked MEX when the mask is all true."""
+    u = fortran(u)
+    d = [_zeros_like(u) for _ in range(3)]
+    if mask is None or np.all(np.asarray(mask) != 0):
+        gradf_mex(*d, u, h)
+    else:
+        gradfm_mex(*d, u, mask, h)
+    return tuple(d)
+
+
+def gradf_adj(x, y, z, mask=None, h=(1.0, 1.0, 1.0)):
+    """src/utils/fd/gradfAdj.m:29-39."""
+    x = fortran(x)
+    du = _zeros_like(x)
+    if mask is None or np.all(np.asarray(mask) != 0):
+        gradf_adj_mex(du, x, y, z, h)
+    else:
+        gradfm_adj_mex(du, x, y, z, mask, h)
+    return du
+
+
+def lapw(u, h=(1.0, 1.0, 1.0)):
+    """src/unwrap/lapw.m:32-37: zeros, then the MEX fills the interior."""
+    u = fortran(u)
+    d = _zeros_like(u)
+    lapw_mex(d, u, h)
+    return d
