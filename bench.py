@@ -1,0 +1,240 @@
+#!/usr/bin/env python3
+"""bench.py -- TGV-QSM voxel-iterations/s at 256x256x176 on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload (C3, the size the metric is quoted at): seeded 60-ellipsoid phantom,
+256x256x176 at 1 mm, B0 = z, full-box mask, TGV alpha1 = 1e-3,
+alpha0 = 2*alpha1, tol = 0.  A step is one primal-dual iteration over the whole
+volume (one fused kernel pass; the every-10th-iteration convergence check is
+included).  The input is resident in HBM before timing starts.
+
+Reported: FP32 voxel-iters/s (value) and the FP64 drop-in beside it, chi
+error of FP32 against FP64 (the FP64 path is bitwise the oracle's), the
+roofline of the fused kernel measured with HIP events on its stream, and the
+CPU baseline (the oracle, the reference's C loop restated, on a bounded
+sample on this host's cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+for _p in (ROOT / "qsm.m_amd", ROOT):
+    if str(_p) not in sys.path:
+        sys.path.insert(0, str(_p))
+
+METRIC = ("TGV-QSM voxel-iters/s at 256²×176 (FP32 vs FP64, same GPU); "
+          "p99 solve latency")
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_fused_tgv_c3.json"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_input(cfg_name: str):
+    from qsm_amd import fd, phantom
+    from qsm_amd.configs import CONFIGS
+
+    cfg = CONFIGS[cfg_name]
+    t = time.perf_counter()
+    ph = phantom.make(cfg["spec"])
+    d2f = -fd.lapw(ph["field"], cfg["spec"].vsz)  # GPU lapw
+    log(f"[bench] phantom {cfg['spec'].shape} ready in {time.perf_counter() - t:.1f}s")
+    return cfg, np.asfortranarray(d2f), ph["mask"].astype(np.uint8, order="F")
+
+
+def time_solver(kind, dtype, cfg, d2f, mask, steps, warmup):
+    import torch
+    from qsm_amd import pd
+
+    shape = d2f.shape
+    stream = torch.cuda.Stream()
+    s = pd.Solver(kind, dtype, shape, cfg["spec"].vsz, cfg["lam"], cfg["alpha"], stream=stream)
+    s.load(d2f.astype(dtype, order="F"), mask)
+    s.iterate(warmup, 0.0)
+    stream.synchronize()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    w0 = time.perf_counter()
+    e0.record(stream)
+    s.iterate(steps, 0.0)
+    e1.record(stream)
+    e1.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    gpu = e0.elapsed_time(e1) / 1e3
+    it, stopped, _ = s.status()
+    chi = s.chi()
+    s.close()
+    assert it == warmup + steps and not stopped
+    return {"gpu_s": gpu, "wall_s": wall, "chi": chi}
+
+
+def copy_peak_gbs():
+    """Device-to-device bandwidth of the library's plain 16-byte vector copy
+    kernel (qsm_copy_dev) over 2 GiB: the achievable-peak reference."""
+    import torch
+    from qsm_amd._lib import check, lib
+
+    n = 2 ** 29  # float32 elements -> 2 GiB
+    a = torch.empty(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    torch.cuda.synchronize()
+    nbytes = n * 4
+    for _ in range(3):
+        check(lib().qsm_copy_dev(b.data_ptr(), a.data_ptr(), nbytes, None))
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    reps = 10
+    e0.record()
+    for _ in range(reps):
+        check(lib().qsm_copy_dev(b.data_ptr(), a.data_ptr(), nbytes, None))
+    e1.record()
+    e1.synchronize()
+    dt = e0.elapsed_time(e1) / 1e3 / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * n * 4 / dt / 1e9
+
+
+def cpu_baseline(cfg, d2f, mask, iters):
+    from oracle import oracle as orc
+
+    t = time.perf_counter()
+    orc.pd_tgv(d2f.astype(np.float64, order="F"), mask, cfg["spec"].vsz, cfg["lam"],
+               cfg["alpha"], 0.0, iters)
+    wall = time.perf_counter() - t
+    loop = orc.last_loop_seconds()
+    N = int(np.prod(d2f.shape))
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    return {
+        "value": N * iters / loop,
+        "unit": "voxel-iters/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle pd_tgv float64 (C11+OpenMP restatement of pd_tgv_mex.c, "
+                   f"-O2, no FMA), C3 input {d2f.shape[0]}x{d2f.shape[1]}x{d2f.shape[2]} "
+                   f"full-box mask, {iters} iterations, tol=0; iteration-loop time "
+                   f"{loop:.2f}s (end-to-end {wall:.2f}s), {threads} OpenMP threads"),
+    }
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[1])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="c3")
+    ap.add_argument("--cpu-iters", type=int, default=20,
+                    help="iterations of the CPU baseline sample (0: skip)")
+    ap.add_argument("--no-fp64", action="store_true")
+    a = ap.parse_args(argv)
+
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (the inversion of one volume runs on "
+              f"one MI355X); --gpus {a.gpus} is not supported", file=sys.stderr)
+        return 2
+    if int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs as a single process", file=sys.stderr)
+        return 2
+
+    import torch
+    from qsm_amd import _lib
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 2
+    _lib.lib()
+
+    cfg, d2f, mask = make_input(a.workload)
+    shape = d2f.shape
+    N = int(np.prod(shape))
+    n_active = int(np.count_nonzero(mask))
+
+    r32 = time_solver(_lib.TGV, np.float32, cfg, d2f, mask, a.steps, a.warmup)
+    log(f"[bench] fp32: {r32['gpu_s'] / a.steps * 1e3:.4f} ms/iter")
+    r64 = None
+    if not a.no_fp64:
+        r64 = time_solver(_lib.TGV, np.float64, cfg, d2f, mask, a.steps, a.warmup)
+        log(f"[bench] fp64: {r64['gpu_s'] / a.steps * 1e3:.4f} ms/iter")
+
+    v32 = N * a.steps / r32["gpu_s"]
+    out = {
+        "metric": METRIC,
+        "value": v32,
+        "unit": "voxel-iters/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": r32["wall_s"] / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded 60-ellipsoid phantom, dipole field + noise, -lapw)",
+        "config": {
+            "workload": (f"{a.workload.upper()} TGV primal-dual inversion "
+                         f"{shape[0]}x{shape[1]}x{shape[2]}, full-box mask, "
+                         f"alpha1={cfg['lam']}, alpha0={cfg['alpha']}*alpha1, tol=0"),
+            "voxels": N,
+            "n_active": n_active,
+            "schedule": "fused primal(n)+dual(n+1) single pass, 64x8 tiles, z-chunks of 16",
+        },
+    }
+    # roofline of the fused kernel (algorithmic bytes: 41 words + 1 mask byte
+    # per active voxel per iteration, the reference's minimal single-pass
+    # state traffic)
+    kern_s = r32["gpu_s"] / a.steps
+    alg_bytes = n_active * 41 * 4 + N * 1
+    ach = alg_bytes / kern_s / 1e9
+    traffic = None
+    if TRAFFIC_FILE.exists():
+        try:
+            traffic = json.loads(TRAFFIC_FILE.read_text()).get("f32", {}).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+    out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic}
+    if r64 is not None:
+        v64 = N * a.steps / r64["gpu_s"]
+        c32 = r32["chi"].astype(np.float64)
+        c64 = r64["chi"]
+        n64 = float(np.linalg.norm(c64))
+        out["fp64"] = {
+            "value": v64,
+            "ms_per_step": r64["wall_s"] / a.steps * 1e3,
+            "roofline_frac": (n_active * 41 * 8 + N) / (r64["gpu_s"] / a.steps) / 1e9 / HBM_PEAK_GBS,
+        }
+        out["fp32_over_fp64"] = v32 / v64
+        out["chi_err_fp32_vs_fp64"] = {
+            "rel_l2": float(np.linalg.norm(c32 - c64) / n64) if n64 > 0 else 0.0,
+            "max_abs": float(np.max(np.abs(c32 - c64))),
+            "iterations": a.warmup + a.steps,
+        }
+    try:
+        out["copy_peak_gbs"] = copy_peak_gbs()
+        out["roofline"]["frac_of_copy_peak"] = ach / out["copy_peak_gbs"]
+    except Exception as e:  # measurement aid only
+        log(f"[bench] copy peak failed: {e}")
+    if a.cpu_iters > 0:
+        log(f"[bench] CPU baseline: {a.cpu_iters} iterations of the oracle ...")
+        out["cpu_baseline"] = cpu_baseline(cfg, d2f, mask, a.cpu_iters)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -127,6 +127,12 @@ void qsm_pd_destroy(qsm_pd *s);
  * kernel then primal kernel).  Both give identical results. */
 int qsm_pd_set_schedule(qsm_pd *s, int schedule);
 
+/* Tile shape of the fused pass: tx x ty voxels per block (supported:
+ * 64x4, 64x8, 64x16, 32x8, 32x16) marching kz planes.  Results do not
+ * depend on it; speed does (see the autotuner).  Any time between calls. */
+int qsm_pd_set_tiles(qsm_pd *s, int tx, int ty, int kz);
+int qsm_pd_get_tiles(qsm_pd *s, int *tx, int *ty, int *kz);
+
 /* ------------------------------------------------------------------------
  * Primal-dual single sweeps on device arrays (per-operator access; the
  * layouts are the reference's, asynchronous on stream)
@@ -224,6 +230,16 @@ int qsm_gradfm_adj_dev(int dtype, void *du, const void *x, const void *y,
                        const double h[3], void *stream);
 int qsm_lapw_dev(int dtype, void *d2u, const void *u, const size_t sz[3],
                  const double h[3], void *stream);
+
+/* ------------------------------------------------------------------------
+ * Device utilities
+ * ------------------------------------------------------------------------ */
+
+/* Device-to-device copy of `bytes` (16-byte aligned pointers) with a
+ * bandwidth-saturating vector kernel on stream.  No reference counterpart:
+ * moves volumes between resident solves and serves as the plain copy kernel
+ * the HBM traffic counters are calibrated against. */
+int qsm_copy_dev(void *dst, const void *src, size_t bytes, void *stream);
 
 #ifdef __cplusplus
 }

@@ -56,6 +56,8 @@ def lib():
         L.orc_gradfm.argtypes = [C.c_int, _p, _p, _p, _p, _p, szp, dp]
         L.orc_gradfm_adj.argtypes = [C.c_int, _p, _p, _p, _p, _p, szp, dp]
         L.orc_lapw.argtypes = [C.c_int, _p, _p, szp, dp]
+        L.orc_last_loop_seconds.restype = _d
+        L.orc_last_loop_seconds.argtypes = []
         _lib = L
     return _lib
 
@@ -125,6 +127,11 @@ def pd_tv(d2f, mask, h, lam, tol, maxiter, verbose=0):
     if rc:
         raise ValueError("oracle rejected the arguments")
     return chi, it.value, (nr[0], nr[1])
+
+
+def last_loop_seconds() -> float:
+    """Iteration-loop time of the last pd_tgv/pd_tv call."""
+    return lib().orc_last_loop_seconds()
 
 
 # ---- single sweeps, in place on the given (Fortran-ordered) arrays ----------

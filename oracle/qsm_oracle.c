@@ -6,12 +6,27 @@
  * Build like the reference MEX (src/inversion/mex/make.m:7-8):
  *   gcc -std=c11 -O2 -fopenmp -fPIC -shared  (no -march: no FMA contraction)
  */
+#define _POSIX_C_SOURCE 199309L
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include "qsm_oracle.h"
+#include <time.h>
+
+/* Seconds spent in the iteration loop of the last whole solve (the
+ * reference's loop only: allocation and mask set-up excluded). */
+static double orc_loop_seconds_ = 0.0;
+
+static double orc_now(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+double orc_last_loop_seconds(void) { return orc_loop_seconds_; }
 
 /* ------------------------------------------------------------------------ */
 /* shared pieces                                                            */

@@ -123,6 +123,10 @@ int orc_pd_tv(int dtype, const void *d2f, const uint8_t *mask,
               uint32_t maxiter, uint32_t verbose,
               void *chi_out, uint32_t *iters_out, double *nr_out);
 
+/* Seconds the last orc_pd_tgv/orc_pd_tv call spent in its iteration loop
+ * (clock_gettime(CLOCK_MONOTONIC); allocation and set-up excluded). */
+double orc_last_loop_seconds(void);
+
 /* ---- finite-difference operators (outputs overwritten in place) -------- */
 
 /* gradfp_mex.c:147-242 / :50-144  periodic forward gradient. */

@@ -131,11 +131,11 @@ int iterate_impl(qsm_pd *s, uint32_t niter, double tol)
                 QSM_HIP(e);
                 s->dual_ahead = 1;
             }
-            const Bank<T> &in = (n1 & 1) ? b0 : b1;
-            const Bank<T> &out = (n1 & 1) ? b1 : b0;
+            const T *in = static_cast<const T *>(s->bank((n1 & 1) ? 0 : 1));
+            T *out = static_cast<T *>(s->bank((n1 & 1) ? 1 : 0));
             const hipError_t e = tgv
-                ? launch_fused<T, true>(in, out, f, s->mask(), g, s->shape, dc, pc, stop, part, s->stream)
-                : launch_fused<T, false>(in, out, f, s->mask(), g, s->shape, dc, pc, stop, part, s->stream);
+                ? launch_fused<T, true>(in, out, s->N, f, s->mask(), g, s->shape, dc, pc, stop, part, s->stream)
+                : launch_fused<T, false>(in, out, s->N, f, s->mask(), g, s->shape, dc, pc, stop, part, s->stream);
             QSM_HIP(e);
             nblk = fused_blocks(g, s->shape);
         } else {
@@ -212,6 +212,9 @@ int create_impl(qsm_pd **out, int kind, int dtype, const size_t sz[3],
     s->alpha = alpha;
     s->stream = as_stream(stream);
     s->shape = default_shape(kind, dtype);
+    // Banks above 4 GiB exceed the fused pass's 32-bit offsets: use the
+    // two-sweep schedule (same results) for such volumes.
+    s->schedule = fused_size_supported(kind, dtype, s->N) ? 0 : 1;
     qsm_step_sizes(h, &s->tau, &s->sigma);
     const Geom g = s->geom();
     s->npart = std::max({primal_sweep_blocks(g), fused_blocks(g, FusedShape{32, 8, 1}), 1});
@@ -401,8 +404,32 @@ int qsm_pd_set_schedule(qsm_pd *s, int schedule)
 {
     QSM_ARG(s, "null handle");
     QSM_ARG(schedule == 0 || schedule == 1, "schedule must be 0 or 1");
+    QSM_ARG(schedule == 1 || fused_size_supported(s->kind, s->dtype, s->N),
+            "volume too large for the fused schedule (bank >= 4 GiB)");
     QSM_ARG(s->n == 0 && !s->dual_ahead, "schedule can only change before the first iteration");
     s->schedule = schedule;
+    return QSM_OK;
+}
+
+int qsm_pd_set_tiles(qsm_pd *s, int tx, int ty, int kz)
+{
+    QSM_ARG(s, "null handle");
+    QSM_ARG(fused_shape_supported(tx, ty), "unsupported tile shape %dx%d", tx, ty);
+    QSM_ARG(kz >= 1, "kz must be >= 1");
+    const FusedShape sh{tx, ty, kz};
+    QSM_ARG(fused_blocks(s->geom(), sh) <= s->npart,
+            "tile shape needs %d partial-sum slots, %d allocated",
+            fused_blocks(s->geom(), sh), s->npart);
+    s->shape = sh;
+    return QSM_OK;
+}
+
+int qsm_pd_get_tiles(qsm_pd *s, int *tx, int *ty, int *kz)
+{
+    QSM_ARG(s, "null handle");
+    if (tx) *tx = s->shape.tx;
+    if (ty) *ty = s->shape.ty;
+    if (kz) *kz = s->shape.kz;
     return QSM_OK;
 }
 

@@ -69,14 +69,18 @@ struct FusedShape {
     int tx, ty, kz;
 };
 
+// in/out: bank bases, array a of voxel l at a*N + l (TGV: chi, eta, w1-3,
+// nu, p1-3, q1-6; TV: chi, eta, nu, p1-3).
 template <typename T, bool TGV>
-hipError_t launch_fused(const Bank<T> &in, const Bank<T> &out, const T *f,
+hipError_t launch_fused(const T *in, T *out, size_t N, const T *f,
                         const uint8_t *mk, Geom g, FusedShape sh,
                         const DualCoef<T> &dc, const PrimalCoef<T> &pc,
                         const int *stop, double *partials, hipStream_t st);
 
 int fused_blocks(Geom g, FusedShape sh);
 bool fused_shape_supported(int tx, int ty);
+// The fused pass addresses a bank with 32-bit byte offsets (< 4 GiB).
+bool fused_size_supported(int kind, int dtype, size_t N);
 
 // Reduces nblk partial sums (4 doubles each) and applies the tolerance test.
 hipError_t launch_conv_finalize(const double *partials, int nblk, Ctl *ctl,

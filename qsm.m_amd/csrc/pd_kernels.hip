@@ -190,353 +190,6 @@ hipError_t launch_primal_sweep(const Bank<T> &s, const Bars<T> &b,
 }
 
 // ---------------------------------------------------------------------------
-// fused schedule: primal of iteration n, then dual of iteration n+1
-// ---------------------------------------------------------------------------
-
-// z-queues of one (i, j) column while a block marches up a chunk of planes.
-template <typename T>
-struct ColQ {
-    int l0;             // i + nx*j, or -1 when the column is outside the grid
-    uint8_t mc;         // mask byte at the current plane kp
-    T nzm, nzc;         // nu at kp-1, kp
-    T p3m, q3m, q5m, q6m;  // z-backward neighbours (plane kp-1)
-};
-
-// Primal results at one voxel (zero where the voxel is not updated).
-template <typename T>
-struct POut {
-    T chi, eta, w1, w2, w3;   // new iterates
-    T cb, eb, b1, b2, b3;     // extrapolations
-    T chi_o, eta_o;           // old iterates (convergence terms)
-};
-
-// Primal update of column q at plane kp from bank `in`; also returns the
-// voxel's dual-side values (dv, zero where both mask bits are clear) and
-// advances the column's queues to plane kp+1.
-template <typename T, bool TGV>
-__device__ __forceinline__ void fused_primal_col(const Bank<T> &in,
-                                                 const uint8_t *__restrict__ mk,
-                                                 const PrimalCoef<T> &pc,
-                                                 int sy, int sk, int nz, int kp,
-                                                 ColQ<T> &q, DualVox<T> &dv,
-                                                 bool &pact, POut<T> &o)
-{
-    const uint8_t mc = q.mc;
-    uint8_t mn = 0;
-    T np = T(0);
-    dv = DualVox<T>{};
-    o = POut<T>{};
-    pact = false;
-    if (q.l0 >= 0) {
-        const int l = q.l0 + sk * kp;
-        if (kp + 1 < nz) {
-            mn = mk[l + sk];
-            if (mn & 2)
-                np = in.nu[l + sk];
-        }
-        if (mc & 3) {
-            dv.nu = q.nzc;
-            dv.p1 = in.p1[l];
-            dv.p2 = in.p2[l];
-            dv.p3 = in.p3[l];
-            if constexpr (TGV) {
-                dv.q1 = in.q1[l]; dv.q2 = in.q2[l]; dv.q3 = in.q3[l];
-                dv.q4 = in.q4[l]; dv.q5 = in.q5[l]; dv.q6 = in.q6[l];
-            }
-        }
-        pact = (mc & 1) != 0;
-        if (pact) {
-            PrimalIn<T> pi;
-            pi.nu = Star<T>{q.nzc, in.nu[l - 1], in.nu[l + 1], in.nu[l - sy],
-                            in.nu[l + sy], q.nzm, np};
-            pi.p1c = dv.p1; pi.p1xm = in.p1[l - 1];
-            pi.p2c = dv.p2; pi.p2ym = in.p2[l - sy];
-            pi.p3c = dv.p3; pi.p3zm = q.p3m;
-            if constexpr (TGV) {
-                pi.q1c = dv.q1; pi.q1xm = in.q1[l - 1];
-                pi.q2c = dv.q2; pi.q2xm = in.q2[l - 1]; pi.q2ym = in.q2[l - sy];
-                pi.q3c = dv.q3; pi.q3xm = in.q3[l - 1]; pi.q3zm = q.q3m;
-                pi.q4c = dv.q4; pi.q4ym = in.q4[l - sy];
-                pi.q5c = dv.q5; pi.q5ym = in.q5[l - sy]; pi.q5zm = q.q5m;
-                pi.q6c = dv.q6; pi.q6zm = q.q6m;
-            }
-            o.eta_o = in.eta[l];
-            o.chi_o = in.chi[l];
-            primal_eta_chi(pc, pi, o.eta_o, o.chi_o, o.eta, o.chi);
-            o.eb = extrap(o.eta, o.eta_o);
-            o.cb = extrap(o.chi, o.chi_o);
-            if constexpr (TGV) {
-                const T a1 = in.w1[l], a2 = in.w2[l], a3 = in.w3[l];
-                o.w1 = a1; o.w2 = a2; o.w3 = a3;
-                primal_w(pc, pi, o.w1, o.w2, o.w3);
-                o.b1 = extrap(o.w1, a1);
-                o.b2 = extrap(o.w2, a2);
-                o.b3 = extrap(o.w3, a3);
-            }
-        }
-    }
-    q.mc = mn;
-    q.nzm = q.nzc;
-    q.nzc = np;
-    q.p3m = dv.p3;
-    if constexpr (TGV) {
-        q.q3m = dv.q3;
-        q.q5m = dv.q5;
-        q.q6m = dv.q6;
-    }
-}
-
-// Loads the queues of a column for a march starting at primal plane kp0.
-template <typename T, bool TGV>
-__device__ __forceinline__ void fused_col_init(const Bank<T> &in,
-                                               const uint8_t *__restrict__ mk,
-                                               int sk, int kp0, ColQ<T> &q)
-{
-    q.mc = 0;
-    q.nzm = q.nzc = q.p3m = q.q3m = q.q5m = q.q6m = T(0);
-    if (q.l0 < 0)
-        return;
-    const int l = q.l0 + sk * kp0;
-    q.mc = mk[l];
-    if (q.mc & 2)
-        q.nzc = in.nu[l];
-    if (kp0 >= 1) {
-        const uint8_t mb = mk[l - sk];
-        if (mb & 2)
-            q.nzm = in.nu[l - sk];
-        if (mb & 3) {
-            q.p3m = in.p3[l - sk];
-            if constexpr (TGV) {
-                q.q3m = in.q3[l - sk];
-                q.q5m = in.q5[l - sk];
-                q.q6m = in.q6[l - sk];
-            }
-        }
-    }
-}
-
-// Ring cells (the x/y neighbours of the tile whose extrapolations the dual
-// needs): the -y and +y rows, then the -x and +x columns; no corners.
-template <int TX, int TY>
-__device__ __forceinline__ void ring_pos(int r, int &rx, int &ry)
-{
-    if (r < TX) { rx = r; ry = -1; }
-    else if (r < 2 * TX) { rx = r - TX; ry = TY; }
-    else if (r < 2 * TX + TY) { rx = -1; ry = r - 2 * TX; }
-    else { rx = TX; ry = r - 2 * TX - TY; }
-}
-
-template <typename T, bool TGV, int TX, int TY, bool CHECK>
-__global__ __launch_bounds__(TX *TY) void fused_kernel(
-    Bank<T> in, Bank<T> out, const T *__restrict__ f,
-    const uint8_t *__restrict__ mk, Geom g, int kz, int ntx, int nty,
-    int nchunk, DualCoef<T> dc, PrimalCoef<T> pc, const int *stop,
-    double *partials)
-{
-    constexpr int NT = TX * TY;
-    constexpr int NRING = 2 * TX + 2 * TY;
-    static_assert(NRING <= NT, "one ring cell per thread at most");
-    constexpr int LW = TX + 2;
-    constexpr int LP = (TX + 2) * (TY + 2);
-    constexpr int NB = TGV ? 5 : 2;
-    __shared__ T lds[2][NB][LP];
-
-    if (stop && *stop)
-        return;
-
-    // XCD-aware order: hardware block b runs on XCD b % 8; give each XCD a
-    // contiguous run of logical tiles so ring neighbours share its L2.
-    const int nblk = ntx * nty * nchunk;
-    const int per = nblk / 8, rem = nblk % 8;
-    const int xcd = int(blockIdx.x) % 8, idx = int(blockIdx.x) / 8;
-    const int lb = (xcd < rem) ? xcd * (per + 1) + idx
-                               : rem * (per + 1) + (xcd - rem) * per + idx;
-    const int bx = lb % ntx;
-    const int by = (lb / ntx) % nty;
-    const int bz = lb / (ntx * nty);
-
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const int sy = nx, sk = nx * ny;
-    const int t = threadIdx.x;
-    const int tx = t % TX, ty = t / TX;
-    const int i0 = bx * TX, j0 = by * TY;
-    const int k0 = 1 + bz * kz;
-    const int k1 = min(k0 + kz, nz - 1);  // owned planes [k0, k1)
-
-    // own column
-    const int oi = i0 + tx, oj = j0 + ty;
-    ColQ<T> qo;
-    qo.l0 = (oi < nx && oj < ny) ? oi + sy * oj : -1;
-    // ring column
-    ColQ<T> qr;
-    int rx = 0, ry = 0;
-    qr.l0 = -1;
-    if (t < NRING) {
-        ring_pos<TX, TY>(t, rx, ry);
-        const int ri = i0 + rx, rj = j0 + ry;
-        if (ri >= 0 && ri < nx && rj >= 0 && rj < ny)
-            qr.l0 = ri + sy * rj;
-    }
-    const int lo = (ty + 1) * LW + (tx + 1);
-    const int lr = (ry + 1) * LW + (rx + 1);
-
-    fused_col_init<T, TGV>(in, mk, sk, k0 - 1, qo);
-    if (t < NRING)
-        fused_col_init<T, TGV>(in, mk, sk, k0 - 1, qr);
-
-    // own-column extrapolation queues and the dual-side values one plane back
-    T cbm = T(0), cbc = T(0), ebm = T(0), ebc = T(0);
-    T wc1 = T(0), wc2 = T(0), wc3 = T(0);
-    DualVox<T> dprev{};
-    uint8_t mprev = 0;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-
-    for (int kp = k0 - 1; kp <= k1; ++kp) {
-        const int slot = kp & 1;
-        // ---- primal at plane kp: own column and (some threads) a ring cell
-        const uint8_t mcur = qo.mc;
-        DualVox<T> dv;
-        bool pact;
-        POut<T> o;
-        fused_primal_col<T, TGV>(in, mk, pc, sy, sk, nz, kp, qo, dv, pact, o);
-        lds[slot][0][lo] = o.cb;
-        lds[slot][1][lo] = o.eb;
-        if constexpr (TGV) {
-            lds[slot][2][lo] = o.b1;
-            lds[slot][3][lo] = o.b2;
-            lds[slot][4][lo] = o.b3;
-        }
-        const bool owned_p = pact && kp >= k0 && kp < k1;
-        if (owned_p) {
-            const int l = qo.l0 + sk * kp;
-            out.chi[l] = o.chi;
-            out.eta[l] = o.eta;
-            if constexpr (TGV) {
-                out.w1[l] = o.w1;
-                out.w2[l] = o.w2;
-                out.w3[l] = o.w3;
-            }
-            if constexpr (CHECK)
-                conv_terms(o.chi, o.chi_o, o.eta, o.eta_o, acc);
-        }
-        if (t < NRING) {
-            DualVox<T> rdv;
-            bool ract;
-            POut<T> ro;
-            fused_primal_col<T, TGV>(in, mk, pc, sy, sk, nz, kp, qr, rdv, ract, ro);
-            lds[slot][0][lr] = ro.cb;
-            lds[slot][1][lr] = ro.eb;
-            if constexpr (TGV) {
-                lds[slot][2][lr] = ro.b1;
-                lds[slot][3][lr] = ro.b2;
-                lds[slot][4][lr] = ro.b3;
-            }
-        }
-
-        // ---- dual at plane kd = kp-1 (neighbour extrapolations were
-        // written to the other LDS slot in the previous step)
-        const int kd = kp - 1;
-        if (kd >= k0 && kd < k1 && (mprev & 2)) {
-            const int ps = slot ^ 1;
-            const int l = qo.l0 + sk * kd;
-            const Star<T> cs{cbc, lds[ps][0][lo - 1], lds[ps][0][lo + 1],
-                             lds[ps][0][lo - LW], lds[ps][0][lo + LW], cbm, o.cb};
-            const Star<T> es{ebc, lds[ps][1][lo - 1], lds[ps][1][lo + 1],
-                             lds[ps][1][lo - LW], lds[ps][1][lo + LW], ebm, o.eb};
-            DualVox<T> d = dprev;
-            d.nu = dual_nu(dc, d.nu, es, cs, f[l]);
-            if constexpr (TGV) {
-                tgv_dual_pq(dc, d, cs,
-                            wc1, lds[ps][2][lo + 1], lds[ps][2][lo + LW], o.b1,
-                            wc2, lds[ps][3][lo + 1], lds[ps][3][lo + LW], o.b2,
-                            wc3, lds[ps][4][lo + 1], lds[ps][4][lo + LW], o.b3);
-                out.q1[l] = d.q1; out.q2[l] = d.q2; out.q3[l] = d.q3;
-                out.q4[l] = d.q4; out.q5[l] = d.q5; out.q6[l] = d.q6;
-            } else {
-                tv_dual_p(dc, d, cs);
-            }
-            out.nu[l] = d.nu;
-            out.p1[l] = d.p1;
-            out.p2[l] = d.p2;
-            out.p3[l] = d.p3;
-        }
-        cbm = cbc; cbc = o.cb;
-        ebm = ebc; ebc = o.eb;
-        wc1 = o.b1; wc2 = o.b2; wc3 = o.b3;
-        dprev = dv;
-        mprev = mcur;
-        __syncthreads();
-    }
-
-    if constexpr (CHECK)
-        block_sum4<NT>(acc, partials + 4 * lb);
-}
-
-bool fused_shape_supported(int tx, int ty)
-{
-    return (tx == 64 && (ty == 4 || ty == 8 || ty == 16))
-           || (tx == 32 && (ty == 8 || ty == 16));
-}
-
-static void fused_tiles(Geom g, FusedShape sh, int &ntx, int &nty, int &nch)
-{
-    ntx = (g.nx + sh.tx - 1) / sh.tx;
-    nty = (g.ny + sh.ty - 1) / sh.ty;
-    nch = (g.nz - 2 + sh.kz - 1) / sh.kz;
-}
-
-int fused_blocks(Geom g, FusedShape sh)
-{
-    if (g.nx < 3 || g.ny < 3 || g.nz < 3)
-        return 0;
-    int ntx, nty, nch;
-    fused_tiles(g, sh, ntx, nty, nch);
-    return ntx * nty * nch;
-}
-
-template <typename T, bool TGV, int TX, int TY>
-static hipError_t launch_fused_shape(const Bank<T> &in, const Bank<T> &out,
-                                     const T *f, const uint8_t *mk, Geom g,
-                                     FusedShape sh, const DualCoef<T> &dc,
-                                     const PrimalCoef<T> &pc, const int *stop,
-                                     double *partials, hipStream_t st)
-{
-    int ntx, nty, nch;
-    fused_tiles(g, sh, ntx, nty, nch);
-    const int nblk = ntx * nty * nch;
-    if (partials)
-        fused_kernel<T, TGV, TX, TY, true><<<nblk, TX * TY, 0, st>>>(
-            in, out, f, mk, g, sh.kz, ntx, nty, nch, dc, pc, stop, partials);
-    else
-        fused_kernel<T, TGV, TX, TY, false><<<nblk, TX * TY, 0, st>>>(
-            in, out, f, mk, g, sh.kz, ntx, nty, nch, dc, pc, stop, partials);
-    return hipGetLastError();
-}
-
-template <typename T, bool TGV>
-hipError_t launch_fused(const Bank<T> &in, const Bank<T> &out, const T *f,
-                        const uint8_t *mk, Geom g, FusedShape sh,
-                        const DualCoef<T> &dc, const PrimalCoef<T> &pc,
-                        const int *stop, double *partials, hipStream_t st)
-{
-    if (g.nx < 3 || g.ny < 3 || g.nz < 3)
-        return hipSuccess;
-    if (sh.kz < 1 || !fused_shape_supported(sh.tx, sh.ty))
-        return hipErrorInvalidValue;
-#define QSM_FUSED_CASE(X, Y)                                                  \
-    if (sh.tx == X && sh.ty == Y)                                             \
-        return launch_fused_shape<T, TGV, X, Y>(in, out, f, mk, g, sh, dc, pc, \
-                                                stop, partials, st);
-    QSM_FUSED_CASE(64, 4)
-    QSM_FUSED_CASE(64, 8)
-    QSM_FUSED_CASE(64, 16)
-    QSM_FUSED_CASE(32, 8)
-    QSM_FUSED_CASE(32, 16)
-#undef QSM_FUSED_CASE
-    return hipErrorInvalidValue;
-}
-
-// ---------------------------------------------------------------------------
 // convergence, masks, scaling
 // ---------------------------------------------------------------------------
 
@@ -710,14 +363,6 @@ hipError_t launch_scale(T *f, const T *src, T sg, size_t N, hipStream_t st)
     template hipError_t launch_primal_sweep<T, false>(                        \
         const Bank<T> &, const Bars<T> &, const uint8_t *, uint8_t, Geom,     \
         const PrimalCoef<T> &, const int *, double *, hipStream_t);           \
-    template hipError_t launch_fused<T, true>(                                \
-        const Bank<T> &, const Bank<T> &, const T *, const uint8_t *, Geom,   \
-        FusedShape, const DualCoef<T> &, const PrimalCoef<T> &, const int *,  \
-        double *, hipStream_t);                                               \
-    template hipError_t launch_fused<T, false>(                               \
-        const Bank<T> &, const Bank<T> &, const T *, const uint8_t *, Geom,   \
-        FusedShape, const DualCoef<T> &, const PrimalCoef<T> &, const int *,  \
-        double *, hipStream_t);                                               \
     template hipError_t launch_conv_sums<T>(const T *, const T *, const T *,  \
                                             const T *, size_t, double *, int, \
                                             hipStream_t);                     \

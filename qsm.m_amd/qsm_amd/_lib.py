@@ -58,6 +58,8 @@ _SIGS = {
     "qsm_pd_get_chi": (C.c_int, [_p, _p, C.c_int]),
     "qsm_pd_destroy": (None, [_p]),
     "qsm_pd_set_schedule": (C.c_int, [_p, C.c_int]),
+    "qsm_pd_set_tiles": (C.c_int, [_p, C.c_int, C.c_int, C.c_int]),
+    "qsm_pd_get_tiles": (C.c_int, [_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "qsm_tgv_dual_dev": (C.c_int, [C.c_int, _p, _p, _p, _p, _p, _p, _p, _u8p,
                                    C.POINTER(_sz_t), _dp, C.c_double, C.c_double,
                                    C.c_double, _p]),
@@ -71,6 +73,7 @@ _SIGS = {
     "qsm_convergence_dev": (C.c_int, [C.c_int, _p, _p, _p, _p, _sz_t, C.c_double,
                                       _dp, C.POINTER(C.c_int), _p]),
     "qsm_step_sizes": (None, [_dp, _dp, _dp]),
+    "qsm_copy_dev": (C.c_int, [_p, _p, _sz_t, _p]),
 }
 for _n in ("gradfp", "gradf"):
     _SIGS[f"qsm_{_n}"] = (C.c_int, [C.c_int, _p, _p, _p, _p, C.POINTER(_sz_t), _dp])

@@ -119,6 +119,15 @@ class Solver:
         check(lib().qsm_pd_set_schedule(self._h, int(schedule)))
         return self
 
+    def set_tiles(self, tx: int, ty: int, kz: int):
+        check(lib().qsm_pd_set_tiles(self._h, int(tx), int(ty), int(kz)))
+        return self
+
+    def tiles(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        check(lib().qsm_pd_get_tiles(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
     def iterate(self, niter: int, tol: float = 0.0):
         check(lib().qsm_pd_iterate(self._h, int(niter), float(tol)))
         return self
