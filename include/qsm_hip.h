@@ -232,6 +232,45 @@ int qsm_lapw_dev(int dtype, void *d2u, const void *u, const size_t sz[3],
                  const double h[3], void *stream);
 
 /* ------------------------------------------------------------------------
+ * FFT and the FFT dipole convolution
+ * ------------------------------------------------------------------------ */
+
+/* Smallest 2,3,5,7-smooth integer >= n: third_party/nextfastfft.m:59-123
+ * (inputs above 2^20 are halved until in range, looked up, scaled back). */
+size_t qsm_nextfastfft(size_t n);
+
+/* In-place 3-D complex transform of an interleaved (re, im) array of
+ * M[0] x M[1] x M[2] (x fastest): fft3.m:5 (inverse = 0) or ifft3.m:5
+ * (inverse = 1, including 1/M per axis).  Lengths must be 2,3,5,7-smooth
+ * and at most 2048 (double) / 4096 (single).  Waits for the stream. */
+int qsm_fft3_dev(int dtype, void *data, const size_t M[3], int inverse,
+                 void *stream);
+
+/* Padded size and leading offset per axis of padfastfft(u, pad) ('both',
+ * padfastfft.m:34-51): M = nextfastfft(sz + 2*pad) (sz where pad == 0). */
+int qsm_dipole_pad_size(const size_t sz[3], const size_t pad[3], size_t M[3],
+                        size_t off[3]);
+
+/* Bytes of complex work space qsm_dipole_conv_dev needs. */
+size_t qsm_dipole_work_bytes(int dtype, const size_t sz[3], const size_t pad[3]);
+
+/* field = real(ifft3(D .* fft3(padfastfft(chi, pad)))) cropped back to sz:
+ * the forward model the toolbox writes with dipoleKernel.m:56-66 (k-space
+ * kernel D = 1/3 - (k.b)^2/|k|^2, D(0) = 0, bdir normalised) and
+ * fft3/ifft3 (e.g. ndi.m:100-101, medi.m:151).  D is generated on the fly.
+ * Device pointers; work: qsm_dipole_work_bytes bytes.  Waits for the
+ * stream. */
+int qsm_dipole_conv_dev(int dtype, const void *chi, const size_t sz[3],
+                        const double h[3], const double bdir[3],
+                        const size_t pad[3], void *field, void *work,
+                        void *stream);
+
+/* Host-pointer form of qsm_dipole_conv_dev (synchronous). */
+int qsm_dipole_conv(int dtype, const void *chi, const size_t sz[3],
+                    const double h[3], const double bdir[3],
+                    const size_t pad[3], void *field);
+
+/* ------------------------------------------------------------------------
  * Device utilities
  * ------------------------------------------------------------------------ */
 

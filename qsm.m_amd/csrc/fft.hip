@@ -1,0 +1,696 @@
+// fft.hip -- 3D FFT and the FFT dipole convolution for gfx950.
+//
+// Reference idioms replaced: fft3.m / ifft3.m (separable c2c transforms;
+// ifft includes 1/M), dipoleKernel.m:56-66 (k-space dipole kernel) and
+// padfastfft.m / unpadfastfft.m ('both' zero padding to 7-smooth sizes), in
+// the composition real(ifft3(D .* fft3(pad(chi)))) cropped back.
+//
+// One kernel = one axis pass over a batch of pencils: a block loads P
+// pencils of length M into LDS (coalesced: rows along x, or P consecutive x
+// positions for the y/z axes), runs a mixed-radix Stockham transform in LDS
+// (radices 8, 7, 5, 4, 3, 2; twiddles from a table computed in double from
+// exact rationals, cast for single precision), and stores.  The dipole
+// convolution uses five passes: x (zero padding fused into the load, only
+// rows that hold data), y (only planes that hold data), z forward * D(k) *
+// z inverse in one pass (D generated in-kernel), y inverse, x inverse (crop,
+// real part and 1/M scaling fused into the store).  The passes are
+// HBM-bound: 2*M*sizeof(complex) bytes per pencil.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+
+using namespace qsm;
+
+namespace {
+
+constexpr int MAX_STAGES = 24;
+constexpr int FFT_THREADS = 256;
+
+template <typename T>
+struct Cx {
+    T re, im;
+};
+
+template <typename T>
+__device__ __forceinline__ Cx<T> cmul(Cx<T> a, Cx<T> b)
+{
+    return Cx<T>{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+template <typename T>
+__device__ __forceinline__ Cx<T> cadd(Cx<T> a, Cx<T> b) { return Cx<T>{a.re + b.re, a.im + b.im}; }
+template <typename T>
+__device__ __forceinline__ Cx<T> csub(Cx<T> a, Cx<T> b) { return Cx<T>{a.re - b.re, a.im - b.im}; }
+
+// Stage plan of one transform length (host-built, passed by value).
+struct Plan {
+    int M;
+    int nst;
+    int R[MAX_STAGES];    // radix of stage s
+    int Ns[MAX_STAGES];   // sub-transform length before stage s
+    int tw[MAX_STAGES];   // offset of stage s's twiddles in the table
+};
+
+// Small DFT of v[0..R-1] in place: y_k = sum_r v_r exp(sgn*2*pi*i*r*k/R),
+// R in {2, 3, 4, 5, 7, 8}, with the textbook symmetric-pair butterflies
+// (constants to 22 digits; cast to T).
+template <typename T>
+__device__ __forceinline__ Cx<T> mul_i(Cx<T> a, T sgn)  // a * (sgn*i)
+{
+    return Cx<T>{-sgn * a.im, sgn * a.re};
+}
+
+template <typename T>
+__device__ __forceinline__ Cx<T> scale(Cx<T> a, T c)
+{
+    return Cx<T>{a.re * c, a.im * c};
+}
+
+template <typename T>
+__device__ __forceinline__ void dft4(Cx<T> &a0, Cx<T> &a1, Cx<T> &a2, Cx<T> &a3, T sgn)
+{
+    const Cx<T> t0 = cadd(a0, a2), t1 = csub(a0, a2);
+    const Cx<T> t2 = cadd(a1, a3), t3 = mul_i(csub(a1, a3), sgn);
+    a0 = cadd(t0, t2);
+    a2 = csub(t0, t2);
+    a1 = cadd(t1, t3);
+    a3 = csub(t1, t3);
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void small_dft(Cx<T> *v, T sgn)
+{
+    if constexpr (R == 2) {
+        const Cx<T> a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 3) {
+        const T S3 = T(0.8660254037844386467637);
+        const Cx<T> t = cadd(v[1], v[2]);
+        const Cx<T> d = mul_i(scale(csub(v[1], v[2]), S3), sgn);
+        const Cx<T> m = csub(v[0], scale(t, T(0.5)));
+        v[0] = cadd(v[0], t);
+        v[1] = cadd(m, d);
+        v[2] = csub(m, d);
+    } else if constexpr (R == 4) {
+        dft4(v[0], v[1], v[2], v[3], sgn);
+    } else if constexpr (R == 5) {
+        const T C1 = T(0.3090169943749474241022), C2 = T(-0.8090169943749474241023);
+        const T S1 = T(0.9510565162951535721164), S2 = T(0.5877852522924731291687);
+        const Cx<T> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+        const Cx<T> d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
+        const Cx<T> a1 = cadd(v[0], cadd(scale(t1, C1), scale(t2, C2)));
+        const Cx<T> a2 = cadd(v[0], cadd(scale(t1, C2), scale(t2, C1)));
+        const Cx<T> b1 = mul_i(cadd(scale(d1, S1), scale(d2, S2)), sgn);
+        const Cx<T> b2 = mul_i(csub(scale(d1, S2), scale(d2, S1)), sgn);
+        v[0] = cadd(v[0], cadd(t1, t2));
+        v[1] = cadd(a1, b1);
+        v[4] = csub(a1, b1);
+        v[2] = cadd(a2, b2);
+        v[3] = csub(a2, b2);
+    } else if constexpr (R == 7) {
+        const T C1 = T(0.6234898018587335305250), C2 = T(-0.2225209339563144042889);
+        const T C3 = T(-0.9009688679024191262361);
+        const T S1 = T(0.7818314824680298087084), S2 = T(0.9749279121818236070181);
+        const T S3 = T(0.4338837391175581204758);
+        const Cx<T> t1 = cadd(v[1], v[6]), t2 = cadd(v[2], v[5]), t3 = cadd(v[3], v[4]);
+        const Cx<T> d1 = csub(v[1], v[6]), d2 = csub(v[2], v[5]), d3 = csub(v[3], v[4]);
+        // k = 1: cos(2 pi j/7) pairs (1,2,3); sin signs +,+,+
+        const Cx<T> a1 = cadd(v[0], cadd(cadd(scale(t1, C1), scale(t2, C2)), scale(t3, C3)));
+        const Cx<T> b1 = mul_i(cadd(cadd(scale(d1, S1), scale(d2, S2)), scale(d3, S3)), sgn);
+        // k = 2: jk mod 7 = 2, 4, 6 -> cos C2, C3, C1; sin S2, -S3, -S1
+        const Cx<T> a2 = cadd(v[0], cadd(cadd(scale(t1, C2), scale(t2, C3)), scale(t3, C1)));
+        const Cx<T> b2 = mul_i(csub(csub(scale(d1, S2), scale(d2, S3)), scale(d3, S1)), sgn);
+        // k = 3: jk mod 7 = 3, 6, 2 -> cos C3, C1, C2; sin S3, -S1, S2
+        const Cx<T> a3 = cadd(v[0], cadd(cadd(scale(t1, C3), scale(t2, C1)), scale(t3, C2)));
+        const Cx<T> b3 = mul_i(cadd(csub(scale(d1, S3), scale(d2, S1)), scale(d3, S2)), sgn);
+        v[0] = cadd(v[0], cadd(cadd(t1, t2), t3));
+        v[1] = cadd(a1, b1);
+        v[6] = csub(a1, b1);
+        v[2] = cadd(a2, b2);
+        v[5] = csub(a2, b2);
+        v[3] = cadd(a3, b3);
+        v[4] = csub(a3, b3);
+    } else {
+        static_assert(R == 8, "radix");
+        // 8 = 2 x 4: pair (r, r+4), twiddle the differences by W8^r, then
+        // two DFT-4s give the even and the odd outputs.
+        const T R2 = T(0.7071067811865475244008);
+        Cx<T> t[4], u[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            t[r] = cadd(v[r], v[r + 4]);
+            u[r] = csub(v[r], v[r + 4]);
+        }
+        // W8^1 = (1 + sgn i)/sqrt2, W8^2 = sgn i, W8^3 = (-1 + sgn i)/sqrt2
+        u[1] = scale(cadd(u[1], mul_i(u[1], sgn)), R2);
+        u[2] = mul_i(u[2], sgn);
+        u[3] = scale(csub(mul_i(u[3], sgn), u[3]), R2);
+        dft4(t[0], t[1], t[2], t[3], sgn);
+        dft4(u[0], u[1], u[2], u[3], sgn);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            v[2 * m] = t[m];
+            v[2 * m + 1] = u[m];
+        }
+    }
+}
+
+// One radix-R Stockham stage: A -> B (element i of pencil p at [i*P + p]).
+template <typename T, int R>
+__device__ __forceinline__ void stage(const Cx<T> *A, Cx<T> *B, int P, int M, int Ns,
+                                      const Cx<double> *__restrict__ tws, T sgn)
+{
+    const int nb = M / R;
+    for (int b = threadIdx.x; b < nb * P; b += FFT_THREADS) {
+        const int j = b / P, p = b % P;
+        const int k = j % Ns;
+        Cx<T> v[R];
+        v[0] = A[j * P + p];
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+            const Cx<double> w = tws[k * (R - 1) + (r - 1)];
+            const Cx<T> wt{T(w.re), sgn < 0 ? T(w.im) : T(-w.im)};
+            v[r] = cmul(A[(j + r * nb) * P + p], wt);
+        }
+        small_dft<T, R>(v, sgn);
+        const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            B[(base + r * Ns) * P + p] = v[r];
+    }
+}
+
+// Stockham transform of P pencils held in LDS; returns the buffer holding
+// the result.  sgn = -1 forward, +1 inverse (twiddles conjugated).
+template <typename T>
+__device__ Cx<T> *stockham(Cx<T> *A, Cx<T> *B, int P, const Plan *__restrict__ pl,
+                           const Cx<double> *__restrict__ tw, T sgn)
+{
+    const int M = pl->M, nst = pl->nst;
+    for (int s = 0; s < nst; ++s) {
+        const int R = pl->R[s], Ns = pl->Ns[s];
+        const Cx<double> *tws = tw + pl->tw[s];
+        switch (R) {
+        case 2: stage<T, 2>(A, B, P, M, Ns, tws, sgn); break;
+        case 3: stage<T, 3>(A, B, P, M, Ns, tws, sgn); break;
+        case 4: stage<T, 4>(A, B, P, M, Ns, tws, sgn); break;
+        case 5: stage<T, 5>(A, B, P, M, Ns, tws, sgn); break;
+        case 7: stage<T, 7>(A, B, P, M, Ns, tws, sgn); break;
+        default: stage<T, 8>(A, B, P, M, Ns, tws, sgn); break;
+        }
+        __syncthreads();
+        Cx<T> *t = A; A = B; B = t;
+    }
+    return A;
+}
+
+// Pass geometry.  The volume is complex, dims M0 x M1 x M2, x fastest.
+// axis 0: pencils are rows (i1, i2); the block takes P consecutive rows of
+// one plane.  axis 1/2: pencils run along y/z; the block takes P
+// consecutive x positions (coalesced) of one (i2) / (i1) coordinate.
+struct Vol {
+    int M0, M1, M2;
+};
+
+// ---- generic in-place pass (plain fft3 / ifft3) -------------------------
+template <typename T>
+__global__ __launch_bounds__(FFT_THREADS) void fft_pass_kernel(
+    Cx<T> *data, Vol v, int axis, int P, const Plan *pl, const Cx<double> *tw, T sgn, T scale)
+{
+    extern __shared__ unsigned char smem[];
+    Cx<T> *A = reinterpret_cast<Cx<T> *>(smem);
+    Cx<T> *B = A + size_t(P) * pl->M;
+    const int M = pl->M;
+    size_t base;
+    size_t stride;  // element stride along the axis
+    size_t pstride; // stride between the block's pencils
+    int np;         // pencils in this block
+    if (axis == 0) {
+        const int rows = v.M1 * v.M2;
+        const int r0 = blockIdx.x * P;
+        np = min(P, rows - r0);
+        base = size_t(r0) * v.M0;
+        stride = 1;
+        pstride = v.M0;
+    } else {
+        const int nxb = (v.M0 + P - 1) / P;
+        const int x0 = (blockIdx.x % nxb) * P;
+        const int other = blockIdx.x / nxb;
+        np = min(P, v.M0 - x0);
+        stride = axis == 1 ? size_t(v.M0) : size_t(v.M0) * v.M1;
+        base = axis == 1 ? size_t(x0) + size_t(other) * v.M0 * v.M1 : size_t(x0) + size_t(other) * v.M0;
+        pstride = 1;
+    }
+    for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
+        int p, i;
+        if (axis == 0) { p = e / M; i = e % M; }
+        else { i = e / P; p = e % P; }
+        A[i * P + p] = p < np ? data[base + size_t(p) * pstride + size_t(i) * stride] : Cx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    Cx<T> *R = stockham(A, B, P, pl, tw, sgn);
+    for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
+        int p, i;
+        if (axis == 0) { p = e / M; i = e % M; }
+        else { i = e / P; p = e % P; }
+        if (p < np) {
+            Cx<T> x = R[i * P + p];
+            x.re *= scale;
+            x.im *= scale;
+            data[base + size_t(p) * pstride + size_t(i) * stride] = x;
+        }
+    }
+}
+
+// ---- dipole convolution passes -------------------------------------------
+
+// x pass, forward: reads the real input (nx, ny, nz) zero-padded by
+// (o0, o1, o2) voxels before, for rows (j, k) inside the data region only.
+template <typename T>
+__global__ __launch_bounds__(FFT_THREADS) void dip_x_fwd(
+    Cx<T> *data, const T *chi, Vol v, int nx, int ny, int nz, int o0, int o1, int o2,
+    int P, const Plan *pl, const Cx<double> *tw)
+{
+    extern __shared__ unsigned char smem[];
+    Cx<T> *A = reinterpret_cast<Cx<T> *>(smem);
+    Cx<T> *B = A + size_t(P) * pl->M;
+    const int M = pl->M;
+    const int rows = ny * nz;
+    const int r0 = blockIdx.x * P;
+    const int np = min(P, rows - r0);
+    for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
+        const int p = e / M, i = e % M;
+        T x = T(0);
+        if (p < np && i >= o0 && i < o0 + nx) {
+            const int r = r0 + p;
+            x = chi[size_t(i - o0) + size_t(nx) * r];
+        }
+        A[i * P + p] = Cx<T>{x, T(0)};
+    }
+    __syncthreads();
+    Cx<T> *R = stockham(A, B, P, pl, tw, T(-1));
+    for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
+        const int p = e / M, i = e % M;
+        if (p < np) {
+            const int r = r0 + p;
+            const int j = r % ny + o1, k = r / ny + o2;
+            data[size_t(i) + size_t(v.M0) * (size_t(j) + size_t(v.M1) * k)] = R[i * P + p];
+        }
+    }
+}
+
+// y pass over planes k in [k0, k0 + nk) (forward or inverse).
+template <typename T>
+__global__ __launch_bounds__(FFT_THREADS) void dip_y(
+    Cx<T> *data, Vol v, int k0, int P, const Plan *pl, const Cx<double> *tw, T sgn)
+{
+    extern __shared__ unsigned char smem[];
+    Cx<T> *A = reinterpret_cast<Cx<T> *>(smem);
+    Cx<T> *B = A + size_t(P) * pl->M;
+    const int M = pl->M;
+    const int nxb = (v.M0 + P - 1) / P;
+    const int x0 = (blockIdx.x % nxb) * P;
+    const int k = k0 + blockIdx.x / nxb;
+    const int np = min(P, v.M0 - x0);
+    const size_t base = size_t(x0) + size_t(k) * v.M0 * v.M1;
+    for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
+        const int i = e / P, p = e % P;
+        A[i * P + p] = p < np ? data[base + p + size_t(i) * v.M0] : Cx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    Cx<T> *R = stockham(A, B, P, pl, tw, sgn);
+    for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
+        const int i = e / P, p = e % P;
+        if (p < np)
+            data[base + p + size_t(i) * v.M0] = R[i * P + p];
+    }
+}
+
+// k-space coordinate of index m on an axis of length n, voxel size h
+// (dipoleKernel.m:74-87 followed by ifftshift, :64).
+__device__ __forceinline__ double kcoord(int m, int n, double h)
+{
+    const double dk = 1.0 / (double(n) * h);
+    const int low = n / 2;
+    return double(m < n - low ? m : m - n) * dk;
+}
+
+// z pass: forward transform, multiply by D(k) / (M0*M1*M2), inverse.
+template <typename T>
+__global__ __launch_bounds__(FFT_THREADS) void dip_z(
+    Cx<T> *data, Vol v, int P, const Plan *pl, const Cx<double> *tw,
+    double h0, double h1, double h2, double b0, double b1, double b2, double inv_n)
+{
+    extern __shared__ unsigned char smem[];
+    Cx<T> *A = reinterpret_cast<Cx<T> *>(smem);
+    Cx<T> *B = A + size_t(P) * pl->M;
+    const int M = pl->M;
+    const int nxb = (v.M0 + P - 1) / P;
+    const int x0 = (blockIdx.x % nxb) * P;
+    const int j = blockIdx.x / nxb;
+    const int np = min(P, v.M0 - x0);
+    const size_t plane = size_t(v.M0) * v.M1;
+    const size_t base = size_t(x0) + size_t(j) * v.M0;
+    for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
+        const int i = e / P, p = e % P;
+        A[i * P + p] = p < np ? data[base + p + size_t(i) * plane] : Cx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    Cx<T> *R = stockham(A, B, P, pl, tw, T(-1));
+    Cx<T> *S = (R == A) ? B : A;
+    // D(k) = 1/3 - (k.b)^2 / |k|^2, D(0) = 0 (dipoleKernel.m:59-66), in double
+    const double Y = kcoord(j, v.M1, h1);
+    for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
+        const int i = e / P, p = e % P;
+        const double X = kcoord(x0 + p, v.M0, h0);
+        const double Z = kcoord(i, M, h2);
+        const double kz = (b0 * X + b1 * Y) + b2 * Z;
+        const double k2 = (X * X + Y * Y) + Z * Z;
+        const double D = (k2 == 0.0) ? 0.0 : (1.0 / 3.0 - (kz * kz) / k2);
+        const T d = T(D * inv_n);
+        Cx<T> x = R[i * P + p];
+        x.re *= d;
+        x.im *= d;
+        R[i * P + p] = x;
+    }
+    __syncthreads();
+    Cx<T> *Q = stockham(R, S, P, pl, tw, T(1));
+    for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
+        const int i = e / P, p = e % P;
+        if (p < np)
+            data[base + p + size_t(i) * plane] = Q[i * P + p];
+    }
+}
+
+// x pass, inverse: rows inside the output region; stores the real part,
+// cropped, into out (nx, ny, nz).
+template <typename T>
+__global__ __launch_bounds__(FFT_THREADS) void dip_x_inv(
+    const Cx<T> *data, T *out, Vol v, int nx, int ny, int nz, int o0, int o1, int o2,
+    int P, const Plan *pl, const Cx<double> *tw)
+{
+    extern __shared__ unsigned char smem[];
+    Cx<T> *A = reinterpret_cast<Cx<T> *>(smem);
+    Cx<T> *B = A + size_t(P) * pl->M;
+    const int M = pl->M;
+    const int rows = ny * nz;
+    const int r0 = blockIdx.x * P;
+    const int np = min(P, rows - r0);
+    for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
+        const int p = e / M, i = e % M;
+        Cx<T> x{T(0), T(0)};
+        if (p < np) {
+            const int r = r0 + p;
+            const int j = r % ny + o1, k = r / ny + o2;
+            x = data[size_t(i) + size_t(v.M0) * (size_t(j) + size_t(v.M1) * k)];
+        }
+        A[i * P + p] = x;
+    }
+    __syncthreads();
+    Cx<T> *R = stockham(A, B, P, pl, tw, T(1));
+    for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
+        const int p = e / M, i = e % M;
+        if (p < np && i >= o0 && i < o0 + nx) {
+            const int r = r0 + p;
+            out[size_t(i - o0) + size_t(nx) * r] = R[i * P + p].re;
+        }
+    }
+}
+
+// ---- host-side planning ----------------------------------------------------
+
+bool factor(int M, Plan &pl)
+{
+    pl.M = M;
+    pl.nst = 0;
+    int rest = M, Ns = 1, off = 0;
+    const int radices[] = {8, 7, 5, 4, 3, 2};
+    while (rest > 1) {
+        int R = 0;
+        for (int r : radices)
+            if (rest % r == 0) { R = r; break; }
+        if (!R || pl.nst >= MAX_STAGES)
+            return false;
+        pl.R[pl.nst] = R;
+        pl.Ns[pl.nst] = Ns;
+        pl.tw[pl.nst] = off;
+        off += Ns * (R - 1);
+        Ns *= R;
+        rest /= R;
+        ++pl.nst;
+    }
+    return true;
+}
+
+// exp(-2*pi*i*a/b) in double from the exact rational a/b: the angle is
+// reduced in long double before the sine and cosine.
+Cx<double> twiddle(long a, long b)
+{
+    a %= b;
+    const long double t = (long double)a / (long double)b;  // in [0, 1)
+    const long double ang = 2.0L * 3.141592653589793238462643383279502884L * t;
+    return Cx<double>{double(cosl(ang)), double(-sinl(ang))};
+}
+
+std::vector<Cx<double>> twiddles(const Plan &pl)
+{
+    std::vector<Cx<double>> t;
+    for (int s = 0; s < pl.nst; ++s) {
+        const int R = pl.R[s], Ns = pl.Ns[s];
+        for (int k = 0; k < Ns; ++k)
+            for (int r = 1; r < R; ++r)
+                t.push_back(twiddle(long(r) * k, long(Ns) * R));
+    }
+    if (t.empty())
+        t.push_back(Cx<double>{1.0, 0.0});
+    return t;
+}
+
+// Pencils per block: as many as fit a 64 KB LDS budget for the two
+// buffers, at most 16 (so that P consecutive x positions cover whole cache
+// lines), at least 1.
+int pencils(int M, size_t csize)
+{
+    int P = int((64 * 1024) / (2 * size_t(M) * csize));
+    if (P > 16) P = 16;
+    if (P < 1) P = 1;
+    return P;
+}
+
+struct AxisPlan {
+    Plan pl;                    // host copy
+    const Plan *dpl = nullptr;  // device copy (head of the buffer)
+    const Cx<double> *dtw = nullptr;  // device twiddle table (after it)
+};
+
+// Factors M, builds its twiddles and uploads plan + table into buf.
+template <typename T>
+int make_axis(AxisPlan &ap, DevBuf &buf, size_t M, const char *what)
+{
+    QSM_ARG(M >= 1 && 2 * M * sizeof(Cx<T>) <= 65536, "%s %zu too long (at most %zu)",
+            what, M, 65536 / (2 * sizeof(Cx<T>)));
+    QSM_ARG(factor(int(M), ap.pl), "%s %zu is not 2,3,5,7-smooth", what, M);
+    const std::vector<Cx<double>> t = twiddles(ap.pl);
+    const size_t head = 256;
+    int rc = buf.alloc(head + t.size() * sizeof(t[0]));
+    if (rc) return rc;
+    QSM_HIP(hipMemcpy(buf.p, &ap.pl, sizeof(Plan), hipMemcpyHostToDevice));
+    QSM_HIP(hipMemcpy(static_cast<char *>(buf.p) + head, t.data(), t.size() * sizeof(t[0]),
+                      hipMemcpyHostToDevice));
+    ap.dpl = static_cast<const Plan *>(buf.p);
+    ap.dtw = reinterpret_cast<const Cx<double> *>(static_cast<char *>(buf.p) + head);
+    return QSM_OK;
+}
+
+template <typename T>
+int fft3_dev_t(void *data, const size_t Msz[3], int inverse, hipStream_t st)
+{
+    AxisPlan ap[3];
+    DevBuf buf[3];
+    int rc;
+    for (int a = 0; a < 3; ++a)
+        if ((rc = make_axis<T>(ap[a], buf[a], Msz[a], "FFT length"))) return rc;
+    const Vol v{int(Msz[0]), int(Msz[1]), int(Msz[2])};
+    const T sgn = inverse ? T(1) : T(-1);
+    // fft3.m:5 and ifft3.m:5 apply axis 3, then 2, then 1
+    for (int a = 2; a >= 0; --a) {
+        const Plan &pl = ap[a].pl;
+        if (pl.M == 1) continue;
+        const int P = pencils(pl.M, sizeof(Cx<T>));
+        const size_t lds = 2 * size_t(P) * pl.M * sizeof(Cx<T>);
+        int nblk;
+        if (a == 0) nblk = (v.M1 * v.M2 + P - 1) / P;
+        else nblk = ((v.M0 + P - 1) / P) * (a == 1 ? v.M2 : v.M1);
+        const T scale = inverse ? T(1.0 / double(pl.M)) : T(1);
+        fft_pass_kernel<T><<<nblk, FFT_THREADS, lds, st>>>(
+            static_cast<Cx<T> *>(data), v, a, P, ap[a].dpl, ap[a].dtw, sgn, scale);
+        QSM_HIP(hipGetLastError());
+    }
+    QSM_HIP(hipStreamSynchronize(st));
+    return QSM_OK;
+}
+
+template <typename T>
+int dipole_conv_dev_t(const void *chi, const size_t sz[3], const double h[3],
+                      const double bdir[3], const size_t M[3], const size_t off[3],
+                      void *field, void *work, hipStream_t st)
+{
+    int rc;
+    AxisPlan ap[3];
+    DevBuf buf[3];
+    for (int a = 0; a < 3; ++a)
+        if ((rc = make_axis<T>(ap[a], buf[a], M[a], "padded length"))) return rc;
+    const Vol v{int(M[0]), int(M[1]), int(M[2])};
+    const int nx = int(sz[0]), ny = int(sz[1]), nz = int(sz[2]);
+    const int o0 = int(off[0]), o1 = int(off[1]), o2 = int(off[2]);
+    Cx<T> *d = static_cast<Cx<T> *>(work);
+    const size_t total = size_t(v.M0) * v.M1 * v.M2;
+    // rows outside the data region stay zero through the x pass
+    QSM_HIP(hipMemsetAsync(d, 0, total * sizeof(Cx<T>), st));
+    // normalised b (dipoleKernel.m:35-38)
+    const double bn = std::sqrt(bdir[0] * bdir[0] + bdir[1] * bdir[1] + bdir[2] * bdir[2]);
+    QSM_ARG(bn > 0.0, "B0 direction must be non-zero");
+    double b[3] = {bdir[0], bdir[1], bdir[2]};
+    if (bn != 1.0)
+        for (double &x : b) x /= bn;
+    const size_t cs = sizeof(Cx<T>);
+    {   // x forward on data rows
+        const Plan &pl = ap[0].pl;
+        const int P = pencils(pl.M, cs);
+        dip_x_fwd<T><<<(ny * nz + P - 1) / P, FFT_THREADS, 2 * size_t(P) * pl.M * cs, st>>>(
+            d, static_cast<const T *>(chi), v, nx, ny, nz, o0, o1, o2, P, ap[0].dpl, ap[0].dtw);
+        QSM_HIP(hipGetLastError());
+    }
+    {   // y forward on data planes
+        const Plan &pl = ap[1].pl;
+        const int P = pencils(pl.M, cs);
+        dip_y<T><<<((v.M0 + P - 1) / P) * nz, FFT_THREADS, 2 * size_t(P) * pl.M * cs, st>>>(
+            d, v, o2, P, ap[1].dpl, ap[1].dtw, T(-1));
+        QSM_HIP(hipGetLastError());
+    }
+    {   // z forward * D * z inverse, scaled by 1/(M0 M1 M2)
+        const Plan &pl = ap[2].pl;
+        const int P = pencils(pl.M, cs);
+        dip_z<T><<<((v.M0 + P - 1) / P) * v.M1, FFT_THREADS, 2 * size_t(P) * pl.M * cs, st>>>(
+            d, v, P, ap[2].dpl, ap[2].dtw, h[0], h[1], h[2],
+            b[0], b[1], b[2], 1.0 / double(total));
+        QSM_HIP(hipGetLastError());
+    }
+    {   // y inverse on output planes
+        const Plan &pl = ap[1].pl;
+        const int P = pencils(pl.M, cs);
+        dip_y<T><<<((v.M0 + P - 1) / P) * nz, FFT_THREADS, 2 * size_t(P) * pl.M * cs, st>>>(
+            d, v, o2, P, ap[1].dpl, ap[1].dtw, T(1));
+        QSM_HIP(hipGetLastError());
+    }
+    {   // x inverse on output rows, crop + real part
+        const Plan &pl = ap[0].pl;
+        const int P = pencils(pl.M, cs);
+        dip_x_inv<T><<<(ny * nz + P - 1) / P, FFT_THREADS, 2 * size_t(P) * pl.M * cs, st>>>(
+            d, static_cast<T *>(field), v, nx, ny, nz, o0, o1, o2, P, ap[0].dpl, ap[0].dtw);
+        QSM_HIP(hipGetLastError());
+    }
+    QSM_HIP(hipStreamSynchronize(st));
+    return QSM_OK;
+}
+
+// padfastfft.m:34-51 ('both'): padded length and leading pad of one axis.
+size_t next7(size_t n)
+{
+    // smallest 2,3,5,7-smooth number >= n (nextfastfft.m semantics for
+    // n <= 2^20; above that the value is halved, looked up and doubled)
+    size_t red = 0;
+    double v = double(n);
+    while (v > 1048576.0) { v /= 2.0; ++red; }
+    size_t best = 0;
+    for (size_t p2 = 1; p2 <= (1u << 21); p2 *= 2)
+        for (size_t p3 = p2; p3 <= (1u << 21); p3 *= 3)
+            for (size_t p5 = p3; p5 <= (1u << 21); p5 *= 5)
+                for (size_t p7 = p5; p7 <= (1u << 21); p7 *= 7)
+                    if (double(p7) >= v && (best == 0 || p7 < best))
+                        best = p7;
+    return best << red;
+}
+
+void pad_plan(const size_t sz[3], const size_t pad[3], size_t M[3], size_t off[3])
+{
+    for (int a = 0; a < 3; ++a) {
+        if (pad[a] == 0) {
+            M[a] = sz[a];
+            off[a] = 0;
+            continue;
+        }
+        M[a] = next7(sz[a] + 2 * pad[a]);
+        const size_t tot = M[a] - sz[a];
+        const size_t oddpre = (tot & 1) & (sz[a] & 1);
+        off[a] = tot / 2 + oddpre;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int qsm_fft3_dev(int dtype, void *data, const size_t M[3], int inverse, void *stream)
+{
+    QSM_ARG(data && M, "null pointer");
+    QSM_ARG(dtype == QSM_F64 || dtype == QSM_F32, "bad dtype");
+    return dtype == QSM_F64 ? fft3_dev_t<double>(data, M, inverse, as_stream(stream))
+                            : fft3_dev_t<float>(data, M, inverse, as_stream(stream));
+}
+
+size_t qsm_nextfastfft(size_t n) { return n == 0 ? 1 : next7(n); }
+
+int qsm_dipole_pad_size(const size_t sz[3], const size_t pad[3], size_t M[3], size_t off[3])
+{
+    QSM_ARG(sz && pad && M && off, "null pointer");
+    pad_plan(sz, pad, M, off);
+    return QSM_OK;
+}
+
+size_t qsm_dipole_work_bytes(int dtype, const size_t sz[3], const size_t pad[3])
+{
+    size_t M[3], off[3];
+    pad_plan(sz, pad, M, off);
+    return M[0] * M[1] * M[2] * 2 * dtype_size(dtype);
+}
+
+int qsm_dipole_conv_dev(int dtype, const void *chi, const size_t sz[3],
+                        const double h[3], const double bdir[3], const size_t pad[3],
+                        void *field, void *work, void *stream)
+{
+    int rc = check_grid(sz, h, 1);
+    if (rc) return rc;
+    QSM_ARG(chi && field && work && bdir && pad, "null pointer");
+    QSM_ARG(dtype == QSM_F64 || dtype == QSM_F32, "bad dtype");
+    size_t M[3], off[3];
+    pad_plan(sz, pad, M, off);
+    return dtype == QSM_F64
+        ? dipole_conv_dev_t<double>(chi, sz, h, bdir, M, off, field, work, as_stream(stream))
+        : dipole_conv_dev_t<float>(chi, sz, h, bdir, M, off, field, work, as_stream(stream));
+}
+
+int qsm_dipole_conv(int dtype, const void *chi, const size_t sz[3], const double h[3],
+                    const double bdir[3], const size_t pad[3], void *field)
+{
+    int rc = check_grid(sz, h, 1);
+    if (rc) return rc;
+    QSM_ARG(chi && field && bdir && pad, "null pointer");
+    QSM_ARG(dtype == QSM_F64 || dtype == QSM_F32, "bad dtype");
+    if ((rc = ensure_device())) return rc;
+    const size_t N = sz[0] * sz[1] * sz[2], es = dtype_size(dtype);
+    DevBuf dc, df, dw;
+    if ((rc = dc.alloc(N * es)) || (rc = df.alloc(N * es))
+        || (rc = dw.alloc(qsm_dipole_work_bytes(dtype, sz, pad))))
+        return rc;
+    QSM_HIP(hipMemcpy(dc.p, chi, N * es, hipMemcpyHostToDevice));
+    if ((rc = qsm_dipole_conv_dev(dtype, dc.p, sz, h, bdir, pad, df.p, dw.p, nullptr)))
+        return rc;
+    QSM_HIP(hipMemcpy(field, df.p, N * es, hipMemcpyDeviceToHost));
+    return QSM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,75 @@
+"""GPU FFT and FFT dipole convolution (fft3.m / ifft3.m, dipoleKernel.m,
+padfastfft.m) against numpy's pocketfft (MATLAB's FFT is closed: parity
+with it is unpinned; pocketfft and closed forms pin the transform) and the
+harness's convolution."""
+import numpy as np
+import pytest
+
+from qsm_amd import dipole, fastfft, phantom
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("shape", [(8, 9, 10), (28, 30, 14), (45, 7, 12), (64, 1, 3), (49, 25, 16), (2, 3, 5)])
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+def test_fft3_matches_pocketfft(cuda, shape, dt):
+    rng = np.random.default_rng(0)
+    x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(dt)
+    tol = 1e-13 if dt == np.complex128 else 2e-6
+    X = dipole.fft3(x)
+    ref = np.fft.fftn(x.astype(np.complex128))
+    assert np.linalg.norm(X - ref) / np.linalg.norm(ref) < tol
+    xi = dipole.ifft3(X)
+    assert np.linalg.norm(xi - x) / np.linalg.norm(x) < tol
+
+
+def test_fft3_impulse_and_constant(cuda):
+    x = np.zeros((12, 10, 14), complex)
+    x[0, 0, 0] = 1.0
+    np.testing.assert_allclose(dipole.fft3(x), np.ones(x.shape), atol=1e-15)
+    X = np.full(x.shape, 3.0 + 0j)
+    y = dipole.ifft3(X)
+    want = np.zeros(x.shape, complex)
+    want[0, 0, 0] = 3.0
+    np.testing.assert_allclose(y, want, atol=1e-14)
+
+
+def test_fft3_parseval_radix7(cuda):
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((49, 14, 7)) + 0j
+    X = dipole.fft3(x)
+    assert abs((np.abs(X) ** 2).sum() / x.size - (np.abs(x) ** 2).sum()) < 1e-10 * (np.abs(x) ** 2).sum()
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("shape,vsz,bdir,pad", [
+    ((24, 22, 20), (1.0, 1.0, 1.0), (0, 0, 1), (12, 11, 10)),
+    ((30, 17, 21), (0.9, 0.9, 1.2), (0, -0.258819, 0.965926), (15, 8, 10)),
+    ((16, 16, 16), (1.0, 1.0, 1.0), (1, 1, 1), (0, 0, 0)),
+])
+def test_dipole_conv_matches_fft_reference(cuda, dt, shape, vsz, bdir, pad):
+    rng = np.random.default_rng(2)
+    chi = np.asfortranarray(rng.standard_normal(shape).astype(dt))
+    got = dipole.dipole_conv(chi, vsz, bdir, pad)
+    up = fastfft.padfastfft(chi.astype(np.float64), pad)
+    D = fastfft.dipole_kernel(up.shape, vsz, bdir)
+    ref = np.real(np.fft.ifftn(D * np.fft.fftn(up)))[fastfft.unpad_slices(up.shape, shape, pad)]
+    tol = 1e-12 if dt == np.float64 else 1e-5
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < tol
+
+
+def test_dipole_conv_pad_sizes_match_padfastfft(cuda):
+    for shape, pad in (((24, 22, 20), (12, 11, 10)), ((7, 9, 11), (3, 0, 5)), ((176, 256, 5), (88, 128, 0))):
+        M, off = dipole.pad_size(shape, pad)
+        up = fastfft.padfastfft(np.ones(shape), pad)
+        assert M == up.shape
+        sl = fastfft.unpad_slices(up.shape, shape, pad)
+        assert off == tuple(s.start for s in sl)
+
+
+def test_harness_field_on_gpu_equals_host(cuda):
+    spec = phantom.PhantomSpec(shape=(32, 30, 28), n_ellipsoids=6, semi_axes=(2, 6), mask="sphere",
+                               mask_param=(12.0,), noise=0.0, seed=5, bdir=(0.1, 0.0, 0.99))
+    ph = phantom.make(spec)
+    got = dipole.dipole_conv(ph["chi"], spec.vsz, spec.bdir, [s // 2 for s in spec.shape])
+    assert np.linalg.norm(got - ph["field"]) / np.linalg.norm(ph["field"]) < 1e-12
