@@ -232,6 +232,55 @@ int qsm_lapw_dev(int dtype, void *d2u, const void *u, const size_t sz[3],
                  const double h[3], void *stream);
 
 /* ------------------------------------------------------------------------
+ * Batched inversions: many uneven volumes advanced together (no reference
+ * counterpart -- QSM.m solves one volume per MEX call; this is the batching
+ * runtime built on the same per-voxel arithmetic, so every volume's result
+ * is bitwise the single-volume solve's).
+ * ------------------------------------------------------------------------ */
+typedef struct qsm_batch qsm_batch;
+
+/* A batch of one dtype whose fused passes use tx x ty tiles.  hbm_budget:
+ * bytes of volume state the batch may hold (admission limit). */
+int qsm_batch_create(qsm_batch **out, int dtype, int tx, int ty,
+                     size_t hbm_budget, int max_volumes, void *stream);
+
+/* Device bytes one admitted volume holds. */
+size_t qsm_batch_volume_bytes(int kind, int dtype, const size_t sz[3], int tx,
+                              int ty, int kz, uint32_t maxiter);
+
+/* Admits one inversion (kind QSM_TV/QSM_TGV, the arguments of pd_t*v_mex):
+ * loads it, applies the dual of iteration 1 and makes it live; *slot
+ * receives its slot.  QSM_EFULL when the HBM budget or the slots are
+ * exhausted.  Waits for the stream. */
+int qsm_batch_add(qsm_batch *b, int kind, const size_t sz[3], const double h[3],
+                  double lam, double alpha, const void *d2f, const uint8_t *mask,
+                  int on_device, uint32_t maxiter, double tol, int kz, int *slot);
+
+/* Enqueues `chunk` iterations of every live volume (two launches per
+ * iteration, replayed from a captured hipGraph).  Asynchronous. */
+int qsm_batch_step(qsm_batch *b, uint32_t chunk);
+
+/* Waits for the stream and refreshes the per-slot state. */
+int qsm_batch_sync(qsm_batch *b);
+
+/* Slot state after the last sync: used, live (still iterating), iterations. */
+int qsm_batch_state(qsm_batch *b, int slot, int *used, int *live, uint32_t *iter);
+
+/* Result of a slot: chi (host, N values; may be NULL), iterations run,
+ * whether the tolerance stopped it, last norms.  Waits for the stream. */
+int qsm_batch_result(qsm_batch *b, int slot, void *chi, uint32_t *iters,
+                     int *stopped, double nr[2]);
+
+/* Frees a slot and its HBM. */
+int qsm_batch_remove(qsm_batch *b, int slot);
+
+/* 1 (default): replay captured graphs; 0: launch directly. */
+int qsm_batch_set_graph(qsm_batch *b, int use_graph);
+
+size_t qsm_batch_used_bytes(qsm_batch *b);
+void qsm_batch_destroy(qsm_batch *b);
+
+/* ------------------------------------------------------------------------
  * FFT and the FFT dipole convolution
  * ------------------------------------------------------------------------ */
 

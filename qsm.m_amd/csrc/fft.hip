@@ -494,7 +494,8 @@ int make_axis(AxisPlan &ap, DevBuf &buf, size_t M, const char *what)
             what, M, 65536 / (2 * sizeof(Cx<T>)));
     QSM_ARG(factor(int(M), ap.pl), "%s %zu is not 2,3,5,7-smooth", what, M);
     const std::vector<Cx<double>> t = twiddles(ap.pl);
-    const size_t head = 256;
+    // plan at the head, the table after it on a 256-byte boundary
+    constexpr size_t head = (sizeof(Plan) + 255) / 256 * 256;
     int rc = buf.alloc(head + t.size() * sizeof(t[0]));
     if (rc) return rc;
     QSM_HIP(hipMemcpy(buf.p, &ap.pl, sizeof(Plan), hipMemcpyHostToDevice));

@@ -38,6 +38,17 @@ Layout layout(int kind, int dtype, const size_t sz[3], int npart, int hist_cap)
     return L;
 }
 
+// Partial-sum slots that cover every supported fused shape (at kz = 1, the
+// most blocks a shape can have) and the two-sweep primal.
+int max_partials(Geom g)
+{
+    static const int shapes[][2] = {{64, 4}, {64, 8}, {64, 16}, {32, 8}, {32, 16}, {32, 32}};
+    int n = std::max(primal_sweep_blocks(g), 1);
+    for (const auto &sh : shapes)
+        n = std::max(n, fused_blocks(g, FusedShape{sh[0], sh[1], 1}));
+    return n;
+}
+
 // Default fused tile shape per precision/model (tuned on MI355X; the
 // batching runtime's autotuner may pick others).
 FusedShape default_shape(int kind, int dtype)
@@ -217,7 +228,7 @@ int create_impl(qsm_pd **out, int kind, int dtype, const size_t sz[3],
     s->schedule = fused_size_supported(kind, dtype, s->N) ? 0 : 1;
     qsm_step_sizes(h, &s->tau, &s->sigma);
     const Geom g = s->geom();
-    s->npart = std::max({primal_sweep_blocks(g), fused_blocks(g, FusedShape{32, 8, 1}), 1});
+    s->npart = max_partials(g);
     s->hist_cap = hist_cap;
     s->L = layout(kind, dtype, sz, s->npart, hist_cap);
     const hipError_t e = hipMalloc(reinterpret_cast<void **>(&s->mem), s->L.total);
@@ -309,7 +320,7 @@ size_t qsm_pd_footprint(int kind, int dtype, const size_t sz[3])
 {
     if (!sz) return 0;
     const Geom g{int(sz[0]), int(sz[1]), int(sz[2])};
-    const int npart = std::max({primal_sweep_blocks(g), fused_blocks(g, FusedShape{32, 8, 1}), 1});
+    const int npart = max_partials(g);
     return layout(kind, dtype, sz, npart, 1024).total;
 }
 

@@ -1,0 +1,157 @@
+"""Autotuner for the fused primal-dual pass: picks the tile shape (tx x ty)
+and z-chunk depth kz per size class (kind, precision, volume shape) by
+timing candidates on the device with HIP events, and keeps the winners in
+a JSON table.  Results never depend on the choice (every shape is bitwise
+identical), only speed does.
+
+    from qsm_amd import autotune
+    autotune.tune(TGV, np.float32, (256, 256, 176))      # measure, store
+    autotune.best(TGV, np.float32, (256, 256, 176))      # (tx, ty, kz)
+
+The packaged table `tuning_mi355x.json` holds the MI355X measurements made
+with `python -m qsm_amd.autotune`; tune() adds to a user table (path in
+QSM_TUNING, default ~/.cache/qsm_amd/tuning.json) that takes precedence.
+"""
+from __future__ import annotations
+
+import json
+import os
+from pathlib import Path
+
+import numpy as np
+
+from . import _lib
+
+SHAPES = [(64, 4), (64, 8), (64, 16), (32, 8), (32, 16), (32, 32)]
+PACKAGED = Path(__file__).resolve().parent / "tuning_mi355x.json"
+USER = Path(os.environ.get("QSM_TUNING", Path.home() / ".cache" / "qsm_amd" / "tuning.json"))
+DEFAULT = (64, 8, 16)
+
+
+def key(kind, dtype, shape):
+    k = "tgv" if kind == _lib.TGV else "tv"
+    return f"{k}/{np.dtype(dtype).name}/{shape[0]}x{shape[1]}x{shape[2]}"
+
+
+def _load(path: Path) -> dict:
+    try:
+        return json.loads(path.read_text())
+    except Exception:
+        return {}
+
+
+def table() -> dict:
+    t = _load(PACKAGED)
+    t.update(_load(USER))
+    return t
+
+
+def kz_candidates(nz: int, n_tiles_xy: int, ncu: int = 256):
+    """Chunk depths worth timing: short chunks (many blocks) to whole-column
+    marches that leave exactly one or two blocks per CU."""
+    inner = max(1, nz - 2)
+    c = {8, 12, 16, 22, 32, 44, 58, 88}
+    for blocks in (ncu, 2 * ncu, 4 * ncu):
+        nch = max(1, -(-blocks // max(1, n_tiles_xy)))
+        c.add(-(-inner // nch))
+    return sorted(k for k in c if 1 <= k <= inner)
+
+
+def tune(kind, dtype, shape, iters=40, repeats=3, shapes=SHAPES, store=True, log=None):
+    """Times every (tx, ty, kz) candidate on the current device and returns
+    the fastest as dict(tx, ty, kz, ms)."""
+    import torch
+
+    from .pd import Solver
+
+    dtype = np.dtype(dtype)
+    rng = np.random.default_rng(0)
+    d2f = np.asfortranarray((rng.standard_normal(shape) * 1e-2).astype(dtype))
+    m = np.ones(shape, np.uint8, order="F")
+    st = torch.cuda.Stream()
+    best = None
+    with Solver(kind, dtype, shape, (1, 1, 1), 1e-3, 2.0, stream=st) as s:
+        s.load(d2f, m)
+        for tx, ty in shapes:
+            nxy = -(-shape[0] // tx) * -(-shape[1] // ty)
+            for kz in kz_candidates(shape[2], nxy):
+                try:
+                    s.set_tiles(tx, ty, kz)
+                except _lib.QsmError:
+                    continue
+                s.iterate(4)
+                st.synchronize()
+                ts = []
+                for _ in range(repeats):
+                    e0 = torch.cuda.Event(enable_timing=True)
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    e0.record(st)
+                    s.iterate(iters)
+                    e1.record(st)
+                    e1.synchronize()
+                    ts.append(e0.elapsed_time(e1) / iters)
+                ms = float(np.median(ts))
+                if log:
+                    log(f"{key(kind, dtype, shape)} {tx}x{ty} kz={kz}: {ms:.4f} ms")
+                if best is None or ms < best["ms"]:
+                    best = {"tx": tx, "ty": ty, "kz": kz, "ms": ms}
+    if store and best is not None:
+        t = _load(USER)
+        t[key(kind, dtype, shape)] = best
+        USER.parent.mkdir(parents=True, exist_ok=True)
+        USER.write_text(json.dumps(t, indent=1, sort_keys=True))
+    return best
+
+
+def best(kind, dtype, shape, tiles=None):
+    """(tx, ty, kz) for this size class: the exact entry, else the entry of
+    the same kind/precision with the nearest voxel count (kz clipped), else
+    the default.  With tiles=(tx, ty) only entries of that shape count."""
+    t = table()
+    k = key(kind, dtype, shape)
+    e = t.get(k)
+    if e and (tiles is None or (e["tx"], e["ty"]) == tuple(tiles)):
+        return e["tx"], e["ty"], e["kz"]
+    prefix = k.rsplit("/", 1)[0] + "/"
+    n = shape[0] * shape[1] * shape[2]
+    cands = []
+    for kk, v in t.items():
+        if not kk.startswith(prefix):
+            continue
+        if tiles is not None and (v["tx"], v["ty"]) != tuple(tiles):
+            continue
+        dims = [int(x) for x in kk.rsplit("/", 1)[1].split("x")]
+        cands.append((abs(np.log(dims[0] * dims[1] * dims[2] / n)), v, dims))
+    if cands:
+        _, v, dims = min(cands, key=lambda c: c[0])
+        kz = max(1, min(v["kz"], shape[2] - 2)) if shape[2] > 2 else 1
+        return v["tx"], v["ty"], kz
+    if tiles is not None:
+        return tiles[0], tiles[1], DEFAULT[2]
+    return DEFAULT
+
+
+def main(argv=None):
+    import argparse
+    import sys
+
+    ap = argparse.ArgumentParser(description="tune the fused pass per size class")
+    ap.add_argument("--out", default=None, help="write the table here (default: user table)")
+    ap.add_argument("--sizes", default="96x96x96,128x128x128,160x160x96,224x224x128,256x256x176,320x320x192")
+    ap.add_argument("--iters", type=int, default=40)
+    a = ap.parse_args(argv)
+    res = {}
+    for sz in a.sizes.split(","):
+        shape = tuple(int(x) for x in sz.split("x"))
+        for kind in (_lib.TGV, _lib.TV):
+            for dt in (np.float32, np.float64):
+                b = tune(kind, dt, shape, iters=a.iters, store=False,
+                         log=lambda s: print(s, file=sys.stderr, flush=True))
+                res[key(kind, dt, shape)] = b
+                print(key(kind, dt, shape), b, flush=True)
+    if a.out:
+        Path(a.out).write_text(json.dumps(res, indent=1, sort_keys=True))
+
+
+if __name__ == "__main__":
+    main()

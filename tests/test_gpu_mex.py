@@ -1,0 +1,99 @@
+"""The MEX drop-in wrappers on the GPU, called the way MATLAB calls them
+(mexFunction through the CI shim): pd_tgv_mex / pd_tv_mex return chi
+bitwise equal to the oracle's solve and print the reference's progress
+table (pd_tgv_mex.c:196-221: a header, iteration 1, every
+maxiter/max(verbose, 4)-th iteration, the last one and the one the
+tolerance test stopped at); the finite
+difference wrappers overwrite their first arguments in place."""
+import re
+
+import numpy as np
+import pytest
+
+from helpers import brainish_mask, rand_field
+from mex_driver import LIB, MxArray, call
+from oracle import oracle as orc
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not LIB.exists(), reason="MEX CI library not built")]
+
+H = (0.8, 0.9, 1.1)
+
+
+def _same(a, b):
+    assert a.shape == b.shape and a.dtype == b.dtype
+    np.testing.assert_array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+def _expected_rows(maxiter, verbose, iters):
+    vb = max(verbose, 4)
+    pmod = maxiter // min(vb, maxiter)
+    return [it for it in range(1, iters + 1)
+            if it == 1 or it == maxiter or it % pmod == 0]
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_pd_tgv_mex_matches_oracle_and_prints_table(cuda, dt):
+    shape = (30, 26, 22)
+    rng = np.random.default_rng(0)
+    f = rand_field(rng, shape, dt, 1e-2)
+    m = brainish_mask(shape, 1).astype(bool)
+    maxiter, verbose = 60, 6
+    (chi,), text = call("pd_tgv_mex", 1, f, m, np.array(H), 2e-3, 2.0, 0.0, maxiter, verbose)
+    want, iters, nr = orc.pd_tgv(f, m, H, 2e-3, 2.0, 0.0, maxiter)
+    _same(chi, want)
+    lines = text.strip().splitlines()
+    assert lines[0].split() == ["Iter", "||delta", "chi||", "||delta", "eta||"]
+    rows = [int(re.match(r"\s*(\d+)/(\d+)", ln).group(1)) for ln in lines[1:]]
+    assert rows == _expected_rows(maxiter, verbose, iters)
+    last = lines[-1].split()
+    assert float(last[1]) == pytest.approx(nr[0], rel=1e-3)
+    assert float(last[2]) == pytest.approx(nr[1], rel=1e-3)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_pd_tv_mex_matches_oracle(cuda, dt):
+    shape = (28, 24, 20)
+    rng = np.random.default_rng(2)
+    f = rand_field(rng, shape, dt, 1e-2)
+    m = brainish_mask(shape, 3).astype(bool)
+    (chi,), text = call("pd_tv_mex", 1, f, m, np.array(H), 3e-3, 0.0, 40, 0)
+    want, _, _ = orc.pd_tv(f, m, H, 3e-3, 0.0, 40)
+    _same(chi, want)
+    assert text == ""
+
+
+def test_pd_tgv_mex_tolerance_stop(cuda):
+    """A tolerance reached early ends the solve at a multiple of 10
+    iterations, the same one the oracle stops at."""
+    shape = (24, 22, 20)
+    rng = np.random.default_rng(4)
+    f = rand_field(rng, shape, np.float64, 1e-3)
+    m = brainish_mask(shape, 5).astype(bool)
+    want, iters, _ = orc.pd_tgv(f, m, H, 1e-3, 2.0, 5e-2, 500)
+    assert iters < 500 and iters % 10 == 0
+    (chi,), text = call("pd_tgv_mex", 1, f, m, np.array(H), 1e-3, 2.0, 5e-2, 500, 1)
+    _same(chi, want)
+    last = int(re.match(r"\s*(\d+)/", text.strip().splitlines()[-1]).group(1))
+    assert last == iters
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_fd_mex_in_place(cuda, dt):
+    shape = (21, 17, 13)
+    rng = np.random.default_rng(6)
+    u = rand_field(rng, shape, dt)
+    G = rng.random(shape) < 0.6
+    outs = [MxArray(np.zeros(shape, dt)) for _ in range(3)]
+    call("gradfm_mex", 0, *outs, u, G, np.array(H))
+    for o, w in zip(outs, orc.gradfm(u, G, H)):
+        _same(o.numpy(), w)
+    du = MxArray(np.zeros(shape, dt))
+    x, y, z = (o.numpy() for o in outs)
+    call("gradfp_adj_mex", 0, du, x, y, z, np.array(H))
+    _same(du.numpy(), orc.gradfp_adj(x, y, z, H))
+    d2u = MxArray(np.zeros(shape, dt))
+    call("lapw_mex", 0, d2u, u, np.array(H))
+    _same(d2u.numpy(), orc.lapw(u, H))
+    for o in (*outs, du, d2u):
+        o.free()
