@@ -47,7 +47,7 @@ def make_input(cfg_name: str):
 
     cfg = CONFIGS[cfg_name]
     t = time.perf_counter()
-    ph = phantom.make(cfg["spec"])
+    ph = phantom.make(cfg["spec"], field="gpu")  # GPU dipole convolution
     d2f = -fd.lapw(ph["field"], cfg["spec"].vsz)  # GPU lapw
     log(f"[bench] phantom {cfg['spec'].shape} ready in {time.perf_counter() - t:.1f}s")
     return cfg, np.asfortranarray(d2f), ph["mask"].astype(np.uint8, order="F")
@@ -60,6 +60,8 @@ def time_solver(kind, dtype, cfg, d2f, mask, steps, warmup):
     shape = d2f.shape
     stream = torch.cuda.Stream()
     s = pd.Solver(kind, dtype, shape, cfg["spec"].vsz, cfg["lam"], cfg["alpha"], stream=stream)
+    s.autotune()
+    tiles = s.tiles()
     s.load(d2f.astype(dtype, order="F"), mask)
     s.iterate(warmup, 0.0)
     stream.synchronize()
@@ -78,7 +80,7 @@ def time_solver(kind, dtype, cfg, d2f, mask, steps, warmup):
     chi = s.chi()
     s.close()
     assert it == warmup + steps and not stopped
-    return {"gpu_s": gpu, "wall_s": wall, "chi": chi}
+    return {"gpu_s": gpu, "wall_s": wall, "chi": chi, "tiles": tiles}
 
 
 def copy_peak_gbs():
@@ -132,6 +134,106 @@ def cpu_baseline(cfg, d2f, mask, iters):
     }
 
 
+def _phantom_input(spec):
+    """-lapw(field) of a seeded phantom, field synthesised with the GPU
+    dipole convolution; returns (d2f float64, mask uint8)."""
+    from qsm_amd import fd, phantom
+
+    ph = phantom.make(spec, field="gpu")
+    d2f = -fd.lapw(ph["field"], spec.vsz)
+    return np.asfortranarray(d2f), ph["mask"].astype(np.uint8, order="F")
+
+
+def batch_leg(iters: int, warm: int = 10):
+    """C4: 48 uneven TV/TGV volumes advanced together (FP32, captured
+    chunks of 10 iterations); compared with the same volumes solved one
+    after another by the single-volume solver."""
+    import torch
+    from qsm_amd import _lib, configs, pd
+    from qsm_amd.batch import Batch
+
+    vols = configs.c4_volumes()
+    t = time.perf_counter()
+    inputs = [_phantom_input(v["spec"]) for v in vols]
+    log(f"[bench] C4: {len(vols)} phantoms in {time.perf_counter() - t:.1f}s")
+    kinds = [_lib.TGV if v["kind"] == "tgv" else _lib.TV for v in vols]
+    vox = sum(int(np.prod(v["spec"].shape)) for v in vols)
+    with Batch(np.float32, tiles=(64, 8), hbm_budget=160 << 30, max_volumes=64) as b:
+        for v, k, (d2f, m) in zip(vols, kinds, inputs):
+            b.add(k, d2f, m, v["spec"].vsz, v["lam"], v["alpha"], maxiter=warm + iters)
+        resident = b.used_bytes
+        b.step(warm)
+        b.sync()
+        t0 = time.perf_counter()
+        for _ in range(iters // 10):
+            b.step(10)
+        b.sync()
+        dt = time.perf_counter() - t0
+        done = [b.state(s)[2] for s in range(len(vols))]
+    assert all(d == warm + 10 * (iters // 10) for d in done), done
+    run = 10 * (iters // 10)
+    # the same volumes one after another (per size class and kind, tuned tiles)
+    st = torch.cuda.Stream()
+    per = {}
+    for v, k, (d2f, m) in zip(vols, kinds, inputs):
+        key = (v["spec"].shape, k)
+        if key in per:
+            continue
+        with pd.Solver(k, np.float32, v["spec"].shape, v["spec"].vsz, v["lam"], v["alpha"],
+                       stream=st) as s:
+            s.autotune()
+            s.load(d2f, m)
+            s.iterate(warm)
+            st.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            s.iterate(run)
+            e1.record(st)
+            e1.synchronize()
+            per[key] = e0.elapsed_time(e1) / 1e3 / run
+    seq = sum(per[(v["spec"].shape, k)] for v, k in zip(vols, kinds)) * run
+    return {
+        "workload": ("C4 48 TV/TGV inversions, sizes uniform over 96^3, 128^3, 160x160x96, "
+                     "224x224x128, 256x256x176, own phantom/mask/B0 each, FP32"),
+        "volumes": len(vols), "voxels": vox, "iterations": run,
+        "voxel_iters_per_s": vox * run / dt, "wall_s": dt,
+        "sequential_voxel_iters_per_s": vox * run / seq,
+        "speedup_vs_sequential": seq / dt,
+        "resident_gib": resident / 2 ** 30,
+    }
+
+
+def serve_leg(n: int, rate: float):
+    """C5: Poisson arrivals served by continuous batching; latency from
+    arrival to chi on the host."""
+    from qsm_amd import _lib, configs
+    from qsm_amd.batch import Batch
+    from qsm_amd.serve import Request, Server
+
+    specs = configs.c5_requests(n=n, rate=rate)
+    t = time.perf_counter()
+    reqs = []
+    for i, r in enumerate(specs):
+        d2f, m = _phantom_input(r["spec"])
+        reqs.append(Request(i, r["arrival"], _lib.TGV if r["kind"] == "tgv" else _lib.TV,
+                            d2f.astype(np.float32, order="F"), m, r["spec"].vsz, r["lam"],
+                            r["alpha"], r["maxiter"]))
+    log(f"[bench] C5: {n} requests prepared in {time.perf_counter() - t:.1f}s")
+    with Batch(np.float32, tiles=(64, 8), hbm_budget=160 << 30, max_volumes=64) as b:
+        stats = Server(b, chunk=10, keep_chi=False).run(reqs)
+    assert all(r.iters == s["maxiter"] for r, s in zip(reqs, specs))
+    return {
+        "workload": ("C5 Poisson arrivals, TV/TGV 50/50, 70% 224x224x128, 20% 256x256x176, "
+                     "10% 320x320x192, budgets 300-1000 iterations, FP32"),
+        "requests": stats.n, "rate_per_s": rate,
+        "p50_latency_s": stats.p50, "p99_latency_s": stats.p99,
+        "mean_wait_s": stats.mean_wait, "peak_live": stats.peak_live,
+        "voxel_iters_per_s": stats.voxel_iters_per_s, "wall_s": stats.wall,
+        "last_arrival_s": float(max(r.arrival for r in reqs)),
+    }
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[1])
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +243,11 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-iters", type=int, default=20,
                     help="iterations of the CPU baseline sample (0: skip)")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--batch-iters", type=int, default=100,
+                    help="iterations of the C4 batch leg (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=40,
+                    help="requests of the C5 serving leg (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
     a = ap.parse_args(argv)
 
     if a.gpus != 1:
@@ -191,7 +298,8 @@ def main(argv=None) -> int:
                          f"alpha1={cfg['lam']}, alpha0={cfg['alpha']}*alpha1, tol=0"),
             "voxels": N,
             "n_active": n_active,
-            "schedule": "fused primal(n)+dual(n+1) single pass, 64x8 tiles, z-chunks of 16",
+            "schedule": ("fused primal(n)+dual(n+1) single pass, {}x{} tiles, z-chunks of {} "
+                         "(autotuned)").format(*r32["tiles"]),
         },
     }
     # roofline of the fused kernel (algorithmic bytes: 41 words + 1 mask byte
@@ -229,6 +337,13 @@ def main(argv=None) -> int:
         out["roofline"]["frac_of_copy_peak"] = ach / out["copy_peak_gbs"]
     except Exception as e:  # measurement aid only
         log(f"[bench] copy peak failed: {e}")
+    if a.batch_iters > 0:
+        out["batch"] = batch_leg(a.batch_iters)
+        log(f"[bench] C4 batch: {out['batch']['voxel_iters_per_s']:.3e} voxel-iters/s")
+    if a.serve_requests > 0:
+        out["serving"] = serve_leg(a.serve_requests, a.serve_rate)
+        log(f"[bench] C5 serving: p50 {out['serving']['p50_latency_s']:.3f}s "
+            f"p99 {out['serving']['p99_latency_s']:.3f}s")
     if a.cpu_iters > 0:
         log(f"[bench] CPU baseline: {a.cpu_iters} iterations of the oracle ...")
         out["cpu_baseline"] = cpu_baseline(cfg, d2f, mask, a.cpu_iters)

@@ -57,10 +57,19 @@ class Batch:
         return int(lib().qsm_batch_volume_bytes(kind, dtype_code(self.dtype), sz3(shape),
                                                 self.tiles[0], self.tiles[1], int(kz), int(maxiter)))
 
-    def add(self, kind, d2f, mask, vsz, lam, alpha=0.0, maxiter=1000, tol=0.0, kz=16):
+    def kz_for(self, kind, shape):
+        """The tuned z-chunk depth of this size class for the batch's tiles."""
+        from . import autotune
+
+        return autotune.best(kind, self.dtype, shape, tiles=self.tiles)[2]
+
+    def add(self, kind, d2f, mask, vsz, lam, alpha=0.0, maxiter=1000, tol=0.0, kz=None):
         """Admits one inversion; returns its slot (raises QsmAdmissionError
-        when the HBM budget or the slots are exhausted)."""
+        when the HBM budget or the slots are exhausted).  kz=None takes the
+        tuned depth of the size class."""
         slot = C.c_int()
+        if kz is None:
+            kz = self.kz_for(kind, tuple(int(s) for s in d2f.shape))
         if hasattr(d2f, "data_ptr"):
             shape = tuple(int(s) for s in d2f.shape)
             check(lib().qsm_batch_add(self._h, kind, sz3(shape), h3(vsz), float(lam), float(alpha),

@@ -123,6 +123,13 @@ class Solver:
         check(lib().qsm_pd_set_tiles(self._h, int(tx), int(ty), int(kz)))
         return self
 
+    def autotune(self):
+        """Applies the tuned (tx, ty, kz) of this size class (qsm_amd.autotune
+        table; results are bitwise independent of the choice)."""
+        from . import autotune
+
+        return self.set_tiles(*autotune.best(self.kind, self.dtype, self.shape))
+
     def tiles(self):
         a, b, c = C.c_int(), C.c_int(), C.c_int()
         check(lib().qsm_pd_get_tiles(self._h, C.byref(a), C.byref(b), C.byref(c)))

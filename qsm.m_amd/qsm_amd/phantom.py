@@ -109,13 +109,27 @@ def dipole_field(chi, vsz, bdir, workers=None) -> np.ndarray:
     return np.asfortranarray(f[unpad_slices(up.shape, shape, pads)])
 
 
-def make(spec: PhantomSpec):
-    """Returns dict(chi, field, mask) for the spec (field with noise)."""
+def dipole_field_gpu(chi, vsz, bdir) -> np.ndarray:
+    """dipole_field through the library's GPU dipole convolution
+    (qsm_dipole_conv, float64): same padding, kernel and crop."""
+    from .dipole import dipole_conv
+
+    return dipole_conv(np.asfortranarray(chi), vsz, bdir, [s // 2 for s in chi.shape])
+
+
+def make(spec: PhantomSpec, field: str = "host"):
+    """Returns dict(chi, field, mask) for the spec (field with noise).
+    field="gpu" synthesises the field with the GPU dipole convolution."""
     rng = np.random.default_rng(spec.seed)
     mask = make_mask(spec.shape, spec.mask, spec.mask_param)
     chi = ellipsoids(spec.shape, mask, spec.n_ellipsoids, spec.semi_axes,
                      spec.chi_range, rng)
-    fld = dipole_field(chi, spec.vsz, spec.bdir)
+    if field == "gpu":
+        fld = dipole_field_gpu(chi, spec.vsz, spec.bdir)
+    elif field == "host":
+        fld = dipole_field(chi, spec.vsz, spec.bdir)
+    else:
+        raise ValueError(f"field must be 'host' or 'gpu', not {field!r}")
     sd = float(np.std(fld[mask])) if mask.any() else 0.0
     fld = fld + rng.normal(0.0, spec.noise * sd, size=spec.shape)
     return {"chi": chi, "field": np.asfortranarray(fld), "mask": mask}

@@ -24,7 +24,10 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+from ._lib import QsmAdmissionError
 from .batch import Batch
+
+QSM_EFULL = 5
 
 
 @dataclass
@@ -39,7 +42,7 @@ class Request:
     alpha: float
     maxiter: int
     tol: float = 0.0
-    kz: int = 16
+    kz: int | None = None    # z-chunk depth; None: the tuned one
     # filled in by the server
     admitted: float = float("nan")
     done: float = float("nan")
@@ -86,13 +89,20 @@ class Server:
             # admit in arrival order while the head fits
             while waiting:
                 r = waiting[0]
+                if r.kz is None:
+                    r.kz = self.b.kz_for(r.kind, r.d2f.shape)
                 need = self.b.volume_bytes(r.kind, r.d2f.shape, r.kz, r.maxiter)
+                if need > self._budget():
+                    raise QsmAdmissionError(QSM_EFULL, f"request {r.rid} needs {need} bytes, "
+                                            f"more than the whole budget {self._budget()}")
                 if self.b.used_bytes + need > self._budget():
                     break
                 try:
                     slot = self.b.add(r.kind, r.d2f, r.mask, r.vsz, r.lam, r.alpha,
                                       r.maxiter, r.tol, r.kz)
-                except Exception:
+                except QsmAdmissionError:
+                    if not active:
+                        raise  # refused with nothing resident: it never fits
                     break
                 waiting.popleft()
                 r.admitted = clock() - t0

@@ -28,7 +28,8 @@ def _mask(shape, seed):
 def _same(a, b):
     a, b = np.asarray(a), np.asarray(b)
     assert a.shape == b.shape and a.dtype == b.dtype
-    np.testing.assert_array_equal(a.view(np.uint8), b.view(np.uint8))
+    np.testing.assert_array_equal(np.ascontiguousarray(a).view(np.uint8),
+                                  np.ascontiguousarray(b).view(np.uint8))
 
 
 @pytest.mark.parametrize("shape", SHAPES)

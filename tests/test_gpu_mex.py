@@ -22,7 +22,8 @@ H = (0.8, 0.9, 1.1)
 
 def _same(a, b):
     assert a.shape == b.shape and a.dtype == b.dtype
-    np.testing.assert_array_equal(a.view(np.uint8), b.view(np.uint8))
+    np.testing.assert_array_equal(np.ascontiguousarray(a).view(np.uint8),
+                                  np.ascontiguousarray(b).view(np.uint8))
 
 
 def _expected_rows(maxiter, verbose, iters):
