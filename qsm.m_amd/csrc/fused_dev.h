@@ -39,14 +39,23 @@ __device__ __forceinline__ double bld<double>(rsrc_t r, uint32_t vo, uint32_t so
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, 0));
 }
 
+// Cache policy of the state stores (the instruction's aux operand; on
+// gfx950 bit 1 is the nontemporal hint).  The out bank is not read again
+// in the same pass.
+#ifndef QSM_STORE_AUX
+#define QSM_STORE_AUX 0
+#endif
+
 __device__ __forceinline__ void bst(float v, rsrc_t r, uint32_t vo, uint32_t so)
 {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)vo, (int)so, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)vo, (int)so,
+                                          QSM_STORE_AUX);
 }
 
 __device__ __forceinline__ void bst(double v, rsrc_t r, uint32_t vo, uint32_t so)
 {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)vo, (int)so, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)vo, (int)so,
+                                          QSM_STORE_AUX);
 }
 
 __device__ __forceinline__ uint8_t bld_u8(rsrc_t r, uint32_t vo)
@@ -76,13 +85,66 @@ struct Ctx {
     __device__ __forceinline__ uint32_t so(int a) const { return uint32_t(a) * nb; }
 };
 
+// Software pipelining of the column march: with prefetching on, a step
+// issues the loads of the next plane's own values before computing the
+// current one (1: own columns, 2: own and ring columns).
+#ifndef QSM_PREFETCH
+#define QSM_PREFETCH 0
+#endif
+
+// A column's own state at one plane (what pcol reads at the centre voxel).
+template <typename T>
+struct Cen {
+    T p1, p2, p3, q1, q2, q3, q4, q5, q6, chi, eta, w1, w2, w3;
+};
+
 template <typename T>
 struct ColQ {
     int l0;             // i + nx*j, or -1 when the column is outside the grid
     uint8_t mc;         // mask byte at the current plane kp
     T nzm, nzc;         // nu at kp-1, kp
     T p3m, q3m, q5m, q6m;  // z-backward neighbours (plane kp-1)
+    // prefetching only: masks at kp+1, kp+2, nu at kp+1, own state at kp
+    uint8_t m1, m2;
+    T nzn;
+    Cen<T> cc;
 };
+
+// Own state at byte offset vb for mask byte m: the dual state where either
+// mask bit is set (q only with wq), the primal state where bit 0 is set.
+template <typename T, bool TGV>
+__device__ __forceinline__ void load_cen(const Ctx<T> &c, uint32_t vb, uint8_t m, bool wq,
+                                         Cen<T> &x)
+{
+    using I = Ix<TGV>;
+    x = Cen<T>{};
+    if (m & 3) {
+        x.p1 = bld<T>(c.in, vb, c.so(I::P1));
+        x.p2 = bld<T>(c.in, vb, c.so(I::P2));
+        x.p3 = bld<T>(c.in, vb, c.so(I::P3));
+        if constexpr (TGV) {
+            if (wq) {
+                x.q1 = bld<T>(c.in, vb, c.so(I::Q1));
+                x.q2 = bld<T>(c.in, vb, c.so(I::Q2));
+                x.q3 = bld<T>(c.in, vb, c.so(I::Q3));
+                x.q4 = bld<T>(c.in, vb, c.so(I::Q4));
+                x.q5 = bld<T>(c.in, vb, c.so(I::Q5));
+                x.q6 = bld<T>(c.in, vb, c.so(I::Q6));
+            }
+        }
+    }
+    if (m & 1) {
+        x.eta = bld<T>(c.in, vb, c.so(I::ETA));
+        x.chi = bld<T>(c.in, vb, c.so(I::CHI));
+        if constexpr (TGV) {
+            if (wq) {
+                x.w1 = bld<T>(c.in, vb, c.so(I::W1));
+                x.w2 = bld<T>(c.in, vb, c.so(I::W2));
+                x.w3 = bld<T>(c.in, vb, c.so(I::W3));
+            }
+        }
+    }
+}
 
 template <typename T>
 struct POut {
@@ -96,7 +158,7 @@ struct POut {
 // state never leaves zero) and advances the column's queues to kp+1.
 // wq = false (TGV ring cells on the -x/-y sides) skips w and q: the dual
 // reads w_ only on the + sides.
-template <typename T, bool TGV>
+template <typename T, bool TGV, bool PF>
 __device__ __forceinline__ void pcol(const Ctx<T> &c, const PrimalCoef<T> &pc,
                                      int kp, bool wq, ColQ<T> &q,
                                      DualVox<T> &dv, bool &pact, POut<T> &o)
@@ -106,30 +168,45 @@ __device__ __forceinline__ void pcol(const Ctx<T> &c, const PrimalCoef<T> &pc,
     const uint8_t mc = q.mc;
     uint8_t mn = 0;
     T np = T(0);
+    uint8_t m3 = 0;   // prefetching: mask at kp+3, nu at kp+2, state at kp+1
+    T nu2 = T(0);
+    Cen<T> cn{};
+    Cen<T> cur{};
     dv = DualVox<T>{};
     o = POut<T>{};
     pact = false;
     if (q.l0 >= 0) {
         const int l = q.l0 + c.sk * kp;
         const uint32_t vb = uint32_t(l) * S;
-        if (kp + 1 < c.nz) {
-            mn = bld_u8(c.mk, uint32_t(l + c.sk));
-            if (mn & 2)
-                np = bld<T>(c.in, vb + c.skb, c.so(I::NU));
+        if constexpr (PF) {
+            if (kp + 3 < c.nz)
+                m3 = bld_u8(c.mk, uint32_t(l + 3 * c.sk));
+            if (q.m2 & 2)
+                nu2 = bld<T>(c.in, vb + 2 * c.skb, c.so(I::NU));
+            load_cen<T, TGV>(c, vb + c.skb, q.m1, wq, cn);
+            np = q.nzn;
+            cur = q.cc;
+        } else {
+            if (kp + 1 < c.nz) {
+                mn = bld_u8(c.mk, uint32_t(l + c.sk));
+                if (mn & 2)
+                    np = bld<T>(c.in, vb + c.skb, c.so(I::NU));
+            }
+            load_cen<T, TGV>(c, vb, mc, wq, cur);
         }
         if (mc & 3) {
             dv.nu = q.nzc;
-            dv.p1 = bld<T>(c.in, vb, c.so(I::P1));
-            dv.p2 = bld<T>(c.in, vb, c.so(I::P2));
-            dv.p3 = bld<T>(c.in, vb, c.so(I::P3));
+            dv.p1 = cur.p1;
+            dv.p2 = cur.p2;
+            dv.p3 = cur.p3;
             if constexpr (TGV) {
                 if (wq) {
-                    dv.q1 = bld<T>(c.in, vb, c.so(I::Q1));
-                    dv.q2 = bld<T>(c.in, vb, c.so(I::Q2));
-                    dv.q3 = bld<T>(c.in, vb, c.so(I::Q3));
-                    dv.q4 = bld<T>(c.in, vb, c.so(I::Q4));
-                    dv.q5 = bld<T>(c.in, vb, c.so(I::Q5));
-                    dv.q6 = bld<T>(c.in, vb, c.so(I::Q6));
+                    dv.q1 = cur.q1;
+                    dv.q2 = cur.q2;
+                    dv.q3 = cur.q3;
+                    dv.q4 = cur.q4;
+                    dv.q5 = cur.q5;
+                    dv.q6 = cur.q6;
                 }
             }
         }
@@ -158,16 +235,16 @@ __device__ __forceinline__ void pcol(const Ctx<T> &c, const PrimalCoef<T> &pc,
                     pi.q6c = dv.q6; pi.q6zm = q.q6m;
                 }
             }
-            o.eta_o = bld<T>(c.in, vb, c.so(I::ETA));
-            o.chi_o = bld<T>(c.in, vb, c.so(I::CHI));
+            o.eta_o = cur.eta;
+            o.chi_o = cur.chi;
             primal_eta_chi(pc, pi, o.eta_o, o.chi_o, o.eta, o.chi);
             o.eb = extrap(o.eta, o.eta_o);
             o.cb = extrap(o.chi, o.chi_o);
             if constexpr (TGV) {
                 if (wq) {
-                    const T a1 = bld<T>(c.in, vb, c.so(I::W1));
-                    const T a2 = bld<T>(c.in, vb, c.so(I::W2));
-                    const T a3 = bld<T>(c.in, vb, c.so(I::W3));
+                    const T a1 = cur.w1;
+                    const T a2 = cur.w2;
+                    const T a3 = cur.w3;
                     o.w1 = a1; o.w2 = a2; o.w3 = a3;
                     primal_w(pc, pi, o.w1, o.w2, o.w3);
                     o.b1 = extrap(o.w1, a1);
@@ -177,7 +254,15 @@ __device__ __forceinline__ void pcol(const Ctx<T> &c, const PrimalCoef<T> &pc,
             }
         }
     }
-    q.mc = mn;
+    if constexpr (PF) {
+        q.mc = q.m1;
+        q.m1 = q.m2;
+        q.m2 = m3;
+        q.nzn = nu2;
+        q.cc = cn;
+    } else {
+        q.mc = mn;
+    }
     q.nzm = q.nzc;
     q.nzc = np;
     q.p3m = dv.p3;
@@ -189,13 +274,14 @@ __device__ __forceinline__ void pcol(const Ctx<T> &c, const PrimalCoef<T> &pc,
 }
 
 // Loads a column's queues for a march whose first primal plane is kp0.
-template <typename T, bool TGV>
+template <typename T, bool TGV, bool PF>
 __device__ __forceinline__ void col_init(const Ctx<T> &c, int kp0, bool wq, ColQ<T> &q)
 {
     using I = Ix<TGV>;
     constexpr uint32_t S = sizeof(T);
-    q.mc = 0;
-    q.nzm = q.nzc = q.p3m = q.q3m = q.q5m = q.q6m = T(0);
+    q.mc = q.m1 = q.m2 = 0;
+    q.nzm = q.nzc = q.nzn = q.p3m = q.q3m = q.q5m = q.q6m = T(0);
+    q.cc = Cen<T>{};
     if (q.l0 < 0)
         return;
     const int l = q.l0 + c.sk * kp0;
@@ -203,6 +289,15 @@ __device__ __forceinline__ void col_init(const Ctx<T> &c, int kp0, bool wq, ColQ
     q.mc = bld_u8(c.mk, uint32_t(l));
     if (q.mc & 2)
         q.nzc = bld<T>(c.in, vb, c.so(I::NU));
+    if constexpr (PF) {
+        if (kp0 + 1 < c.nz)
+            q.m1 = bld_u8(c.mk, uint32_t(l + c.sk));
+        if (kp0 + 2 < c.nz)
+            q.m2 = bld_u8(c.mk, uint32_t(l + 2 * c.sk));
+        if (q.m1 & 2)
+            q.nzn = bld<T>(c.in, vb + c.skb, c.so(I::NU));
+        load_cen<T, TGV>(c, vb, q.mc, wq, q.cc);
+    }
     if (kp0 >= 1) {
         const uint8_t mb = bld_u8(c.mk, uint32_t(l - c.sk));
         const uint32_t vm = vb - c.skb;
@@ -299,24 +394,30 @@ __device__ __forceinline__ void fused_tile(FusedLds<T, TGV, TX, TY> &L, const Ct
     const int lr = (ry + 1) * LW + (rx + 1);
     const bool rplus = rx == TX || ry == TY;  // +x / +y side of the ring
 
-    col_init<T, TGV>(c, k0 - 1, true, qo);
+    constexpr bool PFO = QSM_PREFETCH >= 1, PFR = QSM_PREFETCH >= 2;
+    col_init<T, TGV, PFO>(c, k0 - 1, true, qo);
     if (t < NRING)
-        col_init<T, TGV>(c, k0 - 1, rplus, qr);
+        col_init<T, TGV, PFR>(c, k0 - 1, rplus, qr);
 
     // own-column extrapolation queues and the dual-side values one plane back
     T cbm = T(0), cbc = T(0), ebm = T(0), ebc = T(0);
     T wc1 = T(0), wc2 = T(0), wc3 = T(0);
     DualVox<T> dprev{};
     uint8_t mprev = 0;
+    T fprev = T(0);  // f at the dual plane, loaded one step ahead
 
     for (int kp = k0 - 1; kp <= k1; ++kp) {
         const int slot = kp & 1;
         // ---- primal at plane kp: own column and (some threads) a ring cell
         const uint8_t mcur = qo.mc;
+        // f of plane kp, consumed by the dual of the next step
+        T fcur = T(0);
+        if (kp >= k0 && kp < k1 && (mcur & 2))
+            fcur = bld<T>(c.f, uint32_t(qo.l0 + c.sk * kp) * S, 0);
         DualVox<T> dv;
         bool pact;
         POut<T> o;
-        pcol<T, TGV>(c, pc, kp, true, qo, dv, pact, o);
+        pcol<T, TGV, PFO>(c, pc, kp, true, qo, dv, pact, o);
         lds[slot][0][lo] = o.cb;
         lds[slot][1][lo] = o.eb;
         if constexpr (TGV) {
@@ -340,7 +441,7 @@ __device__ __forceinline__ void fused_tile(FusedLds<T, TGV, TX, TY> &L, const Ct
             DualVox<T> rdv;
             bool ract;
             POut<T> ro;
-            pcol<T, TGV>(c, pc, kp, rplus, qr, rdv, ract, ro);
+            pcol<T, TGV, PFR>(c, pc, kp, rplus, qr, rdv, ract, ro);
             lds[slot][0][lr] = ro.cb;
             lds[slot][1][lr] = ro.eb;
             if constexpr (TGV) {
@@ -361,7 +462,7 @@ __device__ __forceinline__ void fused_tile(FusedLds<T, TGV, TX, TY> &L, const Ct
             const Star<T> es{ebc, lds[ps][1][lo - 1], lds[ps][1][lo + 1],
                              lds[ps][1][lo - LW], lds[ps][1][lo + LW], ebm, o.eb};
             DualVox<T> d = dprev;
-            d.nu = dual_nu(dc, d.nu, es, cs, bld<T>(c.f, vb, 0));
+            d.nu = dual_nu(dc, d.nu, es, cs, fprev);
             if constexpr (TGV) {
                 tgv_dual_pq(dc, d, cs,
                             wc1, lds[ps][2][lo + 1], lds[ps][2][lo + LW], o.b1,
@@ -386,6 +487,7 @@ __device__ __forceinline__ void fused_tile(FusedLds<T, TGV, TX, TY> &L, const Ct
         wc1 = o.b1; wc2 = o.b2; wc3 = o.b3;
         dprev = dv;
         mprev = mcur;
+        fprev = fcur;
         __syncthreads();
     }
 
