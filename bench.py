@@ -234,6 +234,49 @@ def serve_leg(n: int, rate: float):
     }
 
 
+def dipole_leg(shape=(256, 256, 176), reps=10):
+    """FFT dipole convolution of a C3-sized volume (zero padding to
+    nextfastfft(2*dims)), both precisions, timed with HIP events on its
+    stream over `reps` calls after warm-up (plans cached, inputs resident)."""
+    import torch
+    from qsm_amd import dipole
+
+    pad = tuple(n // 2 for n in shape)
+    M, _ = dipole.pad_size(shape, pad)
+    Mt = int(np.prod(M))
+    N = int(np.prod(shape))
+    out = {"shape": list(shape), "padded": list(M)}
+    st = torch.cuda.Stream()
+    for dt, name in ((torch.float32, "f32"), (torch.float64, "f64")):
+        s = 4 if dt == torch.float32 else 8
+        c = 2 * s
+        with torch.cuda.stream(st):
+            chi = torch.randn(shape[::-1], dtype=dt, device="cuda").permute(2, 1, 0)
+            fld = torch.empty_like(chi)
+            work = torch.empty(Mt * c, dtype=torch.uint8, device="cuda")
+        st.synchronize()
+        args = ((1.0, 1.0, 1.0), (0.0, 0.0, 1.0), pad, fld, work, st)
+        dipole.dipole_conv_dev(chi, *args)
+        st.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            dipole.dipole_conv_dev(chi, *args)
+        e1.record(st)
+        e1.synchronize()
+        t = e0.elapsed_time(e1) / 1e3 / reps
+        rows = M[0] * shape[1] * shape[2]   # padded-x rows that hold data
+        planes = M[0] * M[1] * shape[2]     # data planes, all rows
+        actual = 2 * N * s + 4 * rows * c + 4 * planes * c
+        alg = 12 * c * Mt                    # 2 transforms x 3 axis passes x 2*c*M
+        out[name] = {"ms": t * 1e3, "actual_bytes": actual, "actual_gbs": actual / t / 1e9,
+                     "algorithmic_bytes": alg, "algorithmic_gbs": alg / t / 1e9}
+        del chi, fld, work
+    torch.cuda.empty_cache()
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[1])
     ap.add_argument("--gpus", type=int, default=1)
@@ -248,6 +291,8 @@ def main(argv=None) -> int:
     ap.add_argument("--serve-requests", type=int, default=40,
                     help="requests of the C5 serving leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--no-dipole", dest="dipole", action="store_false",
+                    help="skip the FFT dipole convolution leg")
     a = ap.parse_args(argv)
 
     if a.gpus != 1:
@@ -337,6 +382,10 @@ def main(argv=None) -> int:
         out["roofline"]["frac_of_copy_peak"] = ach / out["copy_peak_gbs"]
     except Exception as e:  # measurement aid only
         log(f"[bench] copy peak failed: {e}")
+    if a.dipole:
+        out["dipole_conv"] = dipole_leg()
+        log(f"[bench] dipole conv f32 {out['dipole_conv']['f32']['ms']:.3f} ms, "
+            f"f64 {out['dipole_conv']['f64']['ms']:.3f} ms")
     if a.batch_iters > 0:
         out["batch"] = batch_leg(a.batch_iters)
         log(f"[bench] C4 batch: {out['batch']['voxel_iters_per_s']:.3e} voxel-iters/s")

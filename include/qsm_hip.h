@@ -291,7 +291,8 @@ size_t qsm_nextfastfft(size_t n);
 /* In-place 3-D complex transform of an interleaved (re, im) array of
  * M[0] x M[1] x M[2] (x fastest): fft3.m:5 (inverse = 0) or ifft3.m:5
  * (inverse = 1, including 1/M per axis).  Lengths must be 2,3,5,7-smooth
- * and at most 2048 (double) / 4096 (single).  Waits for the stream. */
+ * and at most 2048 (double) / 4096 (single).  Asynchronous on the stream
+ * (plans are built once per device and length, then cached). */
 int qsm_fft3_dev(int dtype, void *data, const size_t M[3], int inverse,
                  void *stream);
 
@@ -307,8 +308,9 @@ size_t qsm_dipole_work_bytes(int dtype, const size_t sz[3], const size_t pad[3])
  * the forward model the toolbox writes with dipoleKernel.m:56-66 (k-space
  * kernel D = 1/3 - (k.b)^2/|k|^2, D(0) = 0, bdir normalised) and
  * fft3/ifft3 (e.g. ndi.m:100-101, medi.m:151).  D is generated on the fly.
- * Device pointers; work: qsm_dipole_work_bytes bytes.  Waits for the
- * stream. */
+ * Device pointers; work: qsm_dipole_work_bytes bytes (no clearing
+ * needed).  Asynchronous on the stream; work must stay allocated until the
+ * stream has run the call. */
 int qsm_dipole_conv_dev(int dtype, const void *chi, const size_t sz[3],
                         const double h[3], const double bdir[3],
                         const size_t pad[3], void *field, void *work,

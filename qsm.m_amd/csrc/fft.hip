@@ -11,12 +11,15 @@
 // (radices 8, 7, 5, 4, 3, 2; twiddles from a table computed in double from
 // exact rationals, cast for single precision), and stores.  The dipole
 // convolution uses five passes: x (zero padding fused into the load, only
-// rows that hold data), y (only planes that hold data), z forward * D(k) *
-// z inverse in one pass (D generated in-kernel), y inverse, x inverse (crop,
-// real part and 1/M scaling fused into the store).  The passes are
-// HBM-bound: 2*M*sizeof(complex) bytes per pencil.
+// rows that hold data), y (only planes that hold data, zero rows not read),
+// z forward * D(k) * z inverse in one pass (D generated in-kernel; only the
+// data planes read and stored), y inverse (only data rows stored), x
+// inverse (crop, real part and 1/M scaling fused into the store).  The
+// padding is never materialised.  The passes are HBM-bound.
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -302,10 +305,13 @@ __global__ __launch_bounds__(FFT_THREADS) void dip_x_fwd(
     }
 }
 
-// y pass over planes k in [k0, k0 + nk) (forward or inverse).
+// y pass over planes k in [k0, k0 + nk) (forward or inverse).  Rows
+// outside [ylo, yhi) are zero on input (not read); only rows in
+// [slo, shi) are stored (the later passes read no others).
 template <typename T>
 __global__ __launch_bounds__(FFT_THREADS) void dip_y(
-    Cx<T> *data, Vol v, int k0, int P, const Plan *pl, const Cx<double> *tw, T sgn)
+    Cx<T> *data, Vol v, int k0, int P, const Plan *pl, const Cx<double> *tw, T sgn,
+    int ylo, int yhi, int slo, int shi)
 {
     extern __shared__ unsigned char smem[];
     Cx<T> *A = reinterpret_cast<Cx<T> *>(smem);
@@ -318,13 +324,14 @@ __global__ __launch_bounds__(FFT_THREADS) void dip_y(
     const size_t base = size_t(x0) + size_t(k) * v.M0 * v.M1;
     for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
         const int i = e / P, p = e % P;
-        A[i * P + p] = p < np ? data[base + p + size_t(i) * v.M0] : Cx<T>{T(0), T(0)};
+        A[i * P + p] = (p < np && i >= ylo && i < yhi) ? data[base + p + size_t(i) * v.M0]
+                                                       : Cx<T>{T(0), T(0)};
     }
     __syncthreads();
     Cx<T> *R = stockham(A, B, P, pl, tw, sgn);
     for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
         const int i = e / P, p = e % P;
-        if (p < np)
+        if (p < np && i >= slo && i < shi)
             data[base + p + size_t(i) * v.M0] = R[i * P + p];
     }
 }
@@ -339,10 +346,13 @@ __device__ __forceinline__ double kcoord(int m, int n, double h)
 }
 
 // z pass: forward transform, multiply by D(k) / (M0*M1*M2), inverse.
+// Only planes [zlo, zhi) hold data on input and only they are needed on
+// output: the others are neither read (zero) nor stored.
 template <typename T>
 __global__ __launch_bounds__(FFT_THREADS) void dip_z(
     Cx<T> *data, Vol v, int P, const Plan *pl, const Cx<double> *tw,
-    double h0, double h1, double h2, double b0, double b1, double b2, double inv_n)
+    double h0, double h1, double h2, double b0, double b1, double b2, double inv_n,
+    int zlo, int zhi)
 {
     extern __shared__ unsigned char smem[];
     Cx<T> *A = reinterpret_cast<Cx<T> *>(smem);
@@ -356,7 +366,8 @@ __global__ __launch_bounds__(FFT_THREADS) void dip_z(
     const size_t base = size_t(x0) + size_t(j) * v.M0;
     for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
         const int i = e / P, p = e % P;
-        A[i * P + p] = p < np ? data[base + p + size_t(i) * plane] : Cx<T>{T(0), T(0)};
+        A[i * P + p] = (p < np && i >= zlo && i < zhi) ? data[base + p + size_t(i) * plane]
+                                                       : Cx<T>{T(0), T(0)};
     }
     __syncthreads();
     Cx<T> *R = stockham(A, B, P, pl, tw, T(-1));
@@ -380,7 +391,7 @@ __global__ __launch_bounds__(FFT_THREADS) void dip_z(
     Cx<T> *Q = stockham(R, S, P, pl, tw, T(1));
     for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
         const int i = e / P, p = e % P;
-        if (p < np)
+        if (p < np && i >= zlo && i < zhi)
             data[base + p + size_t(i) * plane] = Q[i * P + p];
     }
 }
@@ -486,23 +497,56 @@ struct AxisPlan {
     const Cx<double> *dtw = nullptr;  // device twiddle table (after it)
 };
 
-// Factors M, builds its twiddles and uploads plan + table into buf.
+// Plans are cached per (device, length): built and uploaded on first use,
+// kept until the process ends (a few KB each), so the transforms launch
+// without host synchronisation after the first call.  The device memory is
+// deliberately never freed (freeing at static destruction would race the
+// HIP runtime's own teardown).
+struct PlanCache {
+    std::mutex mu;
+    std::map<std::pair<int, size_t>, AxisPlan> plans;
+};
+
+PlanCache &plan_cache()
+{
+    static PlanCache *c = new PlanCache;  // never destroyed, like its plans
+    return *c;
+}
+
+// Factors M, builds its twiddles and uploads plan + table (once per device).
 template <typename T>
-int make_axis(AxisPlan &ap, DevBuf &buf, size_t M, const char *what)
+int make_axis(AxisPlan &ap, size_t M, const char *what)
 {
     QSM_ARG(M >= 1 && 2 * M * sizeof(Cx<T>) <= 65536, "%s %zu too long (at most %zu)",
             what, M, 65536 / (2 * sizeof(Cx<T>)));
-    QSM_ARG(factor(int(M), ap.pl), "%s %zu is not 2,3,5,7-smooth", what, M);
-    const std::vector<Cx<double>> t = twiddles(ap.pl);
+    int dev = 0;
+    QSM_HIP(hipGetDevice(&dev));
+    PlanCache &pc = plan_cache();
+    std::lock_guard<std::mutex> lock(pc.mu);
+    const auto key = std::make_pair(dev, M);
+    const auto it = pc.plans.find(key);
+    if (it != pc.plans.end()) {
+        ap = it->second;
+        return QSM_OK;
+    }
+    AxisPlan np;
+    QSM_ARG(factor(int(M), np.pl), "%s %zu is not 2,3,5,7-smooth", what, M);
+    const std::vector<Cx<double>> t = twiddles(np.pl);
     // plan at the head, the table after it on a 256-byte boundary
     constexpr size_t head = (sizeof(Plan) + 255) / 256 * 256;
-    int rc = buf.alloc(head + t.size() * sizeof(t[0]));
-    if (rc) return rc;
-    QSM_HIP(hipMemcpy(buf.p, &ap.pl, sizeof(Plan), hipMemcpyHostToDevice));
-    QSM_HIP(hipMemcpy(static_cast<char *>(buf.p) + head, t.data(), t.size() * sizeof(t[0]),
-                      hipMemcpyHostToDevice));
-    ap.dpl = static_cast<const Plan *>(buf.p);
-    ap.dtw = reinterpret_cast<const Cx<double> *>(static_cast<char *>(buf.p) + head);
+    void *p = nullptr;
+    QSM_HIP(hipMalloc(&p, head + t.size() * sizeof(t[0])));
+    if (hipMemcpy(p, &np.pl, sizeof(Plan), hipMemcpyHostToDevice) != hipSuccess
+        || hipMemcpy(static_cast<char *>(p) + head, t.data(), t.size() * sizeof(t[0]),
+                     hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(p);
+        set_error("FFT plan upload failed");
+        return QSM_EHIP;
+    }
+    np.dpl = static_cast<const Plan *>(p);
+    np.dtw = reinterpret_cast<const Cx<double> *>(static_cast<char *>(p) + head);
+    pc.plans.emplace(key, np);
+    ap = np;
     return QSM_OK;
 }
 
@@ -510,10 +554,9 @@ template <typename T>
 int fft3_dev_t(void *data, const size_t Msz[3], int inverse, hipStream_t st)
 {
     AxisPlan ap[3];
-    DevBuf buf[3];
     int rc;
     for (int a = 0; a < 3; ++a)
-        if ((rc = make_axis<T>(ap[a], buf[a], Msz[a], "FFT length"))) return rc;
+        if ((rc = make_axis<T>(ap[a], Msz[a], "FFT length"))) return rc;
     const Vol v{int(Msz[0]), int(Msz[1]), int(Msz[2])};
     const T sgn = inverse ? T(1) : T(-1);
     // fft3.m:5 and ifft3.m:5 apply axis 3, then 2, then 1
@@ -530,7 +573,6 @@ int fft3_dev_t(void *data, const size_t Msz[3], int inverse, hipStream_t st)
             static_cast<Cx<T> *>(data), v, a, P, ap[a].dpl, ap[a].dtw, sgn, scale);
         QSM_HIP(hipGetLastError());
     }
-    QSM_HIP(hipStreamSynchronize(st));
     return QSM_OK;
 }
 
@@ -541,16 +583,16 @@ int dipole_conv_dev_t(const void *chi, const size_t sz[3], const double h[3],
 {
     int rc;
     AxisPlan ap[3];
-    DevBuf buf[3];
     for (int a = 0; a < 3; ++a)
-        if ((rc = make_axis<T>(ap[a], buf[a], M[a], "padded length"))) return rc;
+        if ((rc = make_axis<T>(ap[a], M[a], "padded length"))) return rc;
     const Vol v{int(M[0]), int(M[1]), int(M[2])};
     const int nx = int(sz[0]), ny = int(sz[1]), nz = int(sz[2]);
     const int o0 = int(off[0]), o1 = int(off[1]), o2 = int(off[2]);
     Cx<T> *d = static_cast<Cx<T> *>(work);
     const size_t total = size_t(v.M0) * v.M1 * v.M2;
-    // rows outside the data region stay zero through the x pass
-    QSM_HIP(hipMemsetAsync(d, 0, total * sizeof(Cx<T>), st));
+    // The padding is never stored: the x pass writes the rows that hold
+    // data, and the y and z passes treat the other rows and planes as zero
+    // on load.  The work space therefore needs no clearing.
     // normalised b (dipoleKernel.m:35-38)
     const double bn = std::sqrt(bdir[0] * bdir[0] + bdir[1] * bdir[1] + bdir[2] * bdir[2]);
     QSM_ARG(bn > 0.0, "B0 direction must be non-zero");
@@ -569,7 +611,7 @@ int dipole_conv_dev_t(const void *chi, const size_t sz[3], const double h[3],
         const Plan &pl = ap[1].pl;
         const int P = pencils(pl.M, cs);
         dip_y<T><<<((v.M0 + P - 1) / P) * nz, FFT_THREADS, 2 * size_t(P) * pl.M * cs, st>>>(
-            d, v, o2, P, ap[1].dpl, ap[1].dtw, T(-1));
+            d, v, o2, P, ap[1].dpl, ap[1].dtw, T(-1), o1, o1 + ny, 0, v.M1);
         QSM_HIP(hipGetLastError());
     }
     {   // z forward * D * z inverse, scaled by 1/(M0 M1 M2)
@@ -577,14 +619,14 @@ int dipole_conv_dev_t(const void *chi, const size_t sz[3], const double h[3],
         const int P = pencils(pl.M, cs);
         dip_z<T><<<((v.M0 + P - 1) / P) * v.M1, FFT_THREADS, 2 * size_t(P) * pl.M * cs, st>>>(
             d, v, P, ap[2].dpl, ap[2].dtw, h[0], h[1], h[2],
-            b[0], b[1], b[2], 1.0 / double(total));
+            b[0], b[1], b[2], 1.0 / double(total), o2, o2 + nz);
         QSM_HIP(hipGetLastError());
     }
     {   // y inverse on output planes
         const Plan &pl = ap[1].pl;
         const int P = pencils(pl.M, cs);
         dip_y<T><<<((v.M0 + P - 1) / P) * nz, FFT_THREADS, 2 * size_t(P) * pl.M * cs, st>>>(
-            d, v, o2, P, ap[1].dpl, ap[1].dtw, T(1));
+            d, v, o2, P, ap[1].dpl, ap[1].dtw, T(1), 0, v.M1, o1, o1 + ny);
         QSM_HIP(hipGetLastError());
     }
     {   // x inverse on output rows, crop + real part
@@ -594,7 +636,6 @@ int dipole_conv_dev_t(const void *chi, const size_t sz[3], const double h[3],
             d, static_cast<T *>(field), v, nx, ny, nz, o0, o1, o2, P, ap[0].dpl, ap[0].dtw);
         QSM_HIP(hipGetLastError());
     }
-    QSM_HIP(hipStreamSynchronize(st));
     return QSM_OK;
 }
 

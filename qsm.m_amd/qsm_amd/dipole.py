@@ -57,11 +57,18 @@ def dipole_conv_dev(chi, vsz, bdir, pad, out, work=None, stream=None):
     shape = tuple(int(s) for s in chi.shape)
     code = _lib.F64 if chi.dtype == torch.float64 else _lib.F32
     nb = int(lib().qsm_dipole_work_bytes(code, sz3(shape), sz3(pad)))
-    if work is None:
+    own = work is None
+    if own:
         work = torch.empty(nb, dtype=torch.uint8, device=chi.device)
+    elif work.numel() * work.element_size() < nb:
+        raise ValueError(f"work space too small: {nb} bytes needed")
     sp = stream.cuda_stream if stream is not None else None
     check(lib().qsm_dipole_conv_dev(code, chi.data_ptr(), sz3(shape), h3(vsz), _d3(bdir),
                                     sz3(pad), out.data_ptr(), work.data_ptr(), sp))
+    if own and stream is not None:
+        # the call is asynchronous: keep the block from being reused by
+        # other streams before this one has run it
+        work.record_stream(stream)
     return out
 
 

@@ -73,3 +73,31 @@ def test_harness_field_on_gpu_equals_host(cuda):
     ph = phantom.make(spec)
     got = dipole.dipole_conv(ph["chi"], spec.vsz, spec.bdir, [s // 2 for s in spec.shape])
     assert np.linalg.norm(got - ph["field"]) / np.linalg.norm(ph["field"]) < 1e-12
+
+
+@pytest.mark.parametrize("dt", ["float64", "float32"])
+def test_dipole_conv_device_form_on_side_stream(cuda, dt):
+    """qsm_dipole_conv_dev is asynchronous on its stream; twice in a row on
+    a side stream (cached plans, work space reused uncleared) it matches the
+    host form bitwise."""
+    import torch
+
+    rng = np.random.default_rng(9)
+    shape, vsz, bdir, pad = (26, 20, 18), (0.9, 1.0, 1.1), (0.1, 0.2, 0.95), (13, 10, 9)
+    chi = np.asfortranarray(rng.standard_normal(shape).astype(dt))
+    want = dipole.dipole_conv(chi, vsz, bdir, pad)
+    st = torch.cuda.Stream()
+    with torch.cuda.stream(st):
+        c = torch.from_numpy(np.ascontiguousarray(chi.reshape(-1, order="F"))).to("cuda")
+        c = c.view(shape[::-1]).permute(2, 1, 0)
+        out = torch.empty_like(c)
+        M, _ = dipole.pad_size(shape, pad)
+        work = torch.full((int(np.prod(M)) * 2 * c.element_size(),), 255, dtype=torch.uint8,
+                          device="cuda")
+    for _ in range(2):
+        dipole.dipole_conv_dev(c, vsz, bdir, pad, out, work=work, stream=st)
+        st.synchronize()
+        got = out.permute(2, 1, 0).contiguous().cpu().numpy().reshape(-1)
+        got = got.reshape(shape, order="F")
+        assert got.dtype == want.dtype
+        np.testing.assert_array_equal(got, want)
