@@ -164,7 +164,7 @@ __device__ __forceinline__ void small_dft(Cx<T> *v, T sgn)
 // One radix-R Stockham stage: A -> B (element i of pencil p at [i*P + p]).
 template <typename T, int R>
 __device__ __forceinline__ void stage(const Cx<T> *A, Cx<T> *B, int P, int M, int Ns,
-                                      const Cx<double> *__restrict__ tws, T sgn)
+                                      const Cx<T> *__restrict__ tws, T sgn)
 {
     const int nb = M / R;
     for (int b = threadIdx.x; b < nb * P; b += FFT_THREADS) {
@@ -174,8 +174,8 @@ __device__ __forceinline__ void stage(const Cx<T> *A, Cx<T> *B, int P, int M, in
         v[0] = A[j * P + p];
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-            const Cx<double> w = tws[k * (R - 1) + (r - 1)];
-            const Cx<T> wt{T(w.re), sgn < 0 ? T(w.im) : T(-w.im)};
+            const Cx<T> w = tws[k * (R - 1) + (r - 1)];
+            const Cx<T> wt{w.re, sgn < 0 ? w.im : -w.im};
             v[r] = cmul(A[(j + r * nb) * P + p], wt);
         }
         small_dft<T, R>(v, sgn);
@@ -190,12 +190,12 @@ __device__ __forceinline__ void stage(const Cx<T> *A, Cx<T> *B, int P, int M, in
 // the result.  sgn = -1 forward, +1 inverse (twiddles conjugated).
 template <typename T>
 __device__ Cx<T> *stockham(Cx<T> *A, Cx<T> *B, int P, const Plan *__restrict__ pl,
-                           const Cx<double> *__restrict__ tw, T sgn)
+                           const Cx<T> *__restrict__ tw, T sgn)
 {
     const int M = pl->M, nst = pl->nst;
     for (int s = 0; s < nst; ++s) {
         const int R = pl->R[s], Ns = pl->Ns[s];
-        const Cx<double> *tws = tw + pl->tw[s];
+        const Cx<T> *tws = tw + pl->tw[s];
         switch (R) {
         case 2: stage<T, 2>(A, B, P, M, Ns, tws, sgn); break;
         case 3: stage<T, 3>(A, B, P, M, Ns, tws, sgn); break;
@@ -221,7 +221,7 @@ struct Vol {
 // ---- generic in-place pass (plain fft3 / ifft3) -------------------------
 template <typename T>
 __global__ __launch_bounds__(FFT_THREADS) void fft_pass_kernel(
-    Cx<T> *data, Vol v, int axis, int P, const Plan *pl, const Cx<double> *tw, T sgn, T scale)
+    Cx<T> *data, Vol v, int axis, int P, const Plan *pl, const Cx<T> *tw, T sgn, T scale)
 {
     extern __shared__ unsigned char smem[];
     Cx<T> *A = reinterpret_cast<Cx<T> *>(smem);
@@ -275,7 +275,7 @@ __global__ __launch_bounds__(FFT_THREADS) void fft_pass_kernel(
 template <typename T>
 __global__ __launch_bounds__(FFT_THREADS) void dip_x_fwd(
     Cx<T> *data, const T *chi, Vol v, int nx, int ny, int nz, int o0, int o1, int o2,
-    int P, const Plan *pl, const Cx<double> *tw)
+    int P, const Plan *pl, const Cx<T> *tw)
 {
     extern __shared__ unsigned char smem[];
     Cx<T> *A = reinterpret_cast<Cx<T> *>(smem);
@@ -310,7 +310,7 @@ __global__ __launch_bounds__(FFT_THREADS) void dip_x_fwd(
 // [slo, shi) are stored (the later passes read no others).
 template <typename T>
 __global__ __launch_bounds__(FFT_THREADS) void dip_y(
-    Cx<T> *data, Vol v, int k0, int P, const Plan *pl, const Cx<double> *tw, T sgn,
+    Cx<T> *data, Vol v, int k0, int P, const Plan *pl, const Cx<T> *tw, T sgn,
     int ylo, int yhi, int slo, int shi)
 {
     extern __shared__ unsigned char smem[];
@@ -350,7 +350,7 @@ __device__ __forceinline__ double kcoord(int m, int n, double h)
 // output: the others are neither read (zero) nor stored.
 template <typename T>
 __global__ __launch_bounds__(FFT_THREADS) void dip_z(
-    Cx<T> *data, Vol v, int P, const Plan *pl, const Cx<double> *tw,
+    Cx<T> *data, Vol v, int P, const Plan *pl, const Cx<T> *tw,
     double h0, double h1, double h2, double b0, double b1, double b2, double inv_n,
     int zlo, int zhi)
 {
@@ -401,7 +401,7 @@ __global__ __launch_bounds__(FFT_THREADS) void dip_z(
 template <typename T>
 __global__ __launch_bounds__(FFT_THREADS) void dip_x_inv(
     const Cx<T> *data, T *out, Vol v, int nx, int ny, int nz, int o0, int o1, int o2,
-    int P, const Plan *pl, const Cx<double> *tw)
+    int P, const Plan *pl, const Cx<T> *tw)
 {
     extern __shared__ unsigned char smem[];
     Cx<T> *A = reinterpret_cast<Cx<T> *>(smem);
@@ -494,10 +494,18 @@ int pencils(int M, size_t csize)
 struct AxisPlan {
     Plan pl;                    // host copy
     const Plan *dpl = nullptr;  // device copy (head of the buffer)
-    const Cx<double> *dtw = nullptr;  // device twiddle table (after it)
+    const void *dtw = nullptr;  // device twiddle table in the transform's
+                                // precision (after the plan); the single
+                                // table is the double one rounded
 };
 
-// Plans are cached per (device, length): built and uploaded on first use,
+template <typename T>
+const Cx<T> *tw_of(const AxisPlan &ap)
+{
+    return static_cast<const Cx<T> *>(ap.dtw);
+}
+
+// Plans are cached per (device, length, precision): built and uploaded on first use,
 // kept until the process ends (a few KB each), so the transforms launch
 // without host synchronisation after the first call.  The device memory is
 // deliberately never freed (freeing at static destruction would race the
@@ -523,7 +531,7 @@ int make_axis(AxisPlan &ap, size_t M, const char *what)
     QSM_HIP(hipGetDevice(&dev));
     PlanCache &pc = plan_cache();
     std::lock_guard<std::mutex> lock(pc.mu);
-    const auto key = std::make_pair(dev, M);
+    const auto key = std::make_pair(dev, M * 16 + sizeof(T));
     const auto it = pc.plans.find(key);
     if (it != pc.plans.end()) {
         ap = it->second;
@@ -531,7 +539,10 @@ int make_axis(AxisPlan &ap, size_t M, const char *what)
     }
     AxisPlan np;
     QSM_ARG(factor(int(M), np.pl), "%s %zu is not 2,3,5,7-smooth", what, M);
-    const std::vector<Cx<double>> t = twiddles(np.pl);
+    const std::vector<Cx<double>> td = twiddles(np.pl);
+    std::vector<Cx<T>> t(td.size());
+    for (size_t i = 0; i < td.size(); ++i)
+        t[i] = Cx<T>{T(td[i].re), T(td[i].im)};
     // plan at the head, the table after it on a 256-byte boundary
     constexpr size_t head = (sizeof(Plan) + 255) / 256 * 256;
     void *p = nullptr;
@@ -544,7 +555,7 @@ int make_axis(AxisPlan &ap, size_t M, const char *what)
         return QSM_EHIP;
     }
     np.dpl = static_cast<const Plan *>(p);
-    np.dtw = reinterpret_cast<const Cx<double> *>(static_cast<char *>(p) + head);
+    np.dtw = static_cast<const char *>(p) + head;
     pc.plans.emplace(key, np);
     ap = np;
     return QSM_OK;
@@ -570,7 +581,7 @@ int fft3_dev_t(void *data, const size_t Msz[3], int inverse, hipStream_t st)
         else nblk = ((v.M0 + P - 1) / P) * (a == 1 ? v.M2 : v.M1);
         const T scale = inverse ? T(1.0 / double(pl.M)) : T(1);
         fft_pass_kernel<T><<<nblk, FFT_THREADS, lds, st>>>(
-            static_cast<Cx<T> *>(data), v, a, P, ap[a].dpl, ap[a].dtw, sgn, scale);
+            static_cast<Cx<T> *>(data), v, a, P, ap[a].dpl, tw_of<T>(ap[a]), sgn, scale);
         QSM_HIP(hipGetLastError());
     }
     return QSM_OK;
@@ -604,21 +615,21 @@ int dipole_conv_dev_t(const void *chi, const size_t sz[3], const double h[3],
         const Plan &pl = ap[0].pl;
         const int P = pencils(pl.M, cs);
         dip_x_fwd<T><<<(ny * nz + P - 1) / P, FFT_THREADS, 2 * size_t(P) * pl.M * cs, st>>>(
-            d, static_cast<const T *>(chi), v, nx, ny, nz, o0, o1, o2, P, ap[0].dpl, ap[0].dtw);
+            d, static_cast<const T *>(chi), v, nx, ny, nz, o0, o1, o2, P, ap[0].dpl, tw_of<T>(ap[0]));
         QSM_HIP(hipGetLastError());
     }
     {   // y forward on data planes
         const Plan &pl = ap[1].pl;
         const int P = pencils(pl.M, cs);
         dip_y<T><<<((v.M0 + P - 1) / P) * nz, FFT_THREADS, 2 * size_t(P) * pl.M * cs, st>>>(
-            d, v, o2, P, ap[1].dpl, ap[1].dtw, T(-1), o1, o1 + ny, 0, v.M1);
+            d, v, o2, P, ap[1].dpl, tw_of<T>(ap[1]), T(-1), o1, o1 + ny, 0, v.M1);
         QSM_HIP(hipGetLastError());
     }
     {   // z forward * D * z inverse, scaled by 1/(M0 M1 M2)
         const Plan &pl = ap[2].pl;
         const int P = pencils(pl.M, cs);
         dip_z<T><<<((v.M0 + P - 1) / P) * v.M1, FFT_THREADS, 2 * size_t(P) * pl.M * cs, st>>>(
-            d, v, P, ap[2].dpl, ap[2].dtw, h[0], h[1], h[2],
+            d, v, P, ap[2].dpl, tw_of<T>(ap[2]), h[0], h[1], h[2],
             b[0], b[1], b[2], 1.0 / double(total), o2, o2 + nz);
         QSM_HIP(hipGetLastError());
     }
@@ -626,14 +637,14 @@ int dipole_conv_dev_t(const void *chi, const size_t sz[3], const double h[3],
         const Plan &pl = ap[1].pl;
         const int P = pencils(pl.M, cs);
         dip_y<T><<<((v.M0 + P - 1) / P) * nz, FFT_THREADS, 2 * size_t(P) * pl.M * cs, st>>>(
-            d, v, o2, P, ap[1].dpl, ap[1].dtw, T(1), 0, v.M1, o1, o1 + ny);
+            d, v, o2, P, ap[1].dpl, tw_of<T>(ap[1]), T(1), 0, v.M1, o1, o1 + ny);
         QSM_HIP(hipGetLastError());
     }
     {   // x inverse on output rows, crop + real part
         const Plan &pl = ap[0].pl;
         const int P = pencils(pl.M, cs);
         dip_x_inv<T><<<(ny * nz + P - 1) / P, FFT_THREADS, 2 * size_t(P) * pl.M * cs, st>>>(
-            d, static_cast<T *>(field), v, nx, ny, nz, o0, o1, o2, P, ap[0].dpl, ap[0].dtw);
+            d, static_cast<T *>(field), v, nx, ny, nz, o0, o1, o2, P, ap[0].dpl, tw_of<T>(ap[0]));
         QSM_HIP(hipGetLastError());
     }
     return QSM_OK;

@@ -91,12 +91,52 @@ struct Ctx {
 #ifndef QSM_PREFETCH
 #define QSM_PREFETCH 0
 #endif
+#ifndef QSM_NU_UNCOND
+#define QSM_NU_UNCOND 0
+#endif
 
 // A column's own state at one plane (what pcol reads at the centre voxel).
 template <typename T>
 struct Cen {
     T p1, p2, p3, q1, q2, q3, q4, q5, q6, chi, eta, w1, w2, w3;
 };
+
+// The x/y neighbours the primal reads around a primal-active voxel.
+template <typename T>
+struct Nbr {
+    T nxm, nxp, nym, nyp;             // nu at -x, +x, -y, +y
+    T p1xm, p2ym;
+    T q1xm, q2xm, q2ym, q3xm, q4ym, q5ym;
+};
+
+template <typename T, bool TGV>
+__device__ __forceinline__ void load_nbr(const Ctx<T> &c, uint32_t vb, uint8_t m, bool wq,
+                                         Nbr<T> &x)
+{
+    using I = Ix<TGV>;
+    constexpr uint32_t S = sizeof(T);
+    x = Nbr<T>{};
+    if (m & 1) {
+        // a primal-active voxel is interior: every neighbour offset is in range
+        const uint32_t vxm = vb - S, vym = vb - c.syb, vyp = vb + c.syb;
+        x.nxm = bld<T>(c.in, vxm, c.so(I::NU));
+        x.nxp = bld<T>(c.in, vb + S, c.so(I::NU));
+        x.nym = bld<T>(c.in, vym, c.so(I::NU));
+        x.nyp = bld<T>(c.in, vyp, c.so(I::NU));
+        x.p1xm = bld<T>(c.in, vxm, c.so(I::P1));
+        x.p2ym = bld<T>(c.in, vym, c.so(I::P2));
+        if constexpr (TGV) {
+            if (wq) {
+                x.q1xm = bld<T>(c.in, vxm, c.so(I::Q1));
+                x.q2xm = bld<T>(c.in, vxm, c.so(I::Q2));
+                x.q2ym = bld<T>(c.in, vym, c.so(I::Q2));
+                x.q3xm = bld<T>(c.in, vxm, c.so(I::Q3));
+                x.q4ym = bld<T>(c.in, vym, c.so(I::Q4));
+                x.q5ym = bld<T>(c.in, vym, c.so(I::Q5));
+            }
+        }
+    }
+}
 
 template <typename T>
 struct ColQ {
@@ -108,6 +148,7 @@ struct ColQ {
     uint8_t m1, m2;
     T nzn;
     Cen<T> cc;
+    Nbr<T> nb;          // neighbour prefetching: neighbours at kp
 };
 
 // Own state at byte offset vb for mask byte m: the dual state where either
@@ -158,11 +199,12 @@ struct POut {
 // state never leaves zero) and advances the column's queues to kp+1.
 // wq = false (TGV ring cells on the -x/-y sides) skips w and q: the dual
 // reads w_ only on the + sides.
-template <typename T, bool TGV, bool PF>
+template <typename T, bool TGV, bool PF, bool PFN = false>
 __device__ __forceinline__ void pcol(const Ctx<T> &c, const PrimalCoef<T> &pc,
                                      int kp, bool wq, ColQ<T> &q,
                                      DualVox<T> &dv, bool &pact, POut<T> &o)
 {
+    static_assert(PF || !PFN, "neighbour prefetching needs state prefetching");
     using I = Ix<TGV>;
     constexpr uint32_t S = sizeof(T);
     const uint8_t mc = q.mc;
@@ -172,6 +214,8 @@ __device__ __forceinline__ void pcol(const Ctx<T> &c, const PrimalCoef<T> &pc,
     T nu2 = T(0);
     Cen<T> cn{};
     Cen<T> cur{};
+    Nbr<T> nbn{};
+    Nbr<T> nb{};
     dv = DualVox<T>{};
     o = POut<T>{};
     pact = false;
@@ -184,13 +228,23 @@ __device__ __forceinline__ void pcol(const Ctx<T> &c, const PrimalCoef<T> &pc,
             if (q.m2 & 2)
                 nu2 = bld<T>(c.in, vb + 2 * c.skb, c.so(I::NU));
             load_cen<T, TGV>(c, vb + c.skb, q.m1, wq, cn);
+            if constexpr (PFN) {
+                load_nbr<T, TGV>(c, vb + c.skb, q.m1, wq, nbn);
+                nb = q.nb;
+            }
             np = q.nzn;
             cur = q.cc;
         } else {
             if (kp + 1 < c.nz) {
                 mn = bld_u8(c.mk, uint32_t(l + c.sk));
+#if QSM_NU_UNCOND
+                // nu is zero wherever the dual is inactive (never written
+                // there), so the load need not wait for the mask
+                np = bld<T>(c.in, vb + c.skb, c.so(I::NU));
+#else
                 if (mn & 2)
                     np = bld<T>(c.in, vb + c.skb, c.so(I::NU));
+#endif
             }
             load_cen<T, TGV>(c, vb, mc, wq, cur);
         }
@@ -212,25 +266,22 @@ __device__ __forceinline__ void pcol(const Ctx<T> &c, const PrimalCoef<T> &pc,
         }
         pact = (mc & 1) != 0;
         if (pact) {
-            // pact implies an interior voxel: every neighbour offset is in range
-            const uint32_t vxm = vb - S, vym = vb - c.syb, vyp = vb + c.syb;
+            if constexpr (!PFN)
+                load_nbr<T, TGV>(c, vb, mc, wq, nb);
             PrimalIn<T> pi;
-            pi.nu = Star<T>{q.nzc, bld<T>(c.in, vxm, c.so(I::NU)),
-                            bld<T>(c.in, vb + S, c.so(I::NU)),
-                            bld<T>(c.in, vym, c.so(I::NU)),
-                            bld<T>(c.in, vyp, c.so(I::NU)), q.nzm, np};
-            pi.p1c = dv.p1; pi.p1xm = bld<T>(c.in, vxm, c.so(I::P1));
-            pi.p2c = dv.p2; pi.p2ym = bld<T>(c.in, vym, c.so(I::P2));
+            pi.nu = Star<T>{q.nzc, nb.nxm, nb.nxp, nb.nym, nb.nyp, q.nzm, np};
+            pi.p1c = dv.p1; pi.p1xm = nb.p1xm;
+            pi.p2c = dv.p2; pi.p2ym = nb.p2ym;
             pi.p3c = dv.p3; pi.p3zm = q.p3m;
             if constexpr (TGV) {
                 if (wq) {
-                    pi.q1c = dv.q1; pi.q1xm = bld<T>(c.in, vxm, c.so(I::Q1));
-                    pi.q2c = dv.q2; pi.q2xm = bld<T>(c.in, vxm, c.so(I::Q2));
-                    pi.q2ym = bld<T>(c.in, vym, c.so(I::Q2));
-                    pi.q3c = dv.q3; pi.q3xm = bld<T>(c.in, vxm, c.so(I::Q3));
+                    pi.q1c = dv.q1; pi.q1xm = nb.q1xm;
+                    pi.q2c = dv.q2; pi.q2xm = nb.q2xm;
+                    pi.q2ym = nb.q2ym;
+                    pi.q3c = dv.q3; pi.q3xm = nb.q3xm;
                     pi.q3zm = q.q3m;
-                    pi.q4c = dv.q4; pi.q4ym = bld<T>(c.in, vym, c.so(I::Q4));
-                    pi.q5c = dv.q5; pi.q5ym = bld<T>(c.in, vym, c.so(I::Q5));
+                    pi.q4c = dv.q4; pi.q4ym = nb.q4ym;
+                    pi.q5c = dv.q5; pi.q5ym = nb.q5ym;
                     pi.q5zm = q.q5m;
                     pi.q6c = dv.q6; pi.q6zm = q.q6m;
                 }
@@ -260,6 +311,8 @@ __device__ __forceinline__ void pcol(const Ctx<T> &c, const PrimalCoef<T> &pc,
         q.m2 = m3;
         q.nzn = nu2;
         q.cc = cn;
+        if constexpr (PFN)
+            q.nb = nbn;
     } else {
         q.mc = mn;
     }
@@ -274,7 +327,7 @@ __device__ __forceinline__ void pcol(const Ctx<T> &c, const PrimalCoef<T> &pc,
 }
 
 // Loads a column's queues for a march whose first primal plane is kp0.
-template <typename T, bool TGV, bool PF>
+template <typename T, bool TGV, bool PF, bool PFN = false>
 __device__ __forceinline__ void col_init(const Ctx<T> &c, int kp0, bool wq, ColQ<T> &q)
 {
     using I = Ix<TGV>;
@@ -282,6 +335,7 @@ __device__ __forceinline__ void col_init(const Ctx<T> &c, int kp0, bool wq, ColQ
     q.mc = q.m1 = q.m2 = 0;
     q.nzm = q.nzc = q.nzn = q.p3m = q.q3m = q.q5m = q.q6m = T(0);
     q.cc = Cen<T>{};
+    q.nb = Nbr<T>{};
     if (q.l0 < 0)
         return;
     const int l = q.l0 + c.sk * kp0;
@@ -297,6 +351,8 @@ __device__ __forceinline__ void col_init(const Ctx<T> &c, int kp0, bool wq, ColQ
         if (q.m1 & 2)
             q.nzn = bld<T>(c.in, vb + c.skb, c.so(I::NU));
         load_cen<T, TGV>(c, vb, q.mc, wq, q.cc);
+        if constexpr (PFN)
+            load_nbr<T, TGV>(c, vb, q.mc, wq, q.nb);
     }
     if (kp0 >= 1) {
         const uint8_t mb = bld_u8(c.mk, uint32_t(l - c.sk));
@@ -394,8 +450,12 @@ __device__ __forceinline__ void fused_tile(FusedLds<T, TGV, TX, TY> &L, const Ct
     const int lr = (ry + 1) * LW + (rx + 1);
     const bool rplus = rx == TX || ry == TY;  // +x / +y side of the ring
 
-    constexpr bool PFO = QSM_PREFETCH >= 1, PFR = QSM_PREFETCH >= 2;
-    col_init<T, TGV, PFO>(c, k0 - 1, true, qo);
+    // QSM_PREFETCH: 1 own state, 2 own and ring state, 3 own state and
+    // neighbours, 4 own and ring state, own neighbours
+    constexpr bool PFO = QSM_PREFETCH >= 1;
+    constexpr bool PFR = QSM_PREFETCH == 2 || QSM_PREFETCH == 4;
+    constexpr bool PFN = QSM_PREFETCH >= 3;
+    col_init<T, TGV, PFO, PFN>(c, k0 - 1, true, qo);
     if (t < NRING)
         col_init<T, TGV, PFR>(c, k0 - 1, rplus, qr);
 
@@ -417,7 +477,7 @@ __device__ __forceinline__ void fused_tile(FusedLds<T, TGV, TX, TY> &L, const Ct
         DualVox<T> dv;
         bool pact;
         POut<T> o;
-        pcol<T, TGV, PFO>(c, pc, kp, true, qo, dv, pact, o);
+        pcol<T, TGV, PFO, PFN>(c, pc, kp, true, qo, dv, pact, o);
         lds[slot][0][lo] = o.cb;
         lds[slot][1][lo] = o.eb;
         if constexpr (TGV) {
