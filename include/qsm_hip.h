@@ -240,7 +240,10 @@ int qsm_lapw_dev(int dtype, void *d2u, const void *u, const size_t sz[3],
 typedef struct qsm_batch qsm_batch;
 
 /* A batch of one dtype whose fused passes use tx x ty tiles.  hbm_budget:
- * bytes of volume state the batch may hold (admission limit). */
+ * bytes of volume state the batch may hold (admission limit).  stream:
+ * the HIP stream the batch runs on; NULL makes the batch create (and
+ * destroy) a stream of its own, since iterations are captured into graphs,
+ * which the legacy null stream cannot hold. */
 int qsm_batch_create(qsm_batch **out, int dtype, int tx, int ty,
                      size_t hbm_budget, int max_volumes, void *stream);
 

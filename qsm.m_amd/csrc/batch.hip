@@ -182,6 +182,7 @@ struct qsm_batch {
     size_t budget, used;
     int max_vols;
     hipStream_t stream;
+    bool own_stream = false;     // created here (the caller passed none)
     void *d_vols = nullptr;      // VolDesc<T>[max_vols]
     int *d_tilevol = nullptr;    // tile -> slot
     void *d_tl = nullptr;        // TileList
@@ -487,6 +488,16 @@ int qsm_batch_create(qsm_batch **out, int dtype, int tx, int ty, size_t hbm_budg
     b->used = 0;
     b->max_vols = max_volumes;
     b->stream = as_stream(stream);
+    if (!stream) {
+        // graph capture needs a stream of its own (not the legacy null
+        // stream); a blocking one still orders after null-stream copies
+        const hipError_t se = hipStreamCreate(&b->stream);
+        if (se != hipSuccess) {
+            delete b;
+            return hip_fail(se, "hipStreamCreate", __FILE__, __LINE__);
+        }
+        b->own_stream = true;
+    }
     b->vols.resize(size_t(max_volumes));
     b->desc_size = dtype == QSM_F64 ? sizeof(VolDesc<double>) : sizeof(VolDesc<float>);
     b->h_desc.assign(b->desc_size * size_t(max_volumes), 0);
@@ -604,6 +615,7 @@ void qsm_batch_destroy(qsm_batch *b)
     if (b->d_vols) (void)hipFree(b->d_vols);
     if (b->d_tilevol) (void)hipFree(b->d_tilevol);
     if (b->d_tl) (void)hipFree(b->d_tl);
+    if (b->own_stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
 

@@ -103,9 +103,14 @@ def dipole_field(chi, vsz, bdir, workers=None) -> np.ndarray:
     up = padfastfft(chi, pads)
     D = dipole_kernel(up.shape, vsz, bdir)
     w = workers or min(16, os.cpu_count() or 1)
-    F = sfft.rfftn(up, axes=(0, 1, 2), workers=w)
-    F *= D[:, :, : F.shape[2]]
-    f = sfft.irfftn(F, s=up.shape, axes=(0, 1, 2), workers=w)
+    # full complex transforms and the real part, as the toolbox idiom
+    # real(ifft3(D .* fft3(x))): on even-length axes D is not symmetric
+    # under k -> -k at the Nyquist planes when B0 has two or more non-zero
+    # components, so a real-to-complex transform (which assumes Hermitian
+    # symmetry) would give a different field
+    F = sfft.fftn(up, axes=(0, 1, 2), workers=w)
+    F *= D
+    f = sfft.ifftn(F, axes=(0, 1, 2), workers=w).real
     return np.asfortranarray(f[unpad_slices(up.shape, shape, pads)])
 
 

@@ -38,9 +38,15 @@ def test_gradients_host(cuda, shape, dt):
     rng = np.random.default_rng(0)
     u = rand_field(rng, shape, dt)
     G = _mask(shape, 1)
-    for g, want in ((fd.gradfp(u, H), orc.gradfp(u, H)),
-                    (fd.gradf(u, None, H), orc.gradf(u, H)),
-                    (fd.gradf(u, G, H), orc.gradfm(u, G, H))):
+    cases = [(fd.gradfp(u, H), orc.gradfp(u, H)), (fd.gradf(u, G, H), orc.gradfm(u, G, H))]
+    if min(shape) >= 2:
+        cases.append((fd.gradf(u, None, H), orc.gradf(u, H)))
+    else:
+        # the unmasked forward/backward pair needs two voxels per axis (the
+        # reference reads out of bounds there); rejected, not computed
+        with pytest.raises(_lib.QsmError, match="needs >= 2"):
+            fd.gradf(u, None, H)
+    for g, want in cases:
         for a, b in zip(g, want):
             _same(a, b)
 
@@ -52,8 +58,12 @@ def test_adjoints_host(cuda, shape, dt):
     x, y, z = (rand_field(rng, shape, dt) for _ in range(3))
     G = _mask(shape, 3)
     _same(fd.gradfp_adj(x, y, z, H), orc.gradfp_adj(x, y, z, H))
-    _same(fd.gradf_adj(x, y, z, None, H), orc.gradf_adj(x, y, z, H))
     _same(fd.gradf_adj(x, y, z, G, H), orc.gradfm_adj(x, y, z, G, H))
+    if min(shape) >= 2:
+        _same(fd.gradf_adj(x, y, z, None, H), orc.gradf_adj(x, y, z, H))
+    else:
+        with pytest.raises(_lib.QsmError, match="needs >= 2"):
+            fd.gradf_adj(x, y, z, None, H)
 
 
 @pytest.mark.parametrize("shape", SHAPES)
