@@ -55,16 +55,22 @@ struct VolDesc {
 };
 
 struct TileList {
-    int ntiles;
-    int pad[3];
+    int ntiles[2];   // tiles of TV volumes, of TGV volumes (list order)
+    int pad[2];
 };
 
-template <typename T, int TX, int TY>
+// One persistent launch per kind: kind KIND's tiles are a contiguous part
+// of the tile list (TV tiles first).  Specialising per kind keeps the
+// register count that of one march (two kinds inlined in one kernel would
+// cost occupancy).
+template <typename T, bool TGV, int TX, int TY>
 __global__ __launch_bounds__(TX *TY) void batch_fused_kernel(
-    VolDesc<T> *vols, const int *tile_vol, const TileList *tl)
+    const VolDesc<T> *__restrict__ vols, const int *__restrict__ tile_vol,
+    const TileList *__restrict__ tl)
 {
-    __shared__ FusedLds<T, true, TX, TY> L;   // the TV march uses a prefix
-    const int T_ = tl->ntiles;
+    __shared__ FusedLds<T, TGV, TX, TY> L;
+    const int T_ = tl->ntiles[TGV ? 1 : 0];
+    const int *tv = tile_vol + (TGV ? tl->ntiles[0] : 0);
     // XCD-aware striding: XCD x (= block % 8) walks a contiguous range of
     // the tile list, which keeps a volume's neighbouring tiles on one L2.
     const int xcd = int(blockIdx.x) % 8;
@@ -73,43 +79,41 @@ __global__ __launch_bounds__(TX *TY) void batch_fused_kernel(
     const int span = (T_ + 7) / 8;
     const int t_end = min(T_, (xcd + 1) * span);
     for (int t = xcd * span + bl; t < t_end; t += nper) {
-        const int v = tile_vol[t];
-        VolDesc<T> &d = vols[v];
-        if (!d.live || d.ctl->stop)
+        // the descriptor is read-only here (restrict): its fields stay in
+        // registers through the march
+        const VolDesc<T> *dp = vols + tv[t];
+        if (!dp->live || dp->ctl->stop)
             continue;
-        const int lt = t - d.tile0;
-        const int bx = lt % d.ntx;
-        const int by = (lt / d.ntx) % d.nty;
-        const int bz = lt / (d.ntx * d.nty);
-        const uint32_t it = d.iter;
+        const int lt = t - dp->tile0;
+        const int ntx = dp->ntx, nty = dp->nty;
+        const int bx = lt % ntx;
+        const int by = (lt / ntx) % nty;
+        const int bz = lt / (ntx * nty);
+        const uint32_t it = dp->iter;
         const bool check = ((it + 1) % 10) == 0;  // pd_tgv_mex.c:298,309
+        const T *b0 = dp->bank[0], *b1 = dp->bank[1];
+        const DualCoef<T> dc = dp->dc;
+        const PrimalCoef<T> pc = dp->pc;
+        const int nx = dp->nx, ny = dp->ny, nz = dp->nz, kz = dp->kz;
         constexpr uint32_t S = sizeof(T);
+        constexpr uint32_t NARR = TGV ? 15u : 6u;
         Ctx<T> c;
-        c.nb = d.N * S;
-        const uint32_t narr = d.kind == QSM_TGV ? 15u : 6u;
-        c.in = make_rsrc(d.bank[it & 1u], narr * c.nb);
-        c.out = make_rsrc(d.bank[(it + 1) & 1u], narr * c.nb);
-        c.f = make_rsrc(d.f, c.nb);
-        c.mk = make_rsrc(d.mk, d.N);
-        c.sk = d.nx * d.ny;
-        c.syb = uint32_t(d.nx) * S;
+        c.nb = dp->N * S;
+        c.in = make_rsrc((it & 1u) ? b1 : b0, NARR * c.nb);
+        c.out = make_rsrc((it & 1u) ? b0 : b1, NARR * c.nb);
+        c.f = make_rsrc(dp->f, c.nb);
+        c.mk = make_rsrc(dp->mk, dp->N);
+        c.sk = nx * ny;
+        c.syb = uint32_t(nx) * S;
         c.skb = uint32_t(c.sk) * S;
-        c.nz = d.nz;
+        c.nz = nz;
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        if (d.kind == QSM_TGV) {
-            if (check)
-                fused_tile<T, true, TX, TY, true>(L, c, d.dc, d.pc, d.nx, d.ny, d.nz, bx, by, bz, d.kz, acc);
-            else
-                fused_tile<T, true, TX, TY, false>(L, c, d.dc, d.pc, d.nx, d.ny, d.nz, bx, by, bz, d.kz, acc);
+        if (check) {
+            fused_tile<T, TGV, TX, TY, true>(L, c, dc, pc, nx, ny, nz, bx, by, bz, kz, acc);
+            block_sum4_f<TX * TY>(acc, dp->partials + 4 * lt);
         } else {
-            auto &Lt = reinterpret_cast<FusedLds<T, false, TX, TY> &>(L);
-            if (check)
-                fused_tile<T, false, TX, TY, true>(Lt, c, d.dc, d.pc, d.nx, d.ny, d.nz, bx, by, bz, d.kz, acc);
-            else
-                fused_tile<T, false, TX, TY, false>(Lt, c, d.dc, d.pc, d.nx, d.ny, d.nz, bx, by, bz, d.kz, acc);
+            fused_tile<T, TGV, TX, TY, false>(L, c, dc, pc, nx, ny, nz, bx, by, bz, kz, acc);
         }
-        if (check)
-            block_sum4_f<TX * TY>(acc, d.partials + 4 * lt);
     }
 }
 
@@ -175,6 +179,8 @@ struct BatchVol {
     char *mem = nullptr;
     int ntiles = 0;
     int hist_cap = 0;
+    bool uploaded = false;   // its descriptor is on the device (the device
+                             // then owns iter and live)
 };
 
 struct qsm_batch {
@@ -190,7 +196,7 @@ struct qsm_batch {
     std::vector<BatchVol> vols;
     std::vector<unsigned char> h_desc;  // host mirror of the descriptors
     size_t desc_size = 0;
-    int grid = 0;
+    int grid[2] = {0, 0};         // persistent grid of the TV / TGV kernel
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     uint32_t graph_chunk = 0;
@@ -206,14 +212,14 @@ VolDesc<T> &desc(qsm_batch *b, int v)
     return reinterpret_cast<VolDesc<T> *>(b->h_desc.data())[v];
 }
 
-template <typename T, int TX, int TY>
+template <typename T, bool TGV, int TX, int TY>
 int persistent_grid(int *out)
 {
     int dev = 0, ncu = 0, occ = 0;
     QSM_HIP(hipGetDevice(&dev));
     QSM_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     QSM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &occ, reinterpret_cast<const void *>(&batch_fused_kernel<T, TX, TY>), TX * TY, 0));
+        &occ, reinterpret_cast<const void *>(&batch_fused_kernel<T, TGV, TX, TY>), TX * TY, 0));
     if (occ < 1) occ = 1;
     int g = ncu * occ;
     g = std::max(8, g / 8 * 8);
@@ -221,14 +227,27 @@ int persistent_grid(int *out)
     return QSM_OK;
 }
 
+template <typename T, int TX, int TY>
+int persistent_grids(int out[2])
+{
+    int rc = persistent_grid<T, false, TX, TY>(&out[0]);
+    return rc ? rc : persistent_grid<T, true, TX, TY>(&out[1]);
+}
+
 template <typename T>
 hipError_t launch_iteration(qsm_batch *b)
 {
     auto *vols = static_cast<VolDesc<T> *>(b->d_vols);
     const auto *tl = static_cast<const TileList *>(b->d_tl);
+    // both kinds every iteration (a kind with no tiles exits at once), so
+    // a captured graph stays valid while the mix of kinds changes
 #define QSM_B(X, Y)                                                            \
-    if (b->tx == X && b->ty == Y)                                              \
-        batch_fused_kernel<T, X, Y><<<b->grid, X * Y, 0, b->stream>>>(vols, b->d_tilevol, tl);
+    if (b->tx == X && b->ty == Y) {                                            \
+        batch_fused_kernel<T, false, X, Y><<<b->grid[0], X * Y, 0, b->stream>>>( \
+            vols, b->d_tilevol, tl);                                           \
+        batch_fused_kernel<T, true, X, Y><<<b->grid[1], X * Y, 0, b->stream>>>(  \
+            vols, b->d_tilevol, tl);                                           \
+    }
     QSM_B(64, 4) QSM_B(64, 8) QSM_B(64, 16) QSM_B(32, 8) QSM_B(32, 16) QSM_B(32, 32)
 #undef QSM_B
     hipError_t e = hipGetLastError();
@@ -241,8 +260,8 @@ int grid_for(qsm_batch *b)
 {
 #define QSM_G(X, Y)                                                            \
     if (b->tx == X && b->ty == Y)                                              \
-        return b->dtype == QSM_F64 ? persistent_grid<double, X, Y>(&b->grid)   \
-                                   : persistent_grid<float, X, Y>(&b->grid);
+        return b->dtype == QSM_F64 ? persistent_grids<double, X, Y>(b->grid)   \
+                                   : persistent_grids<float, X, Y>(b->grid);
     QSM_G(64, 4) QSM_G(64, 8) QSM_G(64, 16) QSM_G(32, 8) QSM_G(32, 16) QSM_G(32, 32)
 #undef QSM_G
     set_error("unsupported tile shape");
@@ -262,13 +281,21 @@ int drop_graph(qsm_batch *b)
 template <typename T>
 int upload(qsm_batch *b)
 {
+    // TV volumes' tiles first, then TGV volumes' (one kernel per kind);
+    // tile0 is relative to the kind's part of the list
     std::vector<int> map;
-    for (int v = 0; v < b->max_vols; ++v) {
-        if (!b->vols[v].used) continue;
-        VolDesc<T> &d = desc<T>(b, v);
-        d.tile0 = int(map.size());
-        for (int t = 0; t < b->vols[v].ntiles; ++t)
-            map.push_back(v);
+    int nk[2] = {0, 0};
+    for (int kind = 0; kind < 2; ++kind) {
+        const size_t start = map.size();
+        for (int v = 0; v < b->max_vols; ++v) {
+            if (!b->vols[v].used || (b->vols[v].kind == QSM_TGV) != (kind == 1)) continue;
+            b->vols[v].uploaded = true;
+            VolDesc<T> &d = desc<T>(b, v);
+            d.tile0 = int(map.size() - start);
+            for (int t = 0; t < b->vols[v].ntiles; ++t)
+                map.push_back(v);
+        }
+        nk[kind] = int(map.size() - start);
     }
     if (int(map.size()) > b->tile_cap) {
         if (b->d_tilevol) (void)hipFree(b->d_tilevol);
@@ -279,7 +306,7 @@ int upload(qsm_batch *b)
     }
     if (!map.empty())
         QSM_HIP(hipMemcpy(b->d_tilevol, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice));
-    TileList tl{int(map.size()), {0, 0, 0}};
+    TileList tl{{nk[0], nk[1]}, {0, 0}};
     QSM_HIP(hipMemcpy(b->d_tl, &tl, sizeof tl, hipMemcpyHostToDevice));
     QSM_HIP(hipMemcpy(b->d_vols, b->h_desc.data(), b->h_desc.size(), hipMemcpyHostToDevice));
     b->dirty = false;
@@ -293,6 +320,10 @@ int pull(qsm_batch *b)
     std::vector<unsigned char> tmp(b->h_desc.size());
     QSM_HIP(hipMemcpy(tmp.data(), b->d_vols, tmp.size(), hipMemcpyDeviceToHost));
     for (int v = 0; v < b->max_vols; ++v) {
+        // a volume added since the last upload is not on the device yet:
+        // its host descriptor is the current one
+        if (!b->vols[v].used || !b->vols[v].uploaded)
+            continue;
         const VolDesc<T> &s = reinterpret_cast<const VolDesc<T> *>(tmp.data())[v];
         VolDesc<T> &d = desc<T>(b, v);
         d.iter = s.iter;
@@ -340,6 +371,7 @@ int add_t(qsm_batch *b, int kind, const size_t sz[3], const double h[3], double 
     QSM_HIP(hipMalloc(reinterpret_cast<void **>(&mem), bytes));
     BatchVol &bv = b->vols[v];
     bv.used = true;
+    bv.uploaded = false;
     bv.kind = kind;
     for (int a = 0; a < 3; ++a) bv.sz[a] = sz[a];
     bv.N = N;

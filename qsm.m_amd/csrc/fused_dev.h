@@ -41,9 +41,10 @@ __device__ __forceinline__ double bld<double>(rsrc_t r, uint32_t vo, uint32_t so
 
 // Cache policy of the state stores (the instruction's aux operand; on
 // gfx950 bit 1 is the nontemporal hint).  The out bank is not read again
-// in the same pass.
+// in the same pass, so its stores stream past the caches that hold the
+// in bank's halo (measured 4-6 % faster at 256x256x176, both precisions).
 #ifndef QSM_STORE_AUX
-#define QSM_STORE_AUX 0
+#define QSM_STORE_AUX 2
 #endif
 
 __device__ __forceinline__ void bst(float v, rsrc_t r, uint32_t vo, uint32_t so)
@@ -91,8 +92,10 @@ struct Ctx {
 #ifndef QSM_PREFETCH
 #define QSM_PREFETCH 0
 #endif
+// nu of the next plane is loaded without waiting for that plane's mask
+// (one dependent memory round trip less per plane step).
 #ifndef QSM_NU_UNCOND
-#define QSM_NU_UNCOND 0
+#define QSM_NU_UNCOND 1
 #endif
 
 // A column's own state at one plane (what pcol reads at the centre voxel).

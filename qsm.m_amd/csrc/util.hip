@@ -4,12 +4,14 @@
 
 namespace {
 
-// 16-byte vector copy, grid-stride; the tail (bytes % 16) by single bytes.
+// 16-byte vector copy, one vector per lane in a single pass (no grid
+// stride: on gfx950 this form streams ~6.2 TB/s at 2 GiB, a grid-stride
+// loop ~4.6); the tail (bytes % 16) by single bytes.
 __global__ __launch_bounds__(256) void copy16_kernel(uint4 *dst, const uint4 *src,
                                                      size_t n16)
 {
-    const size_t stride = size_t(gridDim.x) * 256;
-    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n16; i += stride)
+    const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i < n16)
         dst[i] = src[i];
 }
 
@@ -30,13 +32,10 @@ extern "C" int qsm_copy_dev(void *dst, const void *src, size_t bytes, void *stre
     const hipStream_t st = qsm::as_stream(stream);
     const size_t n16 = bytes / 16, tail = bytes % 16;
     if (n16) {
-        int dev = 0, ncu = 256;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        const size_t want = (n16 + 255) / 256;
-        const unsigned grid = unsigned(want < size_t(ncu) * 16 ? want : size_t(ncu) * 16);
-        copy16_kernel<<<grid, 256, 0, st>>>(static_cast<uint4 *>(dst),
-                                            static_cast<const uint4 *>(src), n16);
+        const size_t blocks = (n16 + 255) / 256;
+        QSM_ARG(blocks <= 0x7fffffffu, "copy of %zu bytes too large for one launch", bytes);
+        copy16_kernel<<<unsigned(blocks), 256, 0, st>>>(static_cast<uint4 *>(dst),
+                                                        static_cast<const uint4 *>(src), n16);
         QSM_HIP(hipGetLastError());
     }
     if (tail) {

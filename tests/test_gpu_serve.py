@@ -38,3 +38,28 @@ def test_serving_results_equal_solo_solves(cuda):
             ref, it, _ = orc.pd_tv(r.d2f, r.mask, r.vsz, r.lam, 0.0, r.maxiter)
         assert r.iters == it == r.maxiter
         np.testing.assert_array_equal(r.chi, ref, err_msg=f"request {r.rid}")
+
+
+def test_serving_with_live_voxel_cap(cuda):
+    """With the live work capped at two volumes' worth, requests still match
+    the oracle bitwise, at most two run at once, and they finish in arrival
+    order (equal budgets)."""
+    rng = np.random.default_rng(11)
+    shape = (26, 24, 20)
+    reqs = []
+    for i, t in enumerate(poisson_arrivals(200.0, 5, rng)):
+        kind = _lib.TGV if i % 2 else _lib.TV
+        d2f = _phantom_input(shape, (1.0, 1.0, 1.0), 70 + i, np.float32)
+        m = brainish_mask(shape, seed=80 + i, frac=0.9)
+        reqs.append(Request(i, float(t), kind, d2f, m, (1.0, 1.0, 1.0), 2e-3, 2.0, 40, kz=8))
+    with Batch(np.float32, max_volumes=8) as b:
+        stats = Server(b, chunk=10, max_live_voxels=2 * int(np.prod(shape))).run(reqs)
+    assert stats.peak_live <= 2
+    done = [r.done for r in reqs]
+    assert done == sorted(done)
+    for r in reqs:
+        if r.kind == _lib.TGV:
+            ref = orc.pd_tgv(r.d2f, r.mask, r.vsz, r.lam, r.alpha, 0.0, r.maxiter)[0]
+        else:
+            ref = orc.pd_tv(r.d2f, r.mask, r.vsz, r.lam, 0.0, r.maxiter)[0]
+        np.testing.assert_array_equal(r.chi, ref, err_msg=f"request {r.rid}")
