@@ -42,7 +42,8 @@ Layout layout(int kind, int dtype, const size_t sz[3], int npart, int hist_cap)
 // most blocks a shape can have) and the two-sweep primal.
 int max_partials(Geom g)
 {
-    static const int shapes[][2] = {{64, 4}, {64, 8}, {64, 16}, {32, 8}, {32, 16}, {32, 32}};
+    static const int shapes[][2] = {{64, 4}, {64, 8}, {64, 16}, {32, 8}, {32, 16}, {32, 32},
+                                    {128, 4}, {128, 8}};
     int n = std::max(primal_sweep_blocks(g), 1);
     for (const auto &sh : shapes)
         n = std::max(n, fused_blocks(g, FusedShape{sh[0], sh[1], 1}));
@@ -426,6 +427,8 @@ int qsm_pd_set_tiles(qsm_pd *s, int tx, int ty, int kz)
 {
     QSM_ARG(s, "null handle");
     QSM_ARG(fused_shape_supported(tx, ty), "unsupported tile shape %dx%d", tx, ty);
+    QSM_ARG(fused_shape_valid(tx, ty, s->dtype, s->geom()),
+            "tile shape %dx%d (two voxels per lane) needs single precision and an even nx", tx, ty);
     QSM_ARG(kz >= 1, "kz must be >= 1");
     const FusedShape sh{tx, ty, kz};
     QSM_ARG(fused_blocks(s->geom(), sh) <= s->npart,
