@@ -24,19 +24,15 @@ __device__ __forceinline__ rsrc_t make_rsrc(const void *p, uint32_t bytes)
 
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
-template <typename T>
-__device__ __forceinline__ T bld(rsrc_t r, uint32_t vo, uint32_t so);
-
-template <>
-__device__ __forceinline__ float bld<float>(rsrc_t r, uint32_t vo, uint32_t so)
+// Loads one value; AUX is the instruction's cache-policy operand (0: the
+// default, 2: nontemporal).
+template <typename T, int AUX = 0>
+__device__ __forceinline__ T bld(rsrc_t r, uint32_t vo, uint32_t so)
 {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, 0));
-}
-
-template <>
-__device__ __forceinline__ double bld<double>(rsrc_t r, uint32_t vo, uint32_t so)
-{
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, 0));
+    if constexpr (sizeof(T) == 4)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, AUX));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, AUX));
 }
 
 // Cache policy of the state stores (the instruction's aux operand; on
@@ -85,18 +81,6 @@ struct Ctx {
     int sk, nz;          // z stride in voxels, planes
     __device__ __forceinline__ uint32_t so(int a) const { return uint32_t(a) * nb; }
 };
-
-// Software pipelining of the column march: with prefetching on, a step
-// issues the loads of the next plane's own values before computing the
-// current one (1: own columns, 2: own and ring columns).
-#ifndef QSM_PREFETCH
-#define QSM_PREFETCH 0
-#endif
-// nu of the next plane is loaded without waiting for that plane's mask
-// (one dependent memory round trip less per plane step).
-#ifndef QSM_NU_UNCOND
-#define QSM_NU_UNCOND 1
-#endif
 
 // A column's own state at one plane (what pcol reads at the centre voxel).
 template <typename T>
@@ -147,11 +131,6 @@ struct ColQ {
     uint8_t mc;         // mask byte at the current plane kp
     T nzm, nzc;         // nu at kp-1, kp
     T p3m, q3m, q5m, q6m;  // z-backward neighbours (plane kp-1)
-    // prefetching only: masks at kp+1, kp+2, nu at kp+1, own state at kp
-    uint8_t m1, m2;
-    T nzn;
-    Cen<T> cc;
-    Nbr<T> nb;          // neighbour prefetching: neighbours at kp
 };
 
 // Own state at byte offset vb for mask byte m: the dual state where either
@@ -202,22 +181,17 @@ struct POut {
 // state never leaves zero) and advances the column's queues to kp+1.
 // wq = false (TGV ring cells on the -x/-y sides) skips w and q: the dual
 // reads w_ only on the + sides.
-template <typename T, bool TGV, bool PF, bool PFN = false>
+template <typename T, bool TGV>
 __device__ __forceinline__ void pcol(const Ctx<T> &c, const PrimalCoef<T> &pc,
                                      int kp, bool wq, ColQ<T> &q,
                                      DualVox<T> &dv, bool &pact, POut<T> &o)
 {
-    static_assert(PF || !PFN, "neighbour prefetching needs state prefetching");
     using I = Ix<TGV>;
     constexpr uint32_t S = sizeof(T);
     const uint8_t mc = q.mc;
     uint8_t mn = 0;
     T np = T(0);
-    uint8_t m3 = 0;   // prefetching: mask at kp+3, nu at kp+2, state at kp+1
-    T nu2 = T(0);
-    Cen<T> cn{};
     Cen<T> cur{};
-    Nbr<T> nbn{};
     Nbr<T> nb{};
     dv = DualVox<T>{};
     o = POut<T>{};
@@ -225,32 +199,14 @@ __device__ __forceinline__ void pcol(const Ctx<T> &c, const PrimalCoef<T> &pc,
     if (q.l0 >= 0) {
         const int l = q.l0 + c.sk * kp;
         const uint32_t vb = uint32_t(l) * S;
-        if constexpr (PF) {
-            if (kp + 3 < c.nz)
-                m3 = bld_u8(c.mk, uint32_t(l + 3 * c.sk));
-            if (q.m2 & 2)
-                nu2 = bld<T>(c.in, vb + 2 * c.skb, c.so(I::NU));
-            load_cen<T, TGV>(c, vb + c.skb, q.m1, wq, cn);
-            if constexpr (PFN) {
-                load_nbr<T, TGV>(c, vb + c.skb, q.m1, wq, nbn);
-                nb = q.nb;
-            }
-            np = q.nzn;
-            cur = q.cc;
-        } else {
-            if (kp + 1 < c.nz) {
-                mn = bld_u8(c.mk, uint32_t(l + c.sk));
-#if QSM_NU_UNCOND
-                // nu is zero wherever the dual is inactive (never written
-                // there), so the load need not wait for the mask
-                np = bld<T>(c.in, vb + c.skb, c.so(I::NU));
-#else
-                if (mn & 2)
-                    np = bld<T>(c.in, vb + c.skb, c.so(I::NU));
-#endif
-            }
-            load_cen<T, TGV>(c, vb, mc, wq, cur);
+        if (kp + 1 < c.nz) {
+            mn = bld_u8(c.mk, uint32_t(l + c.sk));
+            // nu is zero wherever the dual is inactive (never written
+            // there), so the load need not wait for the mask: one
+            // dependent memory round trip less per plane step
+            np = bld<T>(c.in, vb + c.skb, c.so(I::NU));
         }
+        load_cen<T, TGV>(c, vb, mc, wq, cur);
         if (mc & 3) {
             dv.nu = q.nzc;
             dv.p1 = cur.p1;
@@ -269,8 +225,7 @@ __device__ __forceinline__ void pcol(const Ctx<T> &c, const PrimalCoef<T> &pc,
         }
         pact = (mc & 1) != 0;
         if (pact) {
-            if constexpr (!PFN)
-                load_nbr<T, TGV>(c, vb, mc, wq, nb);
+            load_nbr<T, TGV>(c, vb, mc, wq, nb);
             PrimalIn<T> pi;
             pi.nu = Star<T>{q.nzc, nb.nxm, nb.nxp, nb.nym, nb.nyp, q.nzm, np};
             pi.p1c = dv.p1; pi.p1xm = nb.p1xm;
@@ -308,17 +263,7 @@ __device__ __forceinline__ void pcol(const Ctx<T> &c, const PrimalCoef<T> &pc,
             }
         }
     }
-    if constexpr (PF) {
-        q.mc = q.m1;
-        q.m1 = q.m2;
-        q.m2 = m3;
-        q.nzn = nu2;
-        q.cc = cn;
-        if constexpr (PFN)
-            q.nb = nbn;
-    } else {
-        q.mc = mn;
-    }
+    q.mc = mn;
     q.nzm = q.nzc;
     q.nzc = np;
     q.p3m = dv.p3;
@@ -330,15 +275,13 @@ __device__ __forceinline__ void pcol(const Ctx<T> &c, const PrimalCoef<T> &pc,
 }
 
 // Loads a column's queues for a march whose first primal plane is kp0.
-template <typename T, bool TGV, bool PF, bool PFN = false>
+template <typename T, bool TGV>
 __device__ __forceinline__ void col_init(const Ctx<T> &c, int kp0, bool wq, ColQ<T> &q)
 {
     using I = Ix<TGV>;
     constexpr uint32_t S = sizeof(T);
-    q.mc = q.m1 = q.m2 = 0;
-    q.nzm = q.nzc = q.nzn = q.p3m = q.q3m = q.q5m = q.q6m = T(0);
-    q.cc = Cen<T>{};
-    q.nb = Nbr<T>{};
+    q.mc = 0;
+    q.nzm = q.nzc = q.p3m = q.q3m = q.q5m = q.q6m = T(0);
     if (q.l0 < 0)
         return;
     const int l = q.l0 + c.sk * kp0;
@@ -346,17 +289,6 @@ __device__ __forceinline__ void col_init(const Ctx<T> &c, int kp0, bool wq, ColQ
     q.mc = bld_u8(c.mk, uint32_t(l));
     if (q.mc & 2)
         q.nzc = bld<T>(c.in, vb, c.so(I::NU));
-    if constexpr (PF) {
-        if (kp0 + 1 < c.nz)
-            q.m1 = bld_u8(c.mk, uint32_t(l + c.sk));
-        if (kp0 + 2 < c.nz)
-            q.m2 = bld_u8(c.mk, uint32_t(l + 2 * c.sk));
-        if (q.m1 & 2)
-            q.nzn = bld<T>(c.in, vb + c.skb, c.so(I::NU));
-        load_cen<T, TGV>(c, vb, q.mc, wq, q.cc);
-        if constexpr (PFN)
-            load_nbr<T, TGV>(c, vb, q.mc, wq, q.nb);
-    }
     if (kp0 >= 1) {
         const uint8_t mb = bld_u8(c.mk, uint32_t(l - c.sk));
         const uint32_t vm = vb - c.skb;
@@ -453,14 +385,9 @@ __device__ __forceinline__ void fused_tile(FusedLds<T, TGV, TX, TY> &L, const Ct
     const int lr = (ry + 1) * LW + (rx + 1);
     const bool rplus = rx == TX || ry == TY;  // +x / +y side of the ring
 
-    // QSM_PREFETCH: 1 own state, 2 own and ring state, 3 own state and
-    // neighbours, 4 own and ring state, own neighbours
-    constexpr bool PFO = QSM_PREFETCH >= 1;
-    constexpr bool PFR = QSM_PREFETCH == 2 || QSM_PREFETCH == 4;
-    constexpr bool PFN = QSM_PREFETCH >= 3;
-    col_init<T, TGV, PFO, PFN>(c, k0 - 1, true, qo);
+    col_init<T, TGV>(c, k0 - 1, true, qo);
     if (t < NRING)
-        col_init<T, TGV, PFR>(c, k0 - 1, rplus, qr);
+        col_init<T, TGV>(c, k0 - 1, rplus, qr);
 
     // own-column extrapolation queues and the dual-side values one plane back
     T cbm = T(0), cbc = T(0), ebm = T(0), ebc = T(0);
@@ -480,7 +407,7 @@ __device__ __forceinline__ void fused_tile(FusedLds<T, TGV, TX, TY> &L, const Ct
         DualVox<T> dv;
         bool pact;
         POut<T> o;
-        pcol<T, TGV, PFO, PFN>(c, pc, kp, true, qo, dv, pact, o);
+        pcol<T, TGV>(c, pc, kp, true, qo, dv, pact, o);
         lds[slot][0][lo] = o.cb;
         lds[slot][1][lo] = o.eb;
         if constexpr (TGV) {
@@ -504,7 +431,7 @@ __device__ __forceinline__ void fused_tile(FusedLds<T, TGV, TX, TY> &L, const Ct
             DualVox<T> rdv;
             bool ract;
             POut<T> ro;
-            pcol<T, TGV, PFR>(c, pc, kp, rplus, qr, rdv, ract, ro);
+            pcol<T, TGV>(c, pc, kp, rplus, qr, rdv, ract, ro);
             lds[slot][0][lr] = ro.cb;
             lds[slot][1][lr] = ro.eb;
             if constexpr (TGV) {
