@@ -507,8 +507,7 @@ int qsm_batch_create(qsm_batch **out, int dtype, int tx, int ty, size_t hbm_budg
     QSM_ARG(out, "null handle pointer");
     *out = nullptr;
     QSM_ARG(dtype == QSM_F64 || dtype == QSM_F32, "bad dtype");
-    QSM_ARG(fused_shape_supported(tx, ty) && !fused_shape_paired(tx),
-            "unsupported tile shape %dx%d for a batch", tx, ty);
+    QSM_ARG(fused_shape_supported(tx, ty), "unsupported tile shape %dx%d", tx, ty);
     QSM_ARG(max_volumes >= 1 && max_volumes <= 4096, "max_volumes out of range");
     int rc = ensure_device();
     if (rc) return rc;

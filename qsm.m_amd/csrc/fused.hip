@@ -16,9 +16,7 @@
 // the block recomputes on a one-cell ring (-y row, +y row, -x column,
 // +x column; the -x/-y cells need only chi_ and eta_).  Extrapolations are
 // exchanged through LDS (two plane slots, one barrier per plane).
-#include <type_traits>
-
-#include "fused_pair.h"
+#include "fused_dev.h"
 
 namespace qsm {
 namespace {
@@ -65,44 +63,6 @@ __global__ __launch_bounds__(TX *TY) void fused_kernel(
         block_sum4_f<TX * TY>(acc, partials + 4 * lb);
 }
 
-// The paired march (single precision, even nx): a lane owns two x-adjacent
-// voxels, so a tile is 2*TX voxels wide; otherwise as fused_kernel.
-template <bool TGV, int TX, int TY, bool CHECK>
-__global__ __launch_bounds__(TX *TY) void fused_pair_kernel(
-    const float *in_base, float *out_base, uint32_t N, const float *f,
-    const uint8_t *mk, Geom g, int kz, int ntx, int nty, int nchunk,
-    DualCoef<float> dc, PrimalCoef<float> pc, const int *stop, double *partials)
-{
-    using I = Ix<TGV>;
-    __shared__ PairLds<TGV, TX, TY> L;
-
-    if (stop && *stop)
-        return;
-    const int nblk = ntx * nty * nchunk;
-    const int per = nblk / 8, rem = nblk % 8;
-    const int xcd = int(blockIdx.x) % 8, idx = int(blockIdx.x) / 8;
-    const int lb = (xcd < rem) ? xcd * (per + 1) + idx
-                               : rem * (per + 1) + (xcd - rem) * per + idx;
-    const int bx = lb % ntx;
-    const int by = (lb / ntx) % nty;
-    const int bz = lb / (ntx * nty);
-
-    Ctx<float> c;
-    c.nb = N * 4u;
-    c.in = make_rsrc(in_base, uint32_t(I::NARR) * c.nb);
-    c.out = make_rsrc(out_base, uint32_t(I::NARR) * c.nb);
-    c.f = make_rsrc(f, c.nb);
-    c.mk = make_rsrc(mk, N);
-    c.sk = g.nx * g.ny;
-    c.syb = uint32_t(g.nx) * 4u;
-    c.skb = uint32_t(c.sk) * 4u;
-    c.nz = g.nz;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    fused_tile_pair<TGV, TX, TY, CHECK>(L, c, dc, pc, g.nx, g.ny, g.nz, bx, by, bz, kz, acc);
-    if constexpr (CHECK)
-        block_sum4_f<TX * TY>(acc, partials + 4 * lb);
-}
-
 void fused_tiles(Geom g, FusedShape sh, int &ntx, int &nty, int &nch)
 {
     ntx = (g.nx + sh.tx - 1) / sh.tx;
@@ -128,41 +88,12 @@ hipError_t launch_shape(const T *in, T *out, size_t N, const T *f,
     return hipGetLastError();
 }
 
-template <bool TGV, int TX, int TY>
-hipError_t launch_pair_shape(const float *in, float *out, size_t N, const float *f,
-                             const uint8_t *mk, Geom g, FusedShape sh,
-                             const DualCoef<float> &dc, const PrimalCoef<float> &pc,
-                             const int *stop, double *partials, hipStream_t st)
-{
-    int ntx, nty, nch;
-    fused_tiles(g, sh, ntx, nty, nch);
-    const int nblk = ntx * nty * nch;
-    if (partials)
-        fused_pair_kernel<TGV, TX, TY, true><<<nblk, TX * TY, 0, st>>>(
-            in, out, uint32_t(N), f, mk, g, sh.kz, ntx, nty, nch, dc, pc, stop, partials);
-    else
-        fused_pair_kernel<TGV, TX, TY, false><<<nblk, TX * TY, 0, st>>>(
-            in, out, uint32_t(N), f, mk, g, sh.kz, ntx, nty, nch, dc, pc, stop, partials);
-    return hipGetLastError();
-}
-
 }  // namespace
 
 bool fused_shape_supported(int tx, int ty)
 {
     return (tx == 64 && (ty == 4 || ty == 8 || ty == 16))
-           || (tx == 32 && (ty == 8 || ty == 16 || ty == 32))
-           || (tx == 128 && (ty == 4 || ty == 8));
-}
-
-bool fused_shape_paired(int tx) { return tx == 128; }
-
-bool fused_shape_valid(int tx, int ty, int dtype, Geom g)
-{
-    if (!fused_shape_supported(tx, ty))
-        return false;
-    // the paired march: single precision and an even row length
-    return !fused_shape_paired(tx) || (dtype == QSM_F32 && g.nx % 2 == 0);
+           || (tx == 32 && (ty == 8 || ty == 16 || ty == 32));
 }
 
 bool fused_size_supported(int kind, int dtype, size_t N)
@@ -190,19 +121,6 @@ hipError_t launch_fused(const T *in, T *out, size_t N, const T *f,
         return hipSuccess;
     if (sh.kz < 1 || !fused_shape_supported(sh.tx, sh.ty))
         return hipErrorInvalidValue;
-    if (fused_shape_paired(sh.tx)) {
-        if constexpr (std::is_same<T, float>::value) {
-            if (g.nx % 2)
-                return hipErrorInvalidValue;
-            if (sh.ty == 4)
-                return launch_pair_shape<TGV, 64, 4>(in, out, N, f, mk, g, sh, dc, pc, stop,
-                                                     partials, st);
-            return launch_pair_shape<TGV, 64, 8>(in, out, N, f, mk, g, sh, dc, pc, stop,
-                                                 partials, st);
-        } else {
-            return hipErrorInvalidValue;
-        }
-    }
 #define QSM_FUSED_CASE(X, Y)                                                   \
     if (sh.tx == X && sh.ty == Y)                                              \
         return launch_shape<T, TGV, X, Y>(in, out, N, f, mk, g, sh, dc, pc,    \

@@ -42,21 +42,33 @@ Layout layout(int kind, int dtype, const size_t sz[3], int npart, int hist_cap)
 // most blocks a shape can have) and the two-sweep primal.
 int max_partials(Geom g)
 {
-    static const int shapes[][2] = {{64, 4}, {64, 8}, {64, 16}, {32, 8}, {32, 16}, {32, 32},
-                                    {128, 4}, {128, 8}};
+    static const int shapes[][2] = {{64, 4}, {64, 8}, {64, 16}, {32, 8}, {32, 16}, {32, 32}};
     int n = std::max(primal_sweep_blocks(g), 1);
     for (const auto &sh : shapes)
         n = std::max(n, fused_blocks(g, FusedShape{sh[0], sh[1], 1}));
     return n;
 }
 
-// Default fused tile shape per precision/model (tuned on MI355X; the
-// batching runtime's autotuner may pick others).
-FusedShape default_shape(int kind, int dtype)
+// Default fused tile shape (the autotuner's table may pick others): 64x16
+// tiles (64x8 for double TGV, whose registers allow fewer waves per CU),
+// and z-chunks sized so that the grid is at most one block per CU -- the
+// march keeps every block resident, and a partial second wave of blocks
+// costs a whole chunk.  On MI355X this reproduces the tuned z-chunks of
+// 96^3 to 256x256x176 (e.g. FP32 64x16x44, FP64 TGV 64x8x87 at
+// 256x256x176).
+FusedShape default_shape(int kind, int dtype, Geom g)
 {
-    if (dtype == QSM_F64 && kind == QSM_TGV)
-        return FusedShape{64, 8, 16};
-    return FusedShape{64, 8, 16};
+    FusedShape sh{64, (dtype == QSM_F64 && kind == QSM_TGV) ? 8 : 16, 16};
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) != hipSuccess
+        || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess
+        || ncu < 1)
+        ncu = 256;
+    const int inner = std::max(1, g.nz - 2);
+    const int nxy = ((g.nx + sh.tx - 1) / sh.tx) * ((g.ny + sh.ty - 1) / sh.ty);
+    const int nch = std::max(1, ncu / nxy);  // at most one block per CU
+    sh.kz = std::max(1, (inner + nch - 1) / nch);
+    return sh;
 }
 
 }  // namespace
@@ -223,7 +235,7 @@ int create_impl(qsm_pd **out, int kind, int dtype, const size_t sz[3],
     s->lam = lam;
     s->alpha = alpha;
     s->stream = as_stream(stream);
-    s->shape = default_shape(kind, dtype);
+    s->shape = default_shape(kind, dtype, Geom{int(sz[0]), int(sz[1]), int(sz[2])});
     // Banks above 4 GiB exceed the fused pass's 32-bit offsets: use the
     // two-sweep schedule (same results) for such volumes.
     s->schedule = fused_size_supported(kind, dtype, s->N) ? 0 : 1;
@@ -427,8 +439,6 @@ int qsm_pd_set_tiles(qsm_pd *s, int tx, int ty, int kz)
 {
     QSM_ARG(s, "null handle");
     QSM_ARG(fused_shape_supported(tx, ty), "unsupported tile shape %dx%d", tx, ty);
-    QSM_ARG(fused_shape_valid(tx, ty, s->dtype, s->geom()),
-            "tile shape %dx%d (two voxels per lane) needs single precision and an even nx", tx, ty);
     QSM_ARG(kz >= 1, "kz must be >= 1");
     const FusedShape sh{tx, ty, kz};
     QSM_ARG(fused_blocks(s->geom(), sh) <= s->npart,

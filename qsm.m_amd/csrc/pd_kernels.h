@@ -79,10 +79,6 @@ hipError_t launch_fused(const T *in, T *out, size_t N, const T *f,
 
 int fused_blocks(Geom g, FusedShape sh);
 bool fused_shape_supported(int tx, int ty);
-// tx = 128 selects the paired march (two voxels per lane): single
-// precision and an even nx only.
-bool fused_shape_paired(int tx);
-bool fused_shape_valid(int tx, int ty, int dtype, Geom g);
 // The fused pass addresses a bank with 32-bit byte offsets (< 4 GiB).
 bool fused_size_supported(int kind, int dtype, size_t N);
 

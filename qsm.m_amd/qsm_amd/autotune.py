@@ -22,9 +22,7 @@ import numpy as np
 
 from . import _lib
 
-# (tx, ty) in voxels; 128-wide tiles are the two-voxels-per-lane march
-# (single precision, even nx), which the solver refuses otherwise
-SHAPES = [(64, 4), (64, 8), (64, 16), (32, 8), (32, 16), (32, 32), (128, 4), (128, 8)]
+SHAPES = [(64, 4), (64, 8), (64, 16), (32, 8), (32, 16), (32, 32)]
 PACKAGED = Path(__file__).resolve().parent / "tuning_mi355x.json"
 USER = Path(os.environ.get("QSM_TUNING", Path.home() / ".cache" / "qsm_amd" / "tuning.json"))
 DEFAULT = (64, 8, 16)

@@ -247,37 +247,3 @@ def test_fused_tile_shapes(cuda, dt, kind):
                 got = s.chi()
             np.testing.assert_array_equal(got, ref, err_msg=f"tiles {tx}x{ty} kz={kz}")
 
-
-@pytest.mark.parametrize("kind", ["tgv", "tv"])
-@pytest.mark.parametrize("shape", [(70, 45, 33), (130, 20, 17), (4, 9, 8), (258, 7, 12)])
-def test_paired_march_matches_oracle(cuda, kind, shape):
-    """The two-voxels-per-lane march (tiles 128 wide, single precision):
-    partial tiles in x, y and z, a holed mask, tol = 0 and a tolerance stop
-    -- chi and the iteration count equal the oracle's bit for bit."""
-    h = (0.9, 1.0, 1.1)
-    d2f = _phantom_input(shape, h, 15, np.float32)
-    m = brainish_mask(shape, seed=9, frac=0.95)
-    k = _lib.TGV if kind == "tgv" else _lib.TV
-    for iters, tol in ((23, 0.0), (400, 3e-2)):
-        if kind == "tgv":
-            ref, it, _ = orc.pd_tgv(d2f, m, h, 2e-3, 2.0, tol, iters)
-        else:
-            ref, it, _ = orc.pd_tv(d2f, m, h, 2e-3, tol, iters)
-        for ty in (4, 8):
-            for kz in (1, 5, 31):
-                with pd.Solver(k, np.float32, shape, h, 2e-3, 2.0) as s:
-                    s.set_tiles(128, ty, kz)
-                    s.load(d2f, m).iterate(iters, tol)
-                    got = s.chi()
-                    n, stopped, _ = s.status()
-                assert n == it, f"128x{ty} kz={kz} tol={tol}"
-                np.testing.assert_array_equal(got, ref, err_msg=f"128x{ty} kz={kz} tol={tol}")
-
-
-def test_paired_march_rejects_double_and_odd_rows(cuda):
-    with pd.Solver(_lib.TGV, np.float64, (20, 10, 10), (1, 1, 1), 1e-3, 2.0) as s:
-        with pytest.raises(_lib.QsmError, match="single precision and an even nx"):
-            s.set_tiles(128, 8, 8)
-    with pd.Solver(_lib.TGV, np.float32, (21, 10, 10), (1, 1, 1), 1e-3, 2.0) as s:
-        with pytest.raises(_lib.QsmError, match="single precision and an even nx"):
-            s.set_tiles(128, 8, 8)
