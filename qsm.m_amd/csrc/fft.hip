@@ -54,6 +54,27 @@ struct Plan {
     int R[MAX_STAGES];    // radix of stage s
     int Ns[MAX_STAGES];   // sub-transform length before stage s
     int tw[MAX_STAGES];   // offset of stage s's twiddles in the table
+    unsigned mNs[MAX_STAGES];  // exact reciprocals (udiv) of Ns[s] and of M
+    unsigned mM;
+};
+
+// n / d for n < 2^16 and 1 <= d <= 4096 as a multiply-high by
+// m = ceil(2^32 / d) (exact in that range; m = 0 stands for d = 1).
+__host__ __device__ inline unsigned recip(unsigned d)
+{
+    return d == 1 ? 0u : unsigned(((1ull << 32) + d - 1) / d);
+}
+
+__device__ __forceinline__ unsigned udiv(unsigned n, unsigned m)
+{
+    return m ? __umulhi(n, m) : n;
+}
+
+// Pencils of a block in LDS: element i of pencil p at [i*PP + p], P a
+// power of two (shifts instead of divisions) and PP = P + 1 (the odd row
+// stride keeps a wavefront's accesses along i on distinct LDS banks).
+struct Pen {
+    int logP, PP;
 };
 
 // Small DFT of v[0..R-1] in place: y_k = sum_r v_r exp(sgn*2*pi*i*r*k/R),
@@ -161,48 +182,52 @@ __device__ __forceinline__ void small_dft(Cx<T> *v, T sgn)
     }
 }
 
-// One radix-R Stockham stage: A -> B (element i of pencil p at [i*P + p]).
+// One radix-R Stockham stage: A -> B.
 template <typename T, int R>
-__device__ __forceinline__ void stage(const Cx<T> *A, Cx<T> *B, int P, int M, int Ns,
-                                      const Cx<T> *__restrict__ tws, T sgn)
+__device__ __forceinline__ void stage(const Cx<T> *A, Cx<T> *B, Pen pn, int M, int Ns,
+                                      unsigned mNs, const Cx<T> *__restrict__ tws, T sgn)
 {
     const int nb = M / R;
-    for (int b = threadIdx.x; b < nb * P; b += FFT_THREADS) {
-        const int j = b / P, p = b % P;
-        const int k = j % Ns;
+    const int PP = pn.PP, pm = (1 << pn.logP) - 1;
+    for (int b = threadIdx.x; b < (nb << pn.logP); b += FFT_THREADS) {
+        const int j = b >> pn.logP, p = b & pm;
+        const int q = int(udiv(unsigned(j), mNs));
+        const int k = j - q * Ns;
         Cx<T> v[R];
-        v[0] = A[j * P + p];
+        v[0] = A[j * PP + p];
 #pragma unroll
         for (int r = 1; r < R; ++r) {
             const Cx<T> w = tws[k * (R - 1) + (r - 1)];
             const Cx<T> wt{w.re, sgn < 0 ? w.im : -w.im};
-            v[r] = cmul(A[(j + r * nb) * P + p], wt);
+            v[r] = cmul(A[(j + r * nb) * PP + p], wt);
         }
         small_dft<T, R>(v, sgn);
-        const int base = (j / Ns) * Ns * R + k;
+        const int base = q * Ns * R + k;
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            B[(base + r * Ns) * P + p] = v[r];
+            B[(base + r * Ns) * PP + p] = v[r];
     }
 }
 
-// Stockham transform of P pencils held in LDS; returns the buffer holding
-// the result.  sgn = -1 forward, +1 inverse (twiddles conjugated).
+// Stockham transform of the block's pencils held in LDS; returns the
+// buffer holding the result.  sgn = -1 forward, +1 inverse (twiddles
+// conjugated).
 template <typename T>
-__device__ Cx<T> *stockham(Cx<T> *A, Cx<T> *B, int P, const Plan *__restrict__ pl,
+__device__ Cx<T> *stockham(Cx<T> *A, Cx<T> *B, Pen pn, const Plan *__restrict__ pl,
                            const Cx<T> *__restrict__ tw, T sgn)
 {
     const int M = pl->M, nst = pl->nst;
     for (int s = 0; s < nst; ++s) {
         const int R = pl->R[s], Ns = pl->Ns[s];
+        const unsigned mNs = pl->mNs[s];
         const Cx<T> *tws = tw + pl->tw[s];
         switch (R) {
-        case 2: stage<T, 2>(A, B, P, M, Ns, tws, sgn); break;
-        case 3: stage<T, 3>(A, B, P, M, Ns, tws, sgn); break;
-        case 4: stage<T, 4>(A, B, P, M, Ns, tws, sgn); break;
-        case 5: stage<T, 5>(A, B, P, M, Ns, tws, sgn); break;
-        case 7: stage<T, 7>(A, B, P, M, Ns, tws, sgn); break;
-        default: stage<T, 8>(A, B, P, M, Ns, tws, sgn); break;
+        case 2: stage<T, 2>(A, B, pn, M, Ns, mNs, tws, sgn); break;
+        case 3: stage<T, 3>(A, B, pn, M, Ns, mNs, tws, sgn); break;
+        case 4: stage<T, 4>(A, B, pn, M, Ns, mNs, tws, sgn); break;
+        case 5: stage<T, 5>(A, B, pn, M, Ns, mNs, tws, sgn); break;
+        case 7: stage<T, 7>(A, B, pn, M, Ns, mNs, tws, sgn); break;
+        default: stage<T, 8>(A, B, pn, M, Ns, mNs, tws, sgn); break;
         }
         __syncthreads();
         Cx<T> *t = A; A = B; B = t;
@@ -221,11 +246,12 @@ struct Vol {
 // ---- generic in-place pass (plain fft3 / ifft3) -------------------------
 template <typename T>
 __global__ __launch_bounds__(FFT_THREADS) void fft_pass_kernel(
-    Cx<T> *data, Vol v, int axis, int P, const Plan *pl, const Cx<T> *tw, T sgn, T scale)
+    Cx<T> *data, Vol v, int axis, Pen pn, const Plan *pl, const Cx<T> *tw, T sgn, T scale)
 {
     extern __shared__ unsigned char smem[];
     Cx<T> *A = reinterpret_cast<Cx<T> *>(smem);
-    Cx<T> *B = A + size_t(P) * pl->M;
+    const int P = 1 << pn.logP, PP = pn.PP;
+    Cx<T> *B = A + size_t(PP) * pl->M;
     const int M = pl->M;
     size_t base;
     size_t stride;  // element stride along the axis
@@ -249,18 +275,18 @@ __global__ __launch_bounds__(FFT_THREADS) void fft_pass_kernel(
     }
     for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
         int p, i;
-        if (axis == 0) { p = e / M; i = e % M; }
-        else { i = e / P; p = e % P; }
-        A[i * P + p] = p < np ? data[base + size_t(p) * pstride + size_t(i) * stride] : Cx<T>{T(0), T(0)};
+        if (axis == 0) { p = int(udiv(unsigned(e), pl->mM)); i = e - p * M; }
+        else { i = e >> pn.logP; p = e & (P - 1); }
+        A[i * PP + p] = p < np ? data[base + size_t(p) * pstride + size_t(i) * stride] : Cx<T>{T(0), T(0)};
     }
     __syncthreads();
-    Cx<T> *R = stockham(A, B, P, pl, tw, sgn);
+    Cx<T> *R = stockham(A, B, pn, pl, tw, sgn);
     for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
         int p, i;
-        if (axis == 0) { p = e / M; i = e % M; }
-        else { i = e / P; p = e % P; }
+        if (axis == 0) { p = int(udiv(unsigned(e), pl->mM)); i = e - p * M; }
+        else { i = e >> pn.logP; p = e & (P - 1); }
         if (p < np) {
-            Cx<T> x = R[i * P + p];
+            Cx<T> x = R[i * PP + p];
             x.re *= scale;
             x.im *= scale;
             data[base + size_t(p) * pstride + size_t(i) * stride] = x;
@@ -275,32 +301,33 @@ __global__ __launch_bounds__(FFT_THREADS) void fft_pass_kernel(
 template <typename T>
 __global__ __launch_bounds__(FFT_THREADS) void dip_x_fwd(
     Cx<T> *data, const T *chi, Vol v, int nx, int ny, int nz, int o0, int o1, int o2,
-    int P, const Plan *pl, const Cx<T> *tw)
+    Pen pn, const Plan *pl, const Cx<T> *tw)
 {
     extern __shared__ unsigned char smem[];
     Cx<T> *A = reinterpret_cast<Cx<T> *>(smem);
-    Cx<T> *B = A + size_t(P) * pl->M;
+    const int P = 1 << pn.logP, PP = pn.PP;
+    Cx<T> *B = A + size_t(PP) * pl->M;
     const int M = pl->M;
     const int rows = ny * nz;
     const int r0 = blockIdx.x * P;
     const int np = min(P, rows - r0);
     for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
-        const int p = e / M, i = e % M;
+        const int p = int(udiv(unsigned(e), pl->mM)), i = e - p * M;
         T x = T(0);
         if (p < np && i >= o0 && i < o0 + nx) {
             const int r = r0 + p;
             x = chi[size_t(i - o0) + size_t(nx) * r];
         }
-        A[i * P + p] = Cx<T>{x, T(0)};
+        A[i * PP + p] = Cx<T>{x, T(0)};
     }
     __syncthreads();
-    Cx<T> *R = stockham(A, B, P, pl, tw, T(-1));
+    Cx<T> *R = stockham(A, B, pn, pl, tw, T(-1));
     for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
-        const int p = e / M, i = e % M;
+        const int p = int(udiv(unsigned(e), pl->mM)), i = e - p * M;
         if (p < np) {
             const int r = r0 + p;
             const int j = r % ny + o1, k = r / ny + o2;
-            data[size_t(i) + size_t(v.M0) * (size_t(j) + size_t(v.M1) * k)] = R[i * P + p];
+            data[size_t(i) + size_t(v.M0) * (size_t(j) + size_t(v.M1) * k)] = R[i * PP + p];
         }
     }
 }
@@ -310,12 +337,13 @@ __global__ __launch_bounds__(FFT_THREADS) void dip_x_fwd(
 // [slo, shi) are stored (the later passes read no others).
 template <typename T>
 __global__ __launch_bounds__(FFT_THREADS) void dip_y(
-    Cx<T> *data, Vol v, int k0, int P, const Plan *pl, const Cx<T> *tw, T sgn,
+    Cx<T> *data, Vol v, int k0, Pen pn, const Plan *pl, const Cx<T> *tw, T sgn,
     int ylo, int yhi, int slo, int shi)
 {
     extern __shared__ unsigned char smem[];
     Cx<T> *A = reinterpret_cast<Cx<T> *>(smem);
-    Cx<T> *B = A + size_t(P) * pl->M;
+    const int P = 1 << pn.logP, PP = pn.PP;
+    Cx<T> *B = A + size_t(PP) * pl->M;
     const int M = pl->M;
     const int nxb = (v.M0 + P - 1) / P;
     const int x0 = (blockIdx.x % nxb) * P;
@@ -323,16 +351,16 @@ __global__ __launch_bounds__(FFT_THREADS) void dip_y(
     const int np = min(P, v.M0 - x0);
     const size_t base = size_t(x0) + size_t(k) * v.M0 * v.M1;
     for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
-        const int i = e / P, p = e % P;
-        A[i * P + p] = (p < np && i >= ylo && i < yhi) ? data[base + p + size_t(i) * v.M0]
+        const int i = e >> pn.logP, p = e & (P - 1);
+        A[i * PP + p] = (p < np && i >= ylo && i < yhi) ? data[base + p + size_t(i) * v.M0]
                                                        : Cx<T>{T(0), T(0)};
     }
     __syncthreads();
-    Cx<T> *R = stockham(A, B, P, pl, tw, sgn);
+    Cx<T> *R = stockham(A, B, pn, pl, tw, sgn);
     for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
-        const int i = e / P, p = e % P;
+        const int i = e >> pn.logP, p = e & (P - 1);
         if (p < np && i >= slo && i < shi)
-            data[base + p + size_t(i) * v.M0] = R[i * P + p];
+            data[base + p + size_t(i) * v.M0] = R[i * PP + p];
     }
 }
 
@@ -350,13 +378,14 @@ __device__ __forceinline__ double kcoord(int m, int n, double h)
 // output: the others are neither read (zero) nor stored.
 template <typename T>
 __global__ __launch_bounds__(FFT_THREADS) void dip_z(
-    Cx<T> *data, Vol v, int P, const Plan *pl, const Cx<T> *tw,
+    Cx<T> *data, Vol v, Pen pn, const Plan *pl, const Cx<T> *tw,
     double h0, double h1, double h2, double b0, double b1, double b2, double inv_n,
     int zlo, int zhi)
 {
     extern __shared__ unsigned char smem[];
     Cx<T> *A = reinterpret_cast<Cx<T> *>(smem);
-    Cx<T> *B = A + size_t(P) * pl->M;
+    const int P = 1 << pn.logP, PP = pn.PP;
+    Cx<T> *B = A + size_t(PP) * pl->M;
     const int M = pl->M;
     const int nxb = (v.M0 + P - 1) / P;
     const int x0 = (blockIdx.x % nxb) * P;
@@ -365,34 +394,34 @@ __global__ __launch_bounds__(FFT_THREADS) void dip_z(
     const size_t plane = size_t(v.M0) * v.M1;
     const size_t base = size_t(x0) + size_t(j) * v.M0;
     for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
-        const int i = e / P, p = e % P;
-        A[i * P + p] = (p < np && i >= zlo && i < zhi) ? data[base + p + size_t(i) * plane]
+        const int i = e >> pn.logP, p = e & (P - 1);
+        A[i * PP + p] = (p < np && i >= zlo && i < zhi) ? data[base + p + size_t(i) * plane]
                                                        : Cx<T>{T(0), T(0)};
     }
     __syncthreads();
-    Cx<T> *R = stockham(A, B, P, pl, tw, T(-1));
+    Cx<T> *R = stockham(A, B, pn, pl, tw, T(-1));
     Cx<T> *S = (R == A) ? B : A;
     // D(k) = 1/3 - (k.b)^2 / |k|^2, D(0) = 0 (dipoleKernel.m:59-66), in double
     const double Y = kcoord(j, v.M1, h1);
     for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
-        const int i = e / P, p = e % P;
+        const int i = e >> pn.logP, p = e & (P - 1);
         const double X = kcoord(x0 + p, v.M0, h0);
         const double Z = kcoord(i, M, h2);
         const double kz = (b0 * X + b1 * Y) + b2 * Z;
         const double k2 = (X * X + Y * Y) + Z * Z;
         const double D = (k2 == 0.0) ? 0.0 : (1.0 / 3.0 - (kz * kz) / k2);
         const T d = T(D * inv_n);
-        Cx<T> x = R[i * P + p];
+        Cx<T> x = R[i * PP + p];
         x.re *= d;
         x.im *= d;
-        R[i * P + p] = x;
+        R[i * PP + p] = x;
     }
     __syncthreads();
-    Cx<T> *Q = stockham(R, S, P, pl, tw, T(1));
+    Cx<T> *Q = stockham(R, S, pn, pl, tw, T(1));
     for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
-        const int i = e / P, p = e % P;
+        const int i = e >> pn.logP, p = e & (P - 1);
         if (p < np && i >= zlo && i < zhi)
-            data[base + p + size_t(i) * plane] = Q[i * P + p];
+            data[base + p + size_t(i) * plane] = Q[i * PP + p];
     }
 }
 
@@ -401,32 +430,33 @@ __global__ __launch_bounds__(FFT_THREADS) void dip_z(
 template <typename T>
 __global__ __launch_bounds__(FFT_THREADS) void dip_x_inv(
     const Cx<T> *data, T *out, Vol v, int nx, int ny, int nz, int o0, int o1, int o2,
-    int P, const Plan *pl, const Cx<T> *tw)
+    Pen pn, const Plan *pl, const Cx<T> *tw)
 {
     extern __shared__ unsigned char smem[];
     Cx<T> *A = reinterpret_cast<Cx<T> *>(smem);
-    Cx<T> *B = A + size_t(P) * pl->M;
+    const int P = 1 << pn.logP, PP = pn.PP;
+    Cx<T> *B = A + size_t(PP) * pl->M;
     const int M = pl->M;
     const int rows = ny * nz;
     const int r0 = blockIdx.x * P;
     const int np = min(P, rows - r0);
     for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
-        const int p = e / M, i = e % M;
+        const int p = int(udiv(unsigned(e), pl->mM)), i = e - p * M;
         Cx<T> x{T(0), T(0)};
         if (p < np) {
             const int r = r0 + p;
             const int j = r % ny + o1, k = r / ny + o2;
             x = data[size_t(i) + size_t(v.M0) * (size_t(j) + size_t(v.M1) * k)];
         }
-        A[i * P + p] = x;
+        A[i * PP + p] = x;
     }
     __syncthreads();
-    Cx<T> *R = stockham(A, B, P, pl, tw, T(1));
+    Cx<T> *R = stockham(A, B, pn, pl, tw, T(1));
     for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
-        const int p = e / M, i = e % M;
+        const int p = int(udiv(unsigned(e), pl->mM)), i = e - p * M;
         if (p < np && i >= o0 && i < o0 + nx) {
             const int r = r0 + p;
-            out[size_t(i - o0) + size_t(nx) * r] = R[i * P + p].re;
+            out[size_t(i - o0) + size_t(nx) * r] = R[i * PP + p].re;
         }
     }
 }
@@ -436,6 +466,7 @@ __global__ __launch_bounds__(FFT_THREADS) void dip_x_inv(
 bool factor(int M, Plan &pl)
 {
     pl.M = M;
+    pl.mM = recip(unsigned(M));
     pl.nst = 0;
     int rest = M, Ns = 1, off = 0;
     const int radices[] = {8, 7, 5, 4, 3, 2};
@@ -447,6 +478,7 @@ bool factor(int M, Plan &pl)
             return false;
         pl.R[pl.nst] = R;
         pl.Ns[pl.nst] = Ns;
+        pl.mNs[pl.nst] = recip(unsigned(Ns));
         pl.tw[pl.nst] = off;
         off += Ns * (R - 1);
         Ns *= R;
@@ -480,15 +512,47 @@ std::vector<Cx<double>> twiddles(const Plan &pl)
     return t;
 }
 
-// Pencils per block: as many as fit a 64 KB LDS budget for the two
-// buffers, at most 16 (so that P consecutive x positions cover whole cache
-// lines), at least 1.
-int pencils(int M, size_t csize)
+// LDS per block: two buffers of P pencils.  80 KB lets two blocks share a
+// CU's 160 KB.
+constexpr size_t LDS_BUDGET = 80 * 1024;
+// one pencil of the longest supported length (2048 double, 4096 single)
+constexpr size_t LDS_MAX = 64 * 1024;
+static_assert(LDS_MAX <= LDS_BUDGET, "the pass kernels are allowed LDS_BUDGET bytes");
+
+size_t lds_bytes(Pen pn, int M, size_t csize) { return 2 * size_t(pn.PP) * M * csize; }
+
+// Pencils per block: the largest power of two P <= 16 (16 consecutive x
+// positions are whole cache lines) whose buffers fit the budget.
+Pen pencils(int M, size_t csize)
 {
-    int P = int((64 * 1024) / (2 * size_t(M) * csize));
-    if (P > 16) P = 16;
-    if (P < 1) P = 1;
-    return P;
+    for (int logP = 4; logP > 0; --logP) {
+        const Pen pn{logP, (1 << logP) + 1};
+        if (lds_bytes(pn, M, csize) <= LDS_BUDGET)
+            return pn;
+    }
+    return Pen{0, 1};
+}
+
+// Lets the pass kernels use more than the default 64 KB of dynamic LDS.
+template <typename T>
+int allow_lds()
+{
+    static const hipError_t e = [] {
+        const void *k[] = {reinterpret_cast<const void *>(&fft_pass_kernel<T>),
+                           reinterpret_cast<const void *>(&dip_x_fwd<T>),
+                           reinterpret_cast<const void *>(&dip_y<T>),
+                           reinterpret_cast<const void *>(&dip_z<T>),
+                           reinterpret_cast<const void *>(&dip_x_inv<T>)};
+        for (const void *f : k) {
+            const hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                     int(LDS_BUDGET));
+            if (r != hipSuccess)
+                return r;
+        }
+        return hipSuccess;
+    }();
+    QSM_HIP(e);
+    return QSM_OK;
 }
 
 struct AxisPlan {
@@ -525,8 +589,8 @@ PlanCache &plan_cache()
 template <typename T>
 int make_axis(AxisPlan &ap, size_t M, const char *what)
 {
-    QSM_ARG(M >= 1 && 2 * M * sizeof(Cx<T>) <= 65536, "%s %zu too long (at most %zu)",
-            what, M, 65536 / (2 * sizeof(Cx<T>)));
+    QSM_ARG(M >= 1 && lds_bytes(Pen{0, 1}, int(M), sizeof(Cx<T>)) <= LDS_MAX,
+            "%s %zu too long (at most %zu)", what, M, LDS_MAX / (2 * sizeof(Cx<T>)));
     int dev = 0;
     QSM_HIP(hipGetDevice(&dev));
     PlanCache &pc = plan_cache();
@@ -568,20 +632,22 @@ int fft3_dev_t(void *data, const size_t Msz[3], int inverse, hipStream_t st)
     int rc;
     for (int a = 0; a < 3; ++a)
         if ((rc = make_axis<T>(ap[a], Msz[a], "FFT length"))) return rc;
+    if ((rc = allow_lds<T>())) return rc;
     const Vol v{int(Msz[0]), int(Msz[1]), int(Msz[2])};
     const T sgn = inverse ? T(1) : T(-1);
     // fft3.m:5 and ifft3.m:5 apply axis 3, then 2, then 1
     for (int a = 2; a >= 0; --a) {
         const Plan &pl = ap[a].pl;
         if (pl.M == 1) continue;
-        const int P = pencils(pl.M, sizeof(Cx<T>));
-        const size_t lds = 2 * size_t(P) * pl.M * sizeof(Cx<T>);
+        const Pen pn = pencils(pl.M, sizeof(Cx<T>));
+        const int P = 1 << pn.logP;
+        const size_t lds = lds_bytes(pn, pl.M, sizeof(Cx<T>));
         int nblk;
         if (a == 0) nblk = (v.M1 * v.M2 + P - 1) / P;
         else nblk = ((v.M0 + P - 1) / P) * (a == 1 ? v.M2 : v.M1);
         const T scale = inverse ? T(1.0 / double(pl.M)) : T(1);
         fft_pass_kernel<T><<<nblk, FFT_THREADS, lds, st>>>(
-            static_cast<Cx<T> *>(data), v, a, P, ap[a].dpl, tw_of<T>(ap[a]), sgn, scale);
+            static_cast<Cx<T> *>(data), v, a, pn, ap[a].dpl, tw_of<T>(ap[a]), sgn, scale);
         QSM_HIP(hipGetLastError());
     }
     return QSM_OK;
@@ -596,6 +662,7 @@ int dipole_conv_dev_t(const void *chi, const size_t sz[3], const double h[3],
     AxisPlan ap[3];
     for (int a = 0; a < 3; ++a)
         if ((rc = make_axis<T>(ap[a], M[a], "padded length"))) return rc;
+    if ((rc = allow_lds<T>())) return rc;
     const Vol v{int(M[0]), int(M[1]), int(M[2])};
     const int nx = int(sz[0]), ny = int(sz[1]), nz = int(sz[2]);
     const int o0 = int(off[0]), o1 = int(off[1]), o2 = int(off[2]);
@@ -613,38 +680,43 @@ int dipole_conv_dev_t(const void *chi, const size_t sz[3], const double h[3],
     const size_t cs = sizeof(Cx<T>);
     {   // x forward on data rows
         const Plan &pl = ap[0].pl;
-        const int P = pencils(pl.M, cs);
-        dip_x_fwd<T><<<(ny * nz + P - 1) / P, FFT_THREADS, 2 * size_t(P) * pl.M * cs, st>>>(
-            d, static_cast<const T *>(chi), v, nx, ny, nz, o0, o1, o2, P, ap[0].dpl, tw_of<T>(ap[0]));
+        const Pen pn = pencils(pl.M, cs);
+        const int P = 1 << pn.logP;
+        dip_x_fwd<T><<<(ny * nz + P - 1) / P, FFT_THREADS, lds_bytes(pn, pl.M, cs), st>>>(
+            d, static_cast<const T *>(chi), v, nx, ny, nz, o0, o1, o2, pn, ap[0].dpl, tw_of<T>(ap[0]));
         QSM_HIP(hipGetLastError());
     }
     {   // y forward on data planes
         const Plan &pl = ap[1].pl;
-        const int P = pencils(pl.M, cs);
-        dip_y<T><<<((v.M0 + P - 1) / P) * nz, FFT_THREADS, 2 * size_t(P) * pl.M * cs, st>>>(
-            d, v, o2, P, ap[1].dpl, tw_of<T>(ap[1]), T(-1), o1, o1 + ny, 0, v.M1);
+        const Pen pn = pencils(pl.M, cs);
+        const int P = 1 << pn.logP;
+        dip_y<T><<<((v.M0 + P - 1) / P) * nz, FFT_THREADS, lds_bytes(pn, pl.M, cs), st>>>(
+            d, v, o2, pn, ap[1].dpl, tw_of<T>(ap[1]), T(-1), o1, o1 + ny, 0, v.M1);
         QSM_HIP(hipGetLastError());
     }
     {   // z forward * D * z inverse, scaled by 1/(M0 M1 M2)
         const Plan &pl = ap[2].pl;
-        const int P = pencils(pl.M, cs);
-        dip_z<T><<<((v.M0 + P - 1) / P) * v.M1, FFT_THREADS, 2 * size_t(P) * pl.M * cs, st>>>(
-            d, v, P, ap[2].dpl, tw_of<T>(ap[2]), h[0], h[1], h[2],
+        const Pen pn = pencils(pl.M, cs);
+        const int P = 1 << pn.logP;
+        dip_z<T><<<((v.M0 + P - 1) / P) * v.M1, FFT_THREADS, lds_bytes(pn, pl.M, cs), st>>>(
+            d, v, pn, ap[2].dpl, tw_of<T>(ap[2]), h[0], h[1], h[2],
             b[0], b[1], b[2], 1.0 / double(total), o2, o2 + nz);
         QSM_HIP(hipGetLastError());
     }
     {   // y inverse on output planes
         const Plan &pl = ap[1].pl;
-        const int P = pencils(pl.M, cs);
-        dip_y<T><<<((v.M0 + P - 1) / P) * nz, FFT_THREADS, 2 * size_t(P) * pl.M * cs, st>>>(
-            d, v, o2, P, ap[1].dpl, tw_of<T>(ap[1]), T(1), 0, v.M1, o1, o1 + ny);
+        const Pen pn = pencils(pl.M, cs);
+        const int P = 1 << pn.logP;
+        dip_y<T><<<((v.M0 + P - 1) / P) * nz, FFT_THREADS, lds_bytes(pn, pl.M, cs), st>>>(
+            d, v, o2, pn, ap[1].dpl, tw_of<T>(ap[1]), T(1), 0, v.M1, o1, o1 + ny);
         QSM_HIP(hipGetLastError());
     }
     {   // x inverse on output rows, crop + real part
         const Plan &pl = ap[0].pl;
-        const int P = pencils(pl.M, cs);
-        dip_x_inv<T><<<(ny * nz + P - 1) / P, FFT_THREADS, 2 * size_t(P) * pl.M * cs, st>>>(
-            d, static_cast<T *>(field), v, nx, ny, nz, o0, o1, o2, P, ap[0].dpl, tw_of<T>(ap[0]));
+        const Pen pn = pencils(pl.M, cs);
+        const int P = 1 << pn.logP;
+        dip_x_inv<T><<<(ny * nz + P - 1) / P, FFT_THREADS, lds_bytes(pn, pl.M, cs), st>>>(
+            d, static_cast<T *>(field), v, nx, ny, nz, o0, o1, o2, pn, ap[0].dpl, tw_of<T>(ap[0]));
         QSM_HIP(hipGetLastError());
     }
     return QSM_OK;

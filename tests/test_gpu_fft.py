@@ -101,3 +101,15 @@ def test_dipole_conv_device_form_on_side_stream(cuda, dt):
         got = got.reshape(shape, order="F")
         assert got.dtype == want.dtype
         np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("dt,n", [(np.complex128, 2048), (np.complex64, 4096), (np.complex128, 1960)])
+def test_fft3_longest_lengths(cuda, dt, n):
+    """The longest supported transform lengths (one pencil per block, more
+    than 64 KB of LDS) against pocketfft."""
+    rng = np.random.default_rng(12)
+    x = (rng.standard_normal((n, 2, 3)) + 1j * rng.standard_normal((n, 2, 3))).astype(dt)
+    tol = 1e-13 if dt == np.complex128 else 3e-6
+    X = dipole.fft3(x)
+    ref = np.fft.fftn(x.astype(np.complex128))
+    assert np.linalg.norm(X - ref) / np.linalg.norm(ref) < tol
