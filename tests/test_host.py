@@ -151,3 +151,30 @@ def test_harness_field_is_the_toolbox_idiom_for_tilted_b0():
     D = fastfft.dipole_kernel(up.shape, spec.vsz, spec.bdir)
     ref = np.real(np.fft.ifftn(D * np.fft.fftn(up)))[fastfft.unpad_slices(up.shape, spec.shape, pad)]
     assert np.linalg.norm(ph["field"] - ref) <= 1e-13 * np.linalg.norm(ref)
+
+
+def test_sstgv_rejects_bad_arguments_before_any_device_work():
+    """sstgv.m:92-136 checks, restated: every rejection happens on the host."""
+    from qsm_amd.sstgv import sstgv
+
+    ph = np.zeros((6, 6, 6))
+    m = np.ones((6, 6, 6), bool)
+    with pytest.raises(TypeError):
+        sstgv(ph.astype(np.int32), m, (1, 1, 1))
+    with pytest.raises(ValueError, match="mask"):
+        sstgv(ph, np.ones((6, 6, 5)), (1, 1, 1))
+    with pytest.raises(ValueError, match="binary"):
+        sstgv(ph, 2 * np.ones((6, 6, 6)), (1, 1, 1))
+    with pytest.raises(ValueError, match="vsz"):
+        sstgv(ph, m, (1, 1, 0))
+    with pytest.raises(ValueError, match="lambda"):
+        sstgv(ph, m, (1, 1, 1), lam=-1.0)
+    with pytest.raises(ValueError, match="maxit"):
+        sstgv(ph, m, (1, 1, 1), maxit=0)
+    with pytest.raises(ValueError, match="finite"):
+        bad = ph.copy()
+        bad[0, 0, 0] = np.nan
+        sstgv(bad, m, (1, 1, 1))
+    # an empty mask returns zeros without touching the device
+    out = sstgv(ph, np.zeros((6, 6, 6), bool), (1, 1, 1))
+    assert out.shape == ph.shape and not out.any()
