@@ -144,6 +144,57 @@ def _phantom_input(spec):
     return np.asfortranarray(d2f), ph["mask"].astype(np.uint8, order="F")
 
 
+def fd_leg(shape=(256, 256, 176), reps=20):
+    """The finite-difference entry points at the C3 size on device arrays
+    (both precisions), against their algorithmic bytes: gradients read 1
+    and write 3 arrays, the adjoints read 3 and write 1, lapw reads 1 and
+    writes 1."""
+    import ctypes as C
+
+    import torch
+    from qsm_amd import _lib
+
+    L = _lib.lib()
+    N = int(np.prod(shape))
+    sz, h = _lib.sz3(shape), _lib.h3((1.0, 1.0, 1.0))
+    st = torch.cuda.Stream()
+    out = {"shape": list(shape)}
+    for dt, name in ((torch.float32, "f32"), (torch.float64, "f64")):
+        s = 4 if dt == torch.float32 else 8
+        code = _lib.F32 if dt == torch.float32 else _lib.F64
+        with torch.cuda.stream(st):
+            a = [torch.randn(N, dtype=dt, device="cuda") for _ in range(4)]
+        st.synchronize()
+        sp = C.c_void_p(st.cuda_stream)
+        ops = {
+            "gradfp": (lambda: L.qsm_gradfp_dev(code, a[1].data_ptr(), a[2].data_ptr(), a[3].data_ptr(),
+                                                a[0].data_ptr(), sz, h, sp), 4),
+            "gradfp_adj": (lambda: L.qsm_gradfp_adj_dev(code, a[0].data_ptr(), a[1].data_ptr(),
+                                                        a[2].data_ptr(), a[3].data_ptr(), sz, h, sp), 4),
+            "gradf": (lambda: L.qsm_gradf_dev(code, a[1].data_ptr(), a[2].data_ptr(), a[3].data_ptr(),
+                                              a[0].data_ptr(), sz, h, sp), 4),
+            "gradf_adj": (lambda: L.qsm_gradf_adj_dev(code, a[0].data_ptr(), a[1].data_ptr(),
+                                                      a[2].data_ptr(), a[3].data_ptr(), sz, h, sp), 4),
+            "lapw": (lambda: L.qsm_lapw_dev(code, a[1].data_ptr(), a[0].data_ptr(), sz, h, sp), 2),
+        }
+        res = {}
+        for op, (fn, words) in ops.items():
+            _lib.check(fn())
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            for _ in range(reps):
+                _lib.check(fn())
+            e1.record(st)
+            e1.synchronize()
+            t = e0.elapsed_time(e1) / 1e3 / reps
+            res[op] = {"ms": t * 1e3, "gbs": words * N * s / t / 1e9}
+        out[name] = res
+        del a
+    torch.cuda.empty_cache()
+    return out
+
+
 def batch_leg(iters: int, warm: int = 10):
     """C4: 48 uneven TV/TGV volumes advanced together (FP32, captured
     chunks of 10 iterations); compared with the same volumes solved one
@@ -292,7 +343,7 @@ def main(argv=None) -> int:
                     help="requests of the C5 serving leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--no-dipole", dest="dipole", action="store_false",
-                    help="skip the FFT dipole convolution leg")
+                    help="skip the finite-difference and FFT dipole convolution legs")
     a = ap.parse_args(argv)
 
     if a.gpus != 1:
@@ -383,6 +434,8 @@ def main(argv=None) -> int:
     except Exception as e:  # measurement aid only
         log(f"[bench] copy peak failed: {e}")
     if a.dipole:
+        out["fd"] = fd_leg()
+        log("[bench] fd f32: " + ", ".join(f"{k} {v['gbs']:.0f} GB/s" for k, v in out["fd"]["f32"].items()))
         out["dipole_conv"] = dipole_leg()
         log(f"[bench] dipole conv f32 {out['dipole_conv']['f32']['ms']:.3f} ms, "
             f"f64 {out['dipole_conv']['f64']['ms']:.3f} ms")

@@ -146,6 +146,22 @@ __global__ __launch_bounds__(FX *FY) void gradfm_adj_kernel(T *du, const T *x,
 __device__ __forceinline__ double fmod_t(double a, double b) { return fmod(a, b); }
 __device__ __forceinline__ float fmod_t(float a, float b) { return fmodf(a, b); }
 
+// fmod(a, tau), exact like fmod itself: for |a| < 2 tau the remainder is
+// a, a - tau or a + tau, and that subtraction is exact (Sterbenz: the
+// operands are within a factor of two); only larger |a| (and NaN/Inf) go
+// to the library fmod.  A zero result may differ from fmod's in sign only,
+// which the caller's +/- pi removes.
+template <typename T>
+__device__ __forceinline__ T fmod_tau(T a, T tau)
+{
+    const T m = a < T(0) ? -a : a;
+    if (m < tau)
+        return a;
+    if (m < T(2) * tau)
+        return a > T(0) ? a - tau : a + tau;
+    return fmod_t(a, tau);
+}
+
 // lapw_mex.c:170-187: wrap to [-pi, pi] by fmod (exact in both precisions).
 template <typename T>
 __device__ __forceinline__ T wraptopi(T x)
@@ -153,7 +169,7 @@ __device__ __forceinline__ T wraptopi(T x)
     const T pi = T(3.14159265358979323846);
     const T tau = T(2 * 3.14159265358979323846);
     x += pi;
-    return x < T(0) ? fmod_t(x, tau) + pi : fmod_t(x, tau) - pi;
+    return x < T(0) ? fmod_tau(x, tau) + pi : fmod_tau(x, tau) - pi;
 }
 
 // lapw_mex.c:133-164: interior voxels only.
