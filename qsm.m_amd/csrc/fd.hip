@@ -146,30 +146,34 @@ __global__ __launch_bounds__(FX *FY) void gradfm_adj_kernel(T *du, const T *x,
 __device__ __forceinline__ double fmod_t(double a, double b) { return fmod(a, b); }
 __device__ __forceinline__ float fmod_t(float a, float b) { return fmodf(a, b); }
 
-// fmod(a, tau), exact like fmod itself: for |a| < 2 tau the remainder is
-// a, a - tau or a + tau, and that subtraction is exact (Sterbenz: the
-// operands are within a factor of two); only larger |a| (and NaN/Inf) go
-// to the library fmod.  A zero result may differ from fmod's in sign only,
-// which the caller's +/- pi removes.
-template <typename T>
-__device__ __forceinline__ T fmod_tau(T a, T tau)
-{
-    const T m = a < T(0) ? -a : a;
-    if (m < tau)
-        return a;
-    if (m < T(2) * tau)
-        return a > T(0) ? a - tau : a + tau;
-    return fmod_t(a, tau);
-}
-
 // lapw_mex.c:170-187: wrap to [-pi, pi] by fmod (exact in both precisions).
 template <typename T>
-__device__ __forceinline__ T wraptopi(T x)
+__device__ __noinline__ T wraptopi_fmod(T x)
 {
     const T pi = T(3.14159265358979323846);
     const T tau = T(2 * 3.14159265358979323846);
     x += pi;
-    return x < T(0) ? fmod_tau(x, tau) + pi : fmod_tau(x, tau) - pi;
+    return x < T(0) ? fmod_t(x, tau) + pi : fmod_t(x, tau) - pi;
+}
+
+// The reference's "if (d < -pi || d > pi) d = wraptopi(d)", bitwise: with
+// x = d + pi and |x| < 2 tau, fmod(x, tau) is x, x - tau or x + tau, and
+// that subtraction is exact (Sterbenz: the operands are within a factor of
+// two; a zero may differ from fmod's in sign only, which the +/- pi below
+// removes), so the common case is a few selects; only |x| >= 2 tau (and
+// NaN/Inf) call the library fmod.
+template <typename T>
+__device__ __forceinline__ T wrapd(T d)
+{
+    const T pi = T(3.14159265358979323846);
+    const T tau = T(2 * 3.14159265358979323846);
+    const T x = d + pi;
+    const T m = x < T(0) ? -x : x;
+    const T r = m < tau ? x : (x > T(0) ? x - tau : x + tau);
+    T w = x < T(0) ? r + pi : r - pi;
+    if (!(m < T(2) * tau))
+        w = wraptopi_fmod(d);
+    return (d < -pi || d > pi) ? w : d;
 }
 
 // lapw_mex.c:133-164: interior voxels only.
@@ -181,22 +185,15 @@ __global__ __launch_bounds__(FX *FY) void lapw_kernel(T *du, const T *u, G3 g,
     if (!vox(g, i, j, k, l)) return;
     if (i < 1 || i + 1 >= g.nx || j < 1 || j + 1 >= g.ny || k < 1 || k + 1 >= g.nz)
         return;
-    const T pi = T(3.14159265358979323846);
     const int sy = g.nx, sk = g.nx * g.ny;
     const T c = u[l];
-    T ub = c - u[l - 1], uf = u[l + 1] - c;
-    ub = (ub < -pi || ub > pi) ? wraptopi(ub) : ub;
-    uf = (uf < -pi || uf > pi) ? wraptopi(uf) : uf;
+    T ub = wrapd(c - u[l - 1]), uf = wrapd(u[l + 1] - c);
     T acc = hx * (uf - ub);
-    ub = c - u[l - sy];
-    uf = u[l + sy] - c;
-    ub = (ub < -pi || ub > pi) ? wraptopi(ub) : ub;
-    uf = (uf < -pi || uf > pi) ? wraptopi(uf) : uf;
+    ub = wrapd(c - u[l - sy]);
+    uf = wrapd(u[l + sy] - c);
     acc = acc + hy * (uf - ub);
-    ub = c - u[l - sk];
-    uf = u[l + sk] - c;
-    ub = (ub < -pi || ub > pi) ? wraptopi(ub) : ub;
-    uf = (uf < -pi || uf > pi) ? wraptopi(uf) : uf;
+    ub = wrapd(c - u[l - sk]);
+    uf = wrapd(u[l + sk] - c);
     du[l] = acc + hz * (uf - ub);
 }
 
