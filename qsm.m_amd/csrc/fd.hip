@@ -146,37 +146,51 @@ __global__ __launch_bounds__(FX *FY) void gradfm_adj_kernel(T *du, const T *x,
 __device__ __forceinline__ double fmod_t(double a, double b) { return fmod(a, b); }
 __device__ __forceinline__ float fmod_t(float a, float b) { return fmodf(a, b); }
 
-// lapw_mex.c:170-187: wrap to [-pi, pi] by fmod (exact in both precisions).
+// lapw_mex.c:170-187: wrap to [-pi, pi] by fmod, applied when
+// d < -pi || d > pi (lapw_mex.c:10-11).
 template <typename T>
-__device__ __noinline__ T wraptopi_fmod(T x)
+__device__ __forceinline__ T wrap_ref(T d)
 {
     const T pi = T(3.14159265358979323846);
     const T tau = T(2 * 3.14159265358979323846);
-    x += pi;
+    if (!(d < -pi || d > pi)) return d;
+    const T x = d + pi;
     return x < T(0) ? fmod_t(x, tau) + pi : fmod_t(x, tau) - pi;
 }
 
-// The reference's "if (d < -pi || d > pi) d = wraptopi(d)", bitwise: with
-// x = d + pi and |x| < 2 tau, fmod(x, tau) is x, x - tau or x + tau, and
-// that subtraction is exact (Sterbenz: the operands are within a factor of
-// two; a zero may differ from fmod's in sign only, which the +/- pi below
-// removes), so the common case is a few selects; only |x| >= 2 tau (and
-// NaN/Inf) call the library fmod.
+// The same wrap, bitwise, for |d + pi| < 2 tau.  On a wrapped lane
+// x = d + pi is nonzero, and fmod(x, tau) is x for |x| < tau, else
+// x - copysign(tau, x), a subtraction that is exact (Sterbenz: tau <= |x| <
+// 2 tau); the zero fmod returns at |x| = tau differs from this one in sign
+// only, which the following -/+ pi removes.  `slow` is raised where that
+// range does not hold (|x| >= 2 tau, Inf, NaN).
 template <typename T>
-__device__ __forceinline__ T wrapd(T d)
+__device__ __forceinline__ T wrap_fast(T d, bool &slow)
 {
     const T pi = T(3.14159265358979323846);
     const T tau = T(2 * 3.14159265358979323846);
     const T x = d + pi;
-    const T m = x < T(0) ? -x : x;
-    const T r = m < tau ? x : (x > T(0) ? x - tau : x + tau);
-    T w = x < T(0) ? r + pi : r - pi;
-    if (!(m < T(2) * tau))
-        w = wraptopi_fmod(d);
-    return (d < -pi || d > pi) ? w : d;
+    const T ax = fabs(x);
+    slow |= !(ax < T(2) * tau);
+    const T r = ax < tau ? x : x - copysign(tau, x);
+    const T w = r - copysign(pi, x);
+    return fabs(d) > pi ? w : d;
 }
 
-// lapw_mex.c:133-164: interior voxels only.
+// One voxel from its seven samples, with the reference's term order
+// (lapw_mex.c:139-162).
+template <typename T>
+__device__ __noinline__ T lapw_voxel_ref(T c, T xm, T xp, T ym, T yp, T zm, T zp,
+                                         T hx, T hy, T hz)
+{
+    T acc = hx * (wrap_ref(xp - c) - wrap_ref(c - xm));
+    acc = acc + hy * (wrap_ref(yp - c) - wrap_ref(c - ym));
+    return acc + hz * (wrap_ref(zp - c) - wrap_ref(c - zm));
+}
+
+// lapw_mex.c:133-164: interior voxels only.  Lanes whose differences leave
+// the fast wrap's range (phase jumps beyond 3 pi, Inf, NaN) redo the voxel
+// with the library fmod.
 template <typename T>
 __global__ __launch_bounds__(FX *FY) void lapw_kernel(T *du, const T *u, G3 g,
                                                       T hx, T hy, T hz)
@@ -186,15 +200,15 @@ __global__ __launch_bounds__(FX *FY) void lapw_kernel(T *du, const T *u, G3 g,
     if (i < 1 || i + 1 >= g.nx || j < 1 || j + 1 >= g.ny || k < 1 || k + 1 >= g.nz)
         return;
     const int sy = g.nx, sk = g.nx * g.ny;
-    const T c = u[l];
-    T ub = wrapd(c - u[l - 1]), uf = wrapd(u[l + 1] - c);
-    T acc = hx * (uf - ub);
-    ub = wrapd(c - u[l - sy]);
-    uf = wrapd(u[l + sy] - c);
-    acc = acc + hy * (uf - ub);
-    ub = wrapd(c - u[l - sk]);
-    uf = wrapd(u[l + sk] - c);
-    du[l] = acc + hz * (uf - ub);
+    const T c = u[l], xm = u[l - 1], xp = u[l + 1];
+    const T ym = u[l - sy], yp = u[l + sy], zm = u[l - sk], zp = u[l + sk];
+    bool slow = false;
+    T acc = hx * (wrap_fast(xp - c, slow) - wrap_fast(c - xm, slow));
+    acc = acc + hy * (wrap_fast(yp - c, slow) - wrap_fast(c - ym, slow));
+    acc = acc + hz * (wrap_fast(zp - c, slow) - wrap_fast(c - zm, slow));
+    if (slow)
+        acc = lapw_voxel_ref(c, xm, xp, ym, yp, zm, zp, hx, hy, hz);
+    du[l] = acc;
 }
 
 dim3 fd_grid(G3 g) { return dim3((g.nx + FX - 1) / FX, (g.ny + FY - 1) / FY, g.nz); }

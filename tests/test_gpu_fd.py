@@ -100,6 +100,33 @@ def test_lapw_wrapped_phase(cuda, dt):
     _same(fd.lapw(u, H), orc.lapw(u, H))
 
 
+@pytest.mark.parametrize("dt", DTS)
+def test_lapw_special_values(cuda, dt):
+    """Samples at and next to 0, +/-pi, +/-2pi, +/-3pi (where the wrap's
+    branches and its 2-tau range limit sit), large values, +/-Inf and NaN,
+    mixed with ordinary ones in the same wavefronts: bitwise where the
+    oracle is a number, NaN where it is NaN."""
+    rng = np.random.default_rng(11)
+    T = np.dtype(dt).type
+    base = [T(0), -T(0), T(1e3), T(-1e30), T(np.inf), T(-np.inf), T(np.nan)]
+    for m in (1, 2, 3, 4):
+        for s in (1, -1):
+            v = T(s * m * np.pi)
+            base += [v, np.nextafter(v, T(np.inf)), np.nextafter(v, T(-np.inf))]
+    shape = (70, 9, 6)
+    u = rng.standard_normal(shape).astype(dt)
+    pick = rng.random(shape) < 0.5
+    u[pick] = rng.choice(np.array(base, dtype=dt), size=int(pick.sum()))
+    u = np.asfortranarray(u)
+    with np.errstate(invalid="ignore"):
+        want = orc.lapw(u, H)
+    got = fd.lapw(u, H)
+    nan = np.isnan(want)
+    assert nan.any() and (~nan).any()
+    np.testing.assert_array_equal(np.isnan(got), nan)
+    _same(np.where(nan, 0, got).astype(dt), np.where(nan, 0, want).astype(dt))
+
+
 def _dev_call(name, dt, outs_dev, ins_dev, shape, G_dev=None, stream=None):
     L = _lib.lib()
     args = [_lib.dtype_code(dt), *(t.data_ptr() for t in outs_dev), *(t.data_ptr() for t in ins_dev)]
