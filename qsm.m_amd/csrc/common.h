@@ -57,4 +57,16 @@ struct DevBuf {
 __device__ __forceinline__ double rsqrt_cr(double x) { return sqrt(x); }
 __device__ __forceinline__ float rsqrt_cr(float x) { return sqrtf(x); }
 
+// XCD-aware block order: the hardware places block b of a 1-D grid on XCD
+// b % 8 (each XCD with its own L2).  Returns a logical block index such that
+// each XCD runs one contiguous range of logical blocks, so blocks that share
+// cache lines (neighbouring tiles, the two halves of a line) meet in one L2.
+__device__ __forceinline__ int xcd_block()
+{
+    const int nblk = int(gridDim.x), b = int(blockIdx.x);
+    const int per = nblk / 8, rem = nblk % 8;
+    const int xcd = b % 8, idx = b / 8;
+    return (xcd < rem) ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+}
+
 }  // namespace qsm

@@ -10,12 +10,14 @@
 // positions for the y/z axes), runs a mixed-radix Stockham transform in LDS
 // (radices 8, 7, 5, 4, 3, 2; twiddles from a table computed in double from
 // exact rationals, cast for single precision), and stores.  The dipole
-// convolution uses five passes: x (zero padding fused into the load, only
-// rows that hold data), y (only planes that hold data, zero rows not read),
-// z forward * D(k) * z inverse in one pass (D generated in-kernel; only the
-// data planes read and stored), y inverse (only data rows stored), x
-// inverse (crop, real part and 1/M scaling fused into the store).  The
-// padding is never materialised.  The passes are HBM-bound.
+// convolution uses five passes over a half spectrum along x (the input is
+// real): x (two real rows per complex transform, zero padding fused into
+// the load, only rows that hold data, H = M0/2 + 1 outputs per row kept),
+// y (only planes that hold data, zero rows not read), z forward * D(k) * z
+// inverse in one pass (D generated in-kernel; only the data planes read and
+// stored), y inverse (only data rows stored), x inverse (two rows per
+// complex transform rebuilt from their half spectra; crop and real output
+// fused into the store).  The padding is never materialised.
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -26,10 +28,17 @@
 
 using namespace qsm;
 
+// No result here is bitwise-pinned (the FFT is checked against numpy to a
+// tolerance), so multiply-adds may fuse.
+#pragma clang fp contract(fast)
+
 namespace {
 
 constexpr int MAX_STAGES = 24;
-constexpr int FFT_THREADS = 256;
+#ifndef QSM_FFT_THREADS
+#define QSM_FFT_THREADS 512
+#endif
+constexpr int FFT_THREADS = QSM_FFT_THREADS;
 
 template <typename T>
 struct Cx {
@@ -211,9 +220,10 @@ __device__ __forceinline__ void stage(const Cx<T> *A, Cx<T> *B, Pen pn, int M, i
 
 // Stockham transform of the block's pencils held in LDS; returns the
 // buffer holding the result.  sgn = -1 forward, +1 inverse (twiddles
-// conjugated).
+// conjugated).  Inlined so that the compiler sees the buffers are LDS (ds_
+// instructions, not flat ones).
 template <typename T>
-__device__ Cx<T> *stockham(Cx<T> *A, Cx<T> *B, Pen pn, const Plan *__restrict__ pl,
+__device__ __forceinline__ Cx<T> *stockham(Cx<T> *A, Cx<T> *B, Pen pn, const Plan *__restrict__ pl,
                            const Cx<T> *__restrict__ tw, T sgn)
 {
     const int M = pl->M, nst = pl->nst;
@@ -243,6 +253,52 @@ struct Vol {
     int M0, M1, M2;
 };
 
+// Block copies between global memory and the LDS pencils, COPY_UNROLL
+// accesses in flight per thread: all loads of a group are issued before
+// the first LDS write (and all LDS reads before the first store), so a
+// thread waits for memory once per group, not once per element.
+constexpr int COPY_UNROLL = 8;
+
+// For e in [0, total): A[dst] = v where v = src(e, dst) (src sets dst, or
+// leaves it negative for nothing to write).
+template <typename T, typename Src>
+__device__ __forceinline__ void fill(Cx<T> *A, int total, Src src)
+{
+    for (int e0 = threadIdx.x; e0 < total; e0 += COPY_UNROLL * FFT_THREADS) {
+        Cx<T> r[COPY_UNROLL];
+        int d[COPY_UNROLL];
+#pragma unroll
+        for (int u = 0; u < COPY_UNROLL; ++u) {
+            const int e = e0 + u * FFT_THREADS;
+            d[u] = -1;
+            r[u] = Cx<T>{T(0), T(0)};
+            if (e < total) r[u] = src(e, d[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < COPY_UNROLL; ++u)
+            if (d[u] >= 0) A[d[u]] = r[u];
+    }
+}
+
+// For e in [0, total): put(e, R[at(e)]) (put stores or drops it).
+template <typename T, typename At, typename Put>
+__device__ __forceinline__ void drain(const Cx<T> *R, int total, At at, Put put)
+{
+    for (int e0 = threadIdx.x; e0 < total; e0 += COPY_UNROLL * FFT_THREADS) {
+        Cx<T> r[COPY_UNROLL];
+#pragma unroll
+        for (int u = 0; u < COPY_UNROLL; ++u) {
+            const int e = e0 + u * FFT_THREADS;
+            if (e < total) r[u] = R[at(e)];
+        }
+#pragma unroll
+        for (int u = 0; u < COPY_UNROLL; ++u) {
+            const int e = e0 + u * FFT_THREADS;
+            if (e < total) put(e, r[u]);
+        }
+    }
+}
+
 // ---- generic in-place pass (plain fft3 / ifft3) -------------------------
 template <typename T>
 __global__ __launch_bounds__(FFT_THREADS) void fft_pass_kernel(
@@ -266,41 +322,52 @@ __global__ __launch_bounds__(FFT_THREADS) void fft_pass_kernel(
         pstride = v.M0;
     } else {
         const int nxb = (v.M0 + P - 1) / P;
-        const int x0 = (blockIdx.x % nxb) * P;
-        const int other = blockIdx.x / nxb;
+        const int lb = xcd_block();  // neighbouring x groups share L2 lines
+        const int x0 = (lb % nxb) * P;
+        const int other = lb / nxb;
         np = min(P, v.M0 - x0);
         stride = axis == 1 ? size_t(v.M0) : size_t(v.M0) * v.M1;
         base = axis == 1 ? size_t(x0) + size_t(other) * v.M0 * v.M1 : size_t(x0) + size_t(other) * v.M0;
         pstride = 1;
     }
-    for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
+    const unsigned mM = pl->mM;
+    const int logP = pn.logP;
+    auto pi = [=](int e, int &p, int &i) {
+        if (axis == 0) { p = int(udiv(unsigned(e), mM)); i = e - p * M; }
+        else { i = e >> logP; p = e & (P - 1); }
+    };
+    fill(A, P * M, [&](int e, int &d) {
         int p, i;
-        if (axis == 0) { p = int(udiv(unsigned(e), pl->mM)); i = e - p * M; }
-        else { i = e >> pn.logP; p = e & (P - 1); }
-        A[i * PP + p] = p < np ? data[base + size_t(p) * pstride + size_t(i) * stride] : Cx<T>{T(0), T(0)};
-    }
+        pi(e, p, i);
+        d = i * PP + p;
+        return p < np ? data[base + size_t(p) * pstride + size_t(i) * stride] : Cx<T>{T(0), T(0)};
+    });
     __syncthreads();
     Cx<T> *R = stockham(A, B, pn, pl, tw, sgn);
-    for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
-        int p, i;
-        if (axis == 0) { p = int(udiv(unsigned(e), pl->mM)); i = e - p * M; }
-        else { i = e >> pn.logP; p = e & (P - 1); }
-        if (p < np) {
-            Cx<T> x = R[i * PP + p];
-            x.re *= scale;
-            x.im *= scale;
-            data[base + size_t(p) * pstride + size_t(i) * stride] = x;
-        }
-    }
+    drain(R, P * M, [&](int e) { int p, i; pi(e, p, i); return i * PP + p; },
+          [&](int e, Cx<T> x) {
+              int p, i;
+              pi(e, p, i);
+              if (p < np) {
+                  x.re *= scale;
+                  x.im *= scale;
+                  data[base + size_t(p) * pstride + size_t(i) * stride] = x;
+              }
+          });
 }
 
 // ---- dipole convolution passes -------------------------------------------
 
 // x pass, forward: reads the real input (nx, ny, nz) zero-padded by
 // (o0, o1, o2) voxels before, for rows (j, k) inside the data region only.
+// Rows 2p and 2p+1 of the block travel as the real and imaginary parts of
+// pencil p; with Z = FFT(a + i b) and W(k) = conj(Z(M - k)) the row spectra
+// are A = (Z + W)/2 and B = (Z - W)/(2i).  Stored: 2A and 2B for
+// k = 0..H-1 (the 1/2 is folded into the z pass's scaling) into the half
+// spectrum volume (H, M1, M2).
 template <typename T>
 __global__ __launch_bounds__(FFT_THREADS) void dip_x_fwd(
-    Cx<T> *data, const T *chi, Vol v, int nx, int ny, int nz, int o0, int o1, int o2,
+    Cx<T> *data, const T *chi, int H, int M1, int nx, int ny, int nz, int o0, int o1, int o2,
     Pen pn, const Plan *pl, const Cx<T> *tw)
 {
     extern __shared__ unsigned char smem[];
@@ -309,25 +376,36 @@ __global__ __launch_bounds__(FFT_THREADS) void dip_x_fwd(
     Cx<T> *B = A + size_t(PP) * pl->M;
     const int M = pl->M;
     const int rows = ny * nz;
-    const int r0 = blockIdx.x * P;
-    const int np = min(P, rows - r0);
-    for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
-        const int p = int(udiv(unsigned(e), pl->mM)), i = e - p * M;
-        T x = T(0);
-        if (p < np && i >= o0 && i < o0 + nx) {
-            const int r = r0 + p;
-            x = chi[size_t(i - o0) + size_t(nx) * r];
+    const int r0 = blockIdx.x * 2 * P;
+    const unsigned mM = pl->mM;
+    fill(A, P * M, [&](int e, int &d) {
+        const int p = int(udiv(unsigned(e), mM)), i = e - p * M;
+        const int ra = r0 + 2 * p;
+        T a = T(0), b = T(0);
+        if (i >= o0 && i < o0 + nx) {
+            if (ra < rows) a = chi[size_t(i - o0) + size_t(nx) * ra];
+            if (ra + 1 < rows) b = chi[size_t(i - o0) + size_t(nx) * (ra + 1)];
         }
-        A[i * PP + p] = Cx<T>{x, T(0)};
-    }
+        d = i * PP + p;
+        return Cx<T>{a, b};
+    });
     __syncthreads();
     Cx<T> *R = stockham(A, B, pn, pl, tw, T(-1));
-    for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
-        const int p = int(udiv(unsigned(e), pl->mM)), i = e - p * M;
-        if (p < np) {
-            const int r = r0 + p;
-            const int j = r % ny + o1, k = r / ny + o2;
-            data[size_t(i) + size_t(v.M0) * (size_t(j) + size_t(v.M1) * k)] = R[i * PP + p];
+    const unsigned mH = recip(unsigned(H));
+    for (int e = threadIdx.x; e < P * H; e += FFT_THREADS) {
+        const int p = int(udiv(unsigned(e), mH)), k = e - p * H;
+        const int ra = r0 + 2 * p;
+        if (ra >= rows) continue;
+        const Cx<T> z = R[k * PP + p];
+        const Cx<T> w = R[(k == 0 ? 0 : M - k) * PP + p];  // conj taken below
+        // 2A = z + conj(w), 2B = -i (z - conj(w))
+        const Cx<T> sa{z.re + w.re, z.im - w.im};
+        const Cx<T> sb{z.im + w.im, w.re - z.re};
+        const int ja = ra % ny + o1, ka = ra / ny + o2;
+        data[size_t(k) + size_t(H) * (size_t(ja) + size_t(M1) * ka)] = sa;
+        if (ra + 1 < rows) {
+            const int jb = (ra + 1) % ny + o1, kb = (ra + 1) / ny + o2;
+            data[size_t(k) + size_t(H) * (size_t(jb) + size_t(M1) * kb)] = sb;
         }
     }
 }
@@ -346,40 +424,58 @@ __global__ __launch_bounds__(FFT_THREADS) void dip_y(
     Cx<T> *B = A + size_t(PP) * pl->M;
     const int M = pl->M;
     const int nxb = (v.M0 + P - 1) / P;
-    const int x0 = (blockIdx.x % nxb) * P;
-    const int k = k0 + blockIdx.x / nxb;
+    const int lb = xcd_block();  // neighbouring x groups share L2 lines
+    const int x0 = (lb % nxb) * P;
+    const int k = k0 + lb / nxb;
     const int np = min(P, v.M0 - x0);
     const size_t base = size_t(x0) + size_t(k) * v.M0 * v.M1;
-    for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
-        const int i = e >> pn.logP, p = e & (P - 1);
-        A[i * PP + p] = (p < np && i >= ylo && i < yhi) ? data[base + p + size_t(i) * v.M0]
-                                                       : Cx<T>{T(0), T(0)};
-    }
+    const int logP = pn.logP;
+    fill(A, P * M, [&](int e, int &d) {
+        const int i = e >> logP, p = e & (P - 1);
+        d = i * PP + p;
+        return (p < np && i >= ylo && i < yhi) ? data[base + p + size_t(i) * v.M0]
+                                               : Cx<T>{T(0), T(0)};
+    });
     __syncthreads();
     Cx<T> *R = stockham(A, B, pn, pl, tw, sgn);
-    for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
-        const int i = e >> pn.logP, p = e & (P - 1);
-        if (p < np && i >= slo && i < shi)
-            data[base + p + size_t(i) * v.M0] = R[i * PP + p];
-    }
+    drain(R, P * M, [&](int e) { return (e >> logP) * PP + (e & (P - 1)); },
+          [&](int e, Cx<T> x) {
+              const int i = e >> logP, p = e & (P - 1);
+              if (p < np && i >= slo && i < shi)
+                  data[base + p + size_t(i) * v.M0] = x;
+          });
 }
 
 // k-space coordinate of index m on an axis of length n, voxel size h
 // (dipoleKernel.m:74-87 followed by ifftshift, :64).
-__device__ __forceinline__ double kcoord(int m, int n, double h)
+// dk = 1 / (n h), formed on the host.
+__device__ __forceinline__ double kcoord(int m, int n, double dk)
 {
-    const double dk = 1.0 / (double(n) * h);
     const int low = n / 2;
     return double(m < n - low ? m : m - n) * dk;
 }
 
-// z pass: forward transform, multiply by D(k) / (M0*M1*M2), inverse.
-// Only planes [zlo, zhi) hold data on input and only they are needed on
-// output: the others are neither read (zero) nor stored.
+// D(k) = 1/3 - (k.b)^2 / |k|^2, D(0) = 0 (dipoleKernel.m:59-66), in double.
+__device__ __forceinline__ double dip(double X, double Y, double Z, double b0, double b1,
+                                      double b2)
+{
+    const double kz = (b0 * X + b1 * Y) + b2 * Z;
+    const double k2 = (X * X + Y * Y) + Z * Z;
+    return (k2 == 0.0) ? 0.0 : (1.0 / 3.0 - (kz * kz) / k2);
+}
+
+// z pass over the half spectrum (x extent v.M0 = H of a true length mx):
+// forward transform, multiply by Ds(k) * scale, inverse.  The reference keeps
+// real(ifftn(D .* fftn(x))); for a real x that is the inverse transform of
+// fftn(x) .* Ds with Ds(k) = (D(k) + D(-k))/2, the form a half spectrum can
+// carry.  Ds = D except where an even axis is at its Nyquist index (there
+// -k has the same index, with the other components negated).  Only planes
+// [zlo, zhi) hold data on input and only they are needed on output: the
+// others are neither read (zero) nor stored.
 template <typename T>
 __global__ __launch_bounds__(FFT_THREADS) void dip_z(
-    Cx<T> *data, Vol v, Pen pn, const Plan *pl, const Cx<T> *tw,
-    double h0, double h1, double h2, double b0, double b1, double b2, double inv_n,
+    Cx<T> *data, Vol v, int mx, Pen pn, const Plan *pl, const Cx<T> *tw,
+    double dk0, double dk1, double dk2, double b0, double b1, double b2, double scale,
     int zlo, int zhi)
 {
     extern __shared__ unsigned char smem[];
@@ -388,29 +484,33 @@ __global__ __launch_bounds__(FFT_THREADS) void dip_z(
     Cx<T> *B = A + size_t(PP) * pl->M;
     const int M = pl->M;
     const int nxb = (v.M0 + P - 1) / P;
-    const int x0 = (blockIdx.x % nxb) * P;
-    const int j = blockIdx.x / nxb;
+    const int lb = xcd_block();  // neighbouring x groups share L2 lines
+    const int x0 = (lb % nxb) * P;
+    const int j = lb / nxb;
     const int np = min(P, v.M0 - x0);
     const size_t plane = size_t(v.M0) * v.M1;
     const size_t base = size_t(x0) + size_t(j) * v.M0;
-    for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
-        const int i = e >> pn.logP, p = e & (P - 1);
-        A[i * PP + p] = (p < np && i >= zlo && i < zhi) ? data[base + p + size_t(i) * plane]
-                                                       : Cx<T>{T(0), T(0)};
-    }
+    const int logP = pn.logP;
+    fill(A, P * M, [&](int e, int &d) {
+        const int i = e >> logP, p = e & (P - 1);
+        d = i * PP + p;
+        return (p < np && i >= zlo && i < zhi) ? data[base + p + size_t(i) * plane]
+                                               : Cx<T>{T(0), T(0)};
+    });
     __syncthreads();
     Cx<T> *R = stockham(A, B, pn, pl, tw, T(-1));
     Cx<T> *S = (R == A) ? B : A;
-    // D(k) = 1/3 - (k.b)^2 / |k|^2, D(0) = 0 (dipoleKernel.m:59-66), in double
-    const double Y = kcoord(j, v.M1, h1);
+    const double Y = kcoord(j, v.M1, dk1);
+    const bool ny_y = 2 * j == v.M1;
     for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
-        const int i = e >> pn.logP, p = e & (P - 1);
-        const double X = kcoord(x0 + p, v.M0, h0);
-        const double Z = kcoord(i, M, h2);
-        const double kz = (b0 * X + b1 * Y) + b2 * Z;
-        const double k2 = (X * X + Y * Y) + Z * Z;
-        const double D = (k2 == 0.0) ? 0.0 : (1.0 / 3.0 - (kz * kz) / k2);
-        const T d = T(D * inv_n);
+        const int i = e >> logP, p = e & (P - 1);
+        const double X = kcoord(x0 + p, mx, dk0);
+        const double Z = kcoord(i, M, dk2);
+        double D = dip(X, Y, Z, b0, b1, b2);
+        const bool ny_x = 2 * (x0 + p) == mx, ny_z = 2 * i == M;
+        if (ny_x || ny_y || ny_z)
+            D = 0.5 * (D + dip(ny_x ? X : -X, ny_y ? Y : -Y, ny_z ? Z : -Z, b0, b1, b2));
+        const T d = T(D * scale);
         Cx<T> x = R[i * PP + p];
         x.re *= d;
         x.im *= d;
@@ -418,18 +518,22 @@ __global__ __launch_bounds__(FFT_THREADS) void dip_z(
     }
     __syncthreads();
     Cx<T> *Q = stockham(R, S, pn, pl, tw, T(1));
-    for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
-        const int i = e >> pn.logP, p = e & (P - 1);
-        if (p < np && i >= zlo && i < zhi)
-            data[base + p + size_t(i) * plane] = Q[i * PP + p];
-    }
+    drain(Q, P * M, [&](int e) { return (e >> logP) * PP + (e & (P - 1)); },
+          [&](int e, Cx<T> x) {
+              const int i = e >> logP, p = e & (P - 1);
+              if (p < np && i >= zlo && i < zhi)
+                  data[base + p + size_t(i) * plane] = x;
+          });
 }
 
-// x pass, inverse: rows inside the output region; stores the real part,
-// cropped, into out (nx, ny, nz).
+// x pass, inverse: rows inside the output region, two per complex
+// transform.  From the half spectra A, B (k = 0..H-1) of rows 2p and 2p+1,
+// pencil p is Z(k) = A(k) + i B(k), with A(k) = conj(A(M - k)) (likewise B)
+// above H; the inverse transform of Z is a + i b.  Stores the cropped rows
+// into out (nx, ny, nz).
 template <typename T>
 __global__ __launch_bounds__(FFT_THREADS) void dip_x_inv(
-    const Cx<T> *data, T *out, Vol v, int nx, int ny, int nz, int o0, int o1, int o2,
+    const Cx<T> *data, T *out, int H, int M1, int nx, int ny, int nz, int o0, int o1, int o2,
     Pen pn, const Plan *pl, const Cx<T> *tw)
 {
     extern __shared__ unsigned char smem[];
@@ -438,27 +542,38 @@ __global__ __launch_bounds__(FFT_THREADS) void dip_x_inv(
     Cx<T> *B = A + size_t(PP) * pl->M;
     const int M = pl->M;
     const int rows = ny * nz;
-    const int r0 = blockIdx.x * P;
-    const int np = min(P, rows - r0);
-    for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
-        const int p = int(udiv(unsigned(e), pl->mM)), i = e - p * M;
-        Cx<T> x{T(0), T(0)};
-        if (p < np) {
-            const int r = r0 + p;
-            const int j = r % ny + o1, k = r / ny + o2;
-            x = data[size_t(i) + size_t(v.M0) * (size_t(j) + size_t(v.M1) * k)];
+    const int r0 = blockIdx.x * 2 * P;
+    const unsigned mM = pl->mM;
+    fill(A, P * M, [&](int e, int &d) {
+        const int p = int(udiv(unsigned(e), mM)), i = e - p * M;
+        const int ra = r0 + 2 * p;
+        const bool lo = i < H;
+        const int k = lo ? i : M - i;
+        Cx<T> a{T(0), T(0)}, b{T(0), T(0)};
+        if (ra < rows) {
+            const int j = ra % ny + o1, kk = ra / ny + o2;
+            a = data[size_t(k) + size_t(H) * (size_t(j) + size_t(M1) * kk)];
         }
-        A[i * PP + p] = x;
-    }
+        if (ra + 1 < rows) {
+            const int j = (ra + 1) % ny + o1, kk = (ra + 1) / ny + o2;
+            b = data[size_t(k) + size_t(H) * (size_t(j) + size_t(M1) * kk)];
+        }
+        d = i * PP + p;
+        // lo: a + i b; above H: conj(a) + i conj(b)
+        return lo ? Cx<T>{a.re - b.im, a.im + b.re} : Cx<T>{a.re + b.im, b.re - a.im};
+    });
     __syncthreads();
     Cx<T> *R = stockham(A, B, pn, pl, tw, T(1));
-    for (int e = threadIdx.x; e < P * M; e += FFT_THREADS) {
-        const int p = int(udiv(unsigned(e), pl->mM)), i = e - p * M;
-        if (p < np && i >= o0 && i < o0 + nx) {
-            const int r = r0 + p;
-            out[size_t(i - o0) + size_t(nx) * r] = R[i * PP + p].re;
-        }
-    }
+    drain(R, P * M,
+          [&](int e) { const int p = int(udiv(unsigned(e), mM)); return (e - p * M) * PP + p; },
+          [&](int e, Cx<T> x) {
+              const int p = int(udiv(unsigned(e), mM)), i = e - p * M;
+              const int ra = r0 + 2 * p;
+              if (i >= o0 && i < o0 + nx) {
+                  if (ra < rows) out[size_t(i - o0) + size_t(nx) * ra] = x.re;
+                  if (ra + 1 < rows) out[size_t(i - o0) + size_t(nx) * (ra + 1)] = x.im;
+              }
+          });
 }
 
 // ---- host-side planning ----------------------------------------------------
@@ -663,11 +778,13 @@ int dipole_conv_dev_t(const void *chi, const size_t sz[3], const double h[3],
     for (int a = 0; a < 3; ++a)
         if ((rc = make_axis<T>(ap[a], M[a], "padded length"))) return rc;
     if ((rc = allow_lds<T>())) return rc;
-    const Vol v{int(M[0]), int(M[1]), int(M[2])};
+    const int M0 = int(M[0]), M1 = int(M[1]), M2 = int(M[2]);
+    const int H = M0 / 2 + 1;        // half spectrum along x
+    const Vol v{H, M1, M2};          // the volume the y and z passes see
     const int nx = int(sz[0]), ny = int(sz[1]), nz = int(sz[2]);
     const int o0 = int(off[0]), o1 = int(off[1]), o2 = int(off[2]);
     Cx<T> *d = static_cast<Cx<T> *>(work);
-    const size_t total = size_t(v.M0) * v.M1 * v.M2;
+    const double total = double(M0) * M1 * M2;
     // The padding is never stored: the x pass writes the rows that hold
     // data, and the y and z passes treat the other rows and planes as zero
     // on load.  The work space therefore needs no clearing.
@@ -678,45 +795,49 @@ int dipole_conv_dev_t(const void *chi, const size_t sz[3], const double h[3],
     if (bn != 1.0)
         for (double &x : b) x /= bn;
     const size_t cs = sizeof(Cx<T>);
-    {   // x forward on data rows
+    {   // x forward on data rows, two per transform
         const Plan &pl = ap[0].pl;
         const Pen pn = pencils(pl.M, cs);
-        const int P = 1 << pn.logP;
-        dip_x_fwd<T><<<(ny * nz + P - 1) / P, FFT_THREADS, lds_bytes(pn, pl.M, cs), st>>>(
-            d, static_cast<const T *>(chi), v, nx, ny, nz, o0, o1, o2, pn, ap[0].dpl, tw_of<T>(ap[0]));
+        const int P2 = 2 << pn.logP;
+        dip_x_fwd<T><<<(ny * nz + P2 - 1) / P2, FFT_THREADS, lds_bytes(pn, pl.M, cs), st>>>(
+            d, static_cast<const T *>(chi), H, M1, nx, ny, nz, o0, o1, o2, pn, ap[0].dpl,
+            tw_of<T>(ap[0]));
         QSM_HIP(hipGetLastError());
     }
     {   // y forward on data planes
         const Plan &pl = ap[1].pl;
         const Pen pn = pencils(pl.M, cs);
         const int P = 1 << pn.logP;
-        dip_y<T><<<((v.M0 + P - 1) / P) * nz, FFT_THREADS, lds_bytes(pn, pl.M, cs), st>>>(
-            d, v, o2, pn, ap[1].dpl, tw_of<T>(ap[1]), T(-1), o1, o1 + ny, 0, v.M1);
+        dip_y<T><<<((H + P - 1) / P) * nz, FFT_THREADS, lds_bytes(pn, pl.M, cs), st>>>(
+            d, v, o2, pn, ap[1].dpl, tw_of<T>(ap[1]), T(-1), o1, o1 + ny, 0, M1);
         QSM_HIP(hipGetLastError());
     }
-    {   // z forward * D * z inverse, scaled by 1/(M0 M1 M2)
+    {   // z forward * Ds * z inverse, scaled by 1/(2 M0 M1 M2) (the 2 undoes
+        // the x pass's doubling)
         const Plan &pl = ap[2].pl;
         const Pen pn = pencils(pl.M, cs);
         const int P = 1 << pn.logP;
-        dip_z<T><<<((v.M0 + P - 1) / P) * v.M1, FFT_THREADS, lds_bytes(pn, pl.M, cs), st>>>(
-            d, v, pn, ap[2].dpl, tw_of<T>(ap[2]), h[0], h[1], h[2],
-            b[0], b[1], b[2], 1.0 / double(total), o2, o2 + nz);
+        dip_z<T><<<((H + P - 1) / P) * M1, FFT_THREADS, lds_bytes(pn, pl.M, cs), st>>>(
+            d, v, M0, pn, ap[2].dpl, tw_of<T>(ap[2]), 1.0 / (double(M0) * h[0]),
+            1.0 / (double(M1) * h[1]), 1.0 / (double(M2) * h[2]),
+            b[0], b[1], b[2], 0.5 / total, o2, o2 + nz);
         QSM_HIP(hipGetLastError());
     }
     {   // y inverse on output planes
         const Plan &pl = ap[1].pl;
         const Pen pn = pencils(pl.M, cs);
         const int P = 1 << pn.logP;
-        dip_y<T><<<((v.M0 + P - 1) / P) * nz, FFT_THREADS, lds_bytes(pn, pl.M, cs), st>>>(
-            d, v, o2, pn, ap[1].dpl, tw_of<T>(ap[1]), T(1), 0, v.M1, o1, o1 + ny);
+        dip_y<T><<<((H + P - 1) / P) * nz, FFT_THREADS, lds_bytes(pn, pl.M, cs), st>>>(
+            d, v, o2, pn, ap[1].dpl, tw_of<T>(ap[1]), T(1), 0, M1, o1, o1 + ny);
         QSM_HIP(hipGetLastError());
     }
-    {   // x inverse on output rows, crop + real part
+    {   // x inverse on output rows, two per transform, cropped
         const Plan &pl = ap[0].pl;
         const Pen pn = pencils(pl.M, cs);
-        const int P = 1 << pn.logP;
-        dip_x_inv<T><<<(ny * nz + P - 1) / P, FFT_THREADS, lds_bytes(pn, pl.M, cs), st>>>(
-            d, static_cast<T *>(field), v, nx, ny, nz, o0, o1, o2, pn, ap[0].dpl, tw_of<T>(ap[0]));
+        const int P2 = 2 << pn.logP;
+        dip_x_inv<T><<<(ny * nz + P2 - 1) / P2, FFT_THREADS, lds_bytes(pn, pl.M, cs), st>>>(
+            d, static_cast<T *>(field), H, M1, nx, ny, nz, o0, o1, o2, pn, ap[0].dpl,
+            tw_of<T>(ap[0]));
         QSM_HIP(hipGetLastError());
     }
     return QSM_OK;
@@ -780,7 +901,7 @@ size_t qsm_dipole_work_bytes(int dtype, const size_t sz[3], const size_t pad[3])
 {
     size_t M[3], off[3];
     pad_plan(sz, pad, M, off);
-    return M[0] * M[1] * M[2] * 2 * dtype_size(dtype);
+    return (M[0] / 2 + 1) * M[1] * M[2] * 2 * dtype_size(dtype);  // half spectrum along x
 }
 
 int qsm_dipole_conv_dev(int dtype, const void *chi, const size_t sz[3],

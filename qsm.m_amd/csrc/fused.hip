@@ -36,13 +36,9 @@ __global__ __launch_bounds__(TX *TY) void fused_kernel(
     if (stop && *stop)
         return;
 
-    // XCD-aware order: hardware block b runs on XCD b % 8; give each XCD a
-    // contiguous run of logical tiles so ring neighbours share its L2.
-    const int nblk = ntx * nty * nchunk;
-    const int per = nblk / 8, rem = nblk % 8;
-    const int xcd = int(blockIdx.x) % 8, idx = int(blockIdx.x) / 8;
-    const int lb = (xcd < rem) ? xcd * (per + 1) + idx
-                               : rem * (per + 1) + (xcd - rem) * per + idx;
+    // each XCD takes a contiguous run of logical tiles, so ring neighbours
+    // share its L2 (the grid is ntx * nty * nchunk blocks)
+    const int lb = xcd_block();
     const int bx = lb % ntx;
     const int by = (lb / ntx) % nty;
     const int bz = lb / (ntx * nty);

@@ -36,6 +36,14 @@ def pad_size(shape, pad):
     return tuple(M), tuple(off)
 
 
+def work_bytes(dtype, shape, pad) -> int:
+    """Bytes of device work space dipole_conv_dev needs (dtype: numpy or
+    torch float32/float64): the half spectrum along x of the padded
+    volume."""
+    code = _lib.F32 if str(dtype).endswith("float32") else _lib.F64
+    return int(lib().qsm_dipole_work_bytes(code, sz3(shape), sz3(pad)))
+
+
 def dipole_conv(chi, vsz=(1.0, 1.0, 1.0), bdir=(0.0, 0.0, 1.0), pad=(0, 0, 0)):
     """field = real(ifft3(D .* fft3(padfastfft(chi, pad)))) cropped back."""
     chi = fortran(chi)
@@ -56,7 +64,7 @@ def dipole_conv_dev(chi, vsz, bdir, pad, out, work=None, stream=None):
         raise ValueError("chi must be a 3-D tensor (use a Fortran-order view)")
     shape = tuple(int(s) for s in chi.shape)
     code = _lib.F64 if chi.dtype == torch.float64 else _lib.F32
-    nb = int(lib().qsm_dipole_work_bytes(code, sz3(shape), sz3(pad)))
+    nb = work_bytes(chi.dtype, shape, pad)
     own = work is None
     if own:
         work = torch.empty(nb, dtype=torch.uint8, device=chi.device)

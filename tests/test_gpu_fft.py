@@ -46,6 +46,11 @@ def test_fft3_parseval_radix7(cuda):
     ((24, 22, 20), (1.0, 1.0, 1.0), (0, 0, 1), (12, 11, 10)),
     ((30, 17, 21), (0.9, 0.9, 1.2), (0, -0.258819, 0.965926), (15, 8, 10)),
     ((16, 16, 16), (1.0, 1.0, 1.0), (1, 1, 1), (0, 0, 0)),
+    # odd padded x length, odd row count (the last row of the x passes has
+    # no partner), Nyquist planes on y and z with a tilted B0
+    ((15, 9, 7), (1.0, 1.1, 0.9), (0.3, -0.2, 0.93), (0, 2, 3)),
+    # a single x column (half spectrum of length 1)
+    ((1, 12, 10), (1.0, 1.0, 1.0), (0.2, 0.1, 0.97), (0, 3, 2)),
 ])
 def test_dipole_conv_matches_fft_reference(cuda, dt, shape, vsz, bdir, pad):
     rng = np.random.default_rng(2)
