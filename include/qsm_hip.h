@@ -293,8 +293,8 @@ size_t qsm_nextfastfft(size_t n);
 
 /* In-place 3-D complex transform of an interleaved (re, im) array of
  * M[0] x M[1] x M[2] (x fastest): fft3.m:5 (inverse = 0) or ifft3.m:5
- * (inverse = 1, including 1/M per axis).  Lengths must be 2,3,5,7-smooth
- * and at most 2048 (double) / 4096 (single).  Asynchronous on the stream
+ * (inverse = 1, including 1/M per axis).  Lengths must have no prime
+ * factor above 13 and be at most 2048 (double) / 4096 (single).  Asynchronous on the stream
  * (plans are built once per device and length, then cached). */
 int qsm_fft3_dev(int dtype, void *data, const size_t M[3], int inverse,
                  void *stream);
@@ -323,6 +323,51 @@ int qsm_dipole_conv_dev(int dtype, const void *chi, const size_t sz[3],
 int qsm_dipole_conv(int dtype, const void *chi, const size_t sz[3],
                     const double h[3], const double bdir[3],
                     const size_t pad[3], void *field);
+
+/* ------------------------------------------------------------------------
+ * TV-regularised dipole inversion by ADMM (src/inversion/tv.m)
+ * ------------------------------------------------------------------------ */
+
+/* The opts fields of tv.m:38-86, same names and meaning. */
+typedef struct qsm_tv_opts {
+    double mu;      /* regularisation weight of the data term (7.5e3) */
+    int tv;         /* 1 anisotropic, 2 isotropic total variation (1) */
+    double rtol;    /* relative stopping tolerance (1e-1) */
+    double atol;    /* absolute stopping tolerance (1e-4) */
+    int maxit;      /* maximum ADMM iterations (20) */
+    int verbose;    /* print the residual table to stdout (1) */
+    double rho;     /* initial ADMM penalty (9) */
+    double alpha;   /* over-relaxation (1.5) */
+    double rtau;    /* penalty update: primal reduction ratio (0.7) */
+    double stau;    /* penalty update: primal/dual balance (1e2) */
+    double gamma;   /* penalty update factor (3) */
+} qsm_tv_opts;
+
+/* Fills *o with tv.m's defaults (tv.m:38-86). */
+void qsm_tv_opts_default(qsm_tv_opts *o);
+
+/* Bytes of device work space qsm_tv_dev needs for one sz[0] x sz[1] x sz[2]
+ * echo. */
+size_t qsm_tv_work_bytes(int dtype, const size_t sz[3]);
+
+/* Replaces tv.m:1-104 for one echo (tv_, tv.m:107-310): x = mask .* u
+ * after at most opts->maxit ADMM iterations on the local field f, with the
+ * dipole kernel of dipoleKernel.m:56-66 on the unpadded grid (bdir
+ * normalised) and periodic gradients (gradfp_mex / gradfp_adj_mex).  Every
+ * length must have no prime factor above 13.  trace (may be NULL): 5
+ * doubles per iteration done -- ||r||, eps_pri, ||s||, eps_dual (the
+ * verbose table) and rho after the iteration's update; *iters (may be
+ * NULL): iterations done.  Device pointers; work: qsm_tv_work_bytes bytes.
+ * Synchronises the stream once per iteration (the stopping rule and the
+ * penalty update are decided on the host, as in tv.m). */
+int qsm_tv_dev(int dtype, const void *f, const uint8_t *mask, const size_t sz[3],
+               const double vsz[3], const double bdir[3], const qsm_tv_opts *opts,
+               void *x, void *work, double *trace, int *iters, void *stream);
+
+/* Host-pointer form of qsm_tv_dev (synchronous). */
+int qsm_tv(int dtype, const void *f, const uint8_t *mask, const size_t sz[3],
+           const double vsz[3], const double bdir[3], const qsm_tv_opts *opts,
+           void *x, double *trace, int *iters);
 
 /* ------------------------------------------------------------------------
  * Device utilities

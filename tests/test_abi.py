@@ -62,3 +62,30 @@ def test_footprint_counts_state_words():
     assert 31 * 4 * N + N <= f32 <= 31 * 4 * N + N + 64 * 2 ** 20
     f64tv = _lib.lib().qsm_pd_footprint(_lib.TV, _lib.F64, sz)
     assert 13 * 8 * N + N <= f64tv <= 13 * 8 * N + N + 64 * 2 ** 20
+
+
+def test_tv_opts_defaults_are_tv_m():
+    """qsm_tv_opts_default fills tv.m:38-86's defaults; the Python mirror
+    keeps them for missing or empty fields and takes given ones."""
+    from qsm_amd import tv as gtv
+
+    o = gtv.make_opts(None)
+    assert (o.mu, o.tv, o.rtol, o.atol, o.maxit, o.verbose) == (7.5e3, 1, 1e-1, 1e-4, 20, 1)
+    assert (o.rho, o.alpha, o.rtau, o.stau, o.gamma) == (9.0, 1.5, 0.7, 1e2, 3.0)
+    o = gtv.make_opts({"mu": 1e3, "tv": 2, "maxit": [], "rho": None})
+    assert (o.mu, o.tv, o.maxit, o.rho) == (1e3, 2, 20, 9.0)
+
+
+@pytest.mark.parametrize("bad,msg", [({"tv": 3}, "opts.tv"), ({"maxit": 0}, "maxit"),
+                                     ({"rho": -1.0}, "rho")])
+def test_tv_argument_checks_without_gpu(bad, msg):
+    """Invalid opts are refused before any device work (QSM_EARG)."""
+    import numpy as np
+
+    from qsm_amd import QsmError
+    from qsm_amd import tv as gtv
+
+    f = np.zeros((8, 8, 8))
+    with pytest.raises(QsmError, match=msg) as e:
+        gtv.tv(f, np.ones(f.shape), (1, 1, 1), None, dict(bad, verbose=0))
+    assert e.value.code == _lib.QSM_EARG

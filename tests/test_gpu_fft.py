@@ -10,7 +10,8 @@ from qsm_amd import dipole, fastfft, phantom
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("shape", [(8, 9, 10), (28, 30, 14), (45, 7, 12), (64, 1, 3), (49, 25, 16), (2, 3, 5)])
+@pytest.mark.parametrize("shape", [(8, 9, 10), (28, 30, 14), (45, 7, 12), (64, 1, 3), (49, 25, 16), (2, 3, 5),
+                                   (11, 13, 22), (176, 26, 33), (143, 2, 1)])
 @pytest.mark.parametrize("dt", [np.complex128, np.complex64])
 def test_fft3_matches_pocketfft(cuda, shape, dt):
     rng = np.random.default_rng(0)
