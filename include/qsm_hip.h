@@ -358,8 +358,10 @@ size_t qsm_tv_work_bytes(int dtype, const size_t sz[3]);
  * doubles per iteration done -- ||r||, eps_pri, ||s||, eps_dual (the
  * verbose table) and rho after the iteration's update; *iters (may be
  * NULL): iterations done.  Device pointers; work: qsm_tv_work_bytes bytes.
- * Synchronises the stream once per iteration (the stopping rule and the
- * penalty update are decided on the host, as in tv.m). */
+ * The stopping rule and the penalty update are taken on the device after
+ * each iteration; the call enqueues opts->maxit iterations (those after the
+ * rule stops return at once) and synchronises the stream once, at the end
+ * (verbose: the residual table is printed then).  opts->maxit <= 65536. */
 int qsm_tv_dev(int dtype, const void *f, const uint8_t *mask, const size_t sz[3],
                const double vsz[3], const double bdir[3], const qsm_tv_opts *opts,
                void *x, void *work, double *trace, int *iters, void *stream);
@@ -368,6 +370,49 @@ int qsm_tv_dev(int dtype, const void *f, const uint8_t *mask, const size_t sz[3]
 int qsm_tv(int dtype, const void *f, const uint8_t *mask, const size_t sz[3],
            const double vsz[3], const double bdir[3], const qsm_tv_opts *opts,
            void *x, double *trace, int *iters);
+
+/* ------------------------------------------------------------------------
+ * Rapid two-step dipole inversion (src/inversion/rts.m)
+ * ------------------------------------------------------------------------ */
+
+/* The opts fields of rts.m:50-102, same names and meaning. */
+typedef struct qsm_rts_opts {
+    int tolls;      /* LSMR iterations of step 1 (4) */
+    double mu;      /* weight of the well-conditioned k-space data (1e5) */
+    double delta;   /* |D| threshold of the well-conditioned region (0.15) */
+    int tv;         /* 1 anisotropic, 2 isotropic total variation (1) */
+    double rtol;    /* relative stopping tolerance (1e-1) */
+    double atol;    /* absolute stopping tolerance (1e-2) */
+    int maxit;      /* maximum ADMM iterations (20) */
+    int verbose;    /* print the residual table to stdout (1) */
+    double rho;     /* initial ADMM penalty (10) */
+    double alpha;   /* over-relaxation (1.5) */
+    double rtau;    /* penalty update: primal reduction ratio (0.7) */
+    double stau;    /* penalty update: primal/dual balance (1e2) */
+    double gamma;   /* penalty update factor (3) */
+} qsm_rts_opts;
+
+/* Fills *o with rts.m's defaults (rts.m:50-102). */
+void qsm_rts_opts_default(qsm_rts_opts *o);
+
+/* Bytes of device work space qsm_rts_dev needs for one echo. */
+size_t qsm_rts_work_bytes(int dtype, const size_t sz[3]);
+
+/* Replaces rts.m:1-118 for one echo (rts_, rts.m:121-308): step 1 x1 =
+ * mask .* real(ifft3(lsmr(D, fft3(f), 0, eps, eps, [], tolls))) (the LSMR
+ * of third_party/lsmr/lsmr.m with the diagonal k-space operator D of
+ * dipoleKernel.m:56-66), step 2 the ADMM of tv.m with the data term
+ * M = mu (|D| > delta) on fft3(x1), starting from x1; x = mask .* x.  trace,
+ * iters and the synchronisation as qsm_tv_dev.  Every length must have no
+ * prime factor above 13. */
+int qsm_rts_dev(int dtype, const void *f, const uint8_t *mask, const size_t sz[3],
+                const double vsz[3], const double bdir[3], const qsm_rts_opts *opts,
+                void *x, void *work, double *trace, int *iters, void *stream);
+
+/* Host-pointer form of qsm_rts_dev (synchronous). */
+int qsm_rts(int dtype, const void *f, const uint8_t *mask, const size_t sz[3],
+            const double vsz[3], const double bdir[3], const qsm_rts_opts *opts,
+            void *x, double *trace, int *iters);
 
 /* ------------------------------------------------------------------------
  * Device utilities

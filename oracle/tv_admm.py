@@ -1,5 +1,6 @@
 """CPU restatement of src/inversion/tv.m (total-variation dipole inversion by
-ADMM), numpy.
+ADMM) and src/inversion/rts.m (rapid two-step inversion: LSMR in k-space,
+then the same ADMM), numpy.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/ (and by bench.py's CPU
 baseline leg), never by the product package.  It follows tv.m line by line
@@ -107,9 +108,10 @@ def grad_adj_(x, y, z, h):
     return orc.gradfp_adj(x, y, z, h)
 
 
-def inv_a(K, L, rho, T):
-    """tv.m:167-168: iA = 1 ./ (K + rho*L), non-finite -> 0 (K single or
-    double, rho*L double, converted to the class of K)."""
+def inv_a(K, L, rho):
+    """tv.m:167-168 / rts.m:158-159: iA = 1 ./ (K + rho*L), non-finite -> 0,
+    in the class of K (rho*L is double, converted to it)."""
+    T = K.dtype.type
     with np.errstate(divide="ignore", invalid="ignore"):
         iA = T(1) / (K + (rho * L).astype(T))
     iA[~np.isfinite(iA)] = 0
@@ -151,20 +153,28 @@ def tv_(f, mask, D, vsz, opts):
     (||r||, eps_pri, ||s||, eps_dual, rho after the update)."""
     T = f.dtype.type
     f = np.asfortranarray(f)
-    N = f.size
+    mu = opts["mu"]
+    F = (T(mu) * np.conj(D)) * fft3(f, T)       # tv.m:159
+    K = T(mu) * np.abs(D) ** 2                  # tv.m:161
+    return admm_(f, F, K, mask, vsz, opts)
+
+
+def admm_(u0, F, K, mask, vsz, opts):
+    """The ADMM loop shared by tv.m:141-308 and rts.m:145-308: u starts at
+    u0, F is the data term's spectrum, K its k-space weight (iA = 1/(K +
+    rho L))."""
+    T = u0.dtype.type
+    N = u0.size
     rtol, atol = opts["rtol"], opts["atol"]
     atolp = np.sqrt(3 * N) * atol
     atold = np.sqrt(N) * atol
-    mu = opts["mu"]
     rho, alpha = opts["rho"], opts["alpha"]
     rtau, stau, gamma = opts["rtau"], opts["stau"], opts["gamma"]
     mtv, maxit = opts["tv"], opts["maxit"]
     irho = 1 / rho
-
-    F = (T(mu) * np.conj(D)) * fft3(f, T)
-    K = T(mu) * np.abs(D) ** 2
-    L = laplace_symbol(f.shape, vsz)
-    iA = inv_a(K, L, rho, T)
+    f = u0
+    L = laplace_symbol(u0.shape, vsz)
+    iA = inv_a(K, L, rho)
 
     u = f
     lm = [np.zeros(f.shape, T, order="F") for _ in range(3)]
@@ -202,17 +212,149 @@ def tv_(f, mask, D, vsz, opts):
         if nr > stau * ns or (nr > rtau * nr_ and flag < 3):
             rho = gamma * rho
             irho = 1 / rho
-            iA = inv_a(K, L, rho, T)
+            iA = inv_a(K, L, rho)
             if flag > 5:
                 flag = 0
         elif nr < ns / stau:
             rho = rho / gamma ** 2
             irho = 1 / rho
-            iA = inv_a(K, L, rho, T)
+            iA = inv_a(K, L, rho)
             flag = flag + 1
         trace.append((nr, ep, ns, ed, rho))
     x = np.asfortranarray((np.asarray(mask) != 0).astype(T) * u)
     return x, np.array(trace)
+
+
+def lsmr(d, b, atol, btol, itnlim, conlim=1e8):
+    """third_party/lsmr/lsmr.m (Fong and Saunders' LSMR, lambda = 0, no
+    local reorthogonalisation) for the diagonal operator A = A' = diag(d)
+    and the right-hand side b, vectors in the class of b, scalars in double.
+    Returns (x, itn, istop)."""
+    d = np.asarray(d)
+    u = b.copy()
+    beta = float(np.linalg.norm(u))
+    if beta > 0:
+        u = u * (1 / beta)
+    v = d * u
+    alpha = float(np.linalg.norm(v))
+    if alpha > 0:
+        v = v * (1 / alpha)
+    itn, zetabar, alphabar, rho, rhobar, cbar, sbar = 0, alpha * beta, alpha, 1.0, 1.0, 1.0, 0.0
+    h, hbar, x = v.copy(), np.zeros_like(v), np.zeros_like(v)
+    betadd, betad, rhodold, tautildeold, thetatilde, zeta, dd = beta, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0
+    normA2, maxrbar, minrbar, normb = alpha ** 2, 0.0, 1e100, beta
+    ctol = 1 / conlim if conlim > 0 else 0.0
+    istop = 0
+    if alpha * beta == 0:
+        return x, 0, 0
+    while itn < itnlim:
+        itn += 1
+        u = d * v - alpha * u
+        beta = float(np.linalg.norm(u))
+        if beta > 0:
+            u = u * (1 / beta)
+            v = d * u - beta * v
+            alpha = float(np.linalg.norm(v))
+            if alpha > 0:
+                v = v * (1 / alpha)
+        alphahat, chat, shat = alphabar, 1.0, 0.0
+        rhoold = rho
+        rho = float(np.hypot(alphahat, beta))
+        c, s = alphahat / rho, beta / rho
+        thetanew = s * alpha
+        alphabar = c * alpha
+        rhobarold, zetaold = rhobar, zeta
+        thetabar, rhotemp = sbar * rho, cbar * rho
+        rhobar = float(np.hypot(cbar * rho, thetanew))
+        cbar, sbar = cbar * rho / rhobar, thetanew / rhobar
+        zeta = cbar * zetabar
+        zetabar = -sbar * zetabar
+        hbar = h - (thetabar * rho / (rhoold * rhobarold)) * hbar
+        x = x + (zeta / (rho * rhobar)) * hbar
+        h = v - (thetanew / rho) * h
+        betaacute, betacheck = chat * betadd, -shat * betadd
+        betahat = c * betaacute
+        betadd = -s * betaacute
+        thetatildeold = thetatilde
+        rhotildeold = float(np.hypot(rhodold, thetabar))
+        ctildeold, stildeold = rhodold / rhotildeold, thetabar / rhotildeold
+        thetatilde = stildeold * rhobar
+        rhodold = ctildeold * rhobar
+        betad = -stildeold * betad + ctildeold * betahat
+        tautildeold = (zetaold - thetatildeold * tautildeold) / rhotildeold
+        taud = (zeta - thetatilde * tautildeold) / rhodold
+        dd = dd + betacheck ** 2
+        normr = np.sqrt(dd + (betad - taud) ** 2 + betadd ** 2)
+        normA2 = normA2 + beta ** 2
+        normA = np.sqrt(normA2)
+        normA2 = normA2 + alpha ** 2
+        maxrbar = max(maxrbar, rhobarold)
+        if itn > 1:
+            minrbar = min(minrbar, rhobarold)
+        condA = max(maxrbar, rhotemp) / min(minrbar, rhotemp)
+        normAr = abs(zetabar)
+        normx = float(np.linalg.norm(x))
+        test1, test2, test3 = normr / normb, normAr / (normA * normr), 1 / condA
+        t1 = test1 / (1 + normA * normx / normb)
+        rtol = btol + atol * normA * normx / normb
+        if itn >= itnlim:
+            istop = 7
+        if 1 + test3 <= 1:
+            istop = 6
+        if 1 + test2 <= 1:
+            istop = 5
+        if 1 + t1 <= 1:
+            istop = 4
+        if test3 <= ctol:
+            istop = 3
+        if test2 <= atol:
+            istop = 2
+        if test1 <= rtol:
+            istop = 1
+        if istop > 0:
+            break
+    return x, itn, istop
+
+
+RTS_DEFAULTS = dict(tolls=4, mu=1e5, delta=0.15, tv=1, rtol=1e-1, atol=1e-2, maxit=20,
+                    verbose=0, rho=10.0, alpha=1.5, rtau=0.7, stau=1e2, gamma=3.0)  # rts.m:50-102
+
+
+def rts_(f, mask, D, vsz, opts):
+    """rts.m:121-308 for one echo.  Returns (x, trace)."""
+    T = f.dtype.type
+    f = np.asfortranarray(f)
+    eps = float(np.finfo(T).eps)
+    # step 1: well-conditioned (rts.m:128-139)
+    x1, _, _ = lsmr(D.reshape(-1, order="F"), fft3(f, T).reshape(-1, order="F"), eps, eps,
+                    opts["tolls"])
+    x1 = x1.reshape(f.shape, order="F")
+    m = (np.asarray(mask) != 0).astype(T)
+    x1 = np.asfortranarray(m * np.real(ifft3(x1, T)).astype(T))
+    # step 2: ill-conditioned (rts.m:145-159); M = mu*(|D| > delta) is double
+    M = opts["mu"] * (np.abs(D) > T(opts["delta"])).astype(np.float64)
+    F = (M * fft3(x1, T)).astype(_cplx(T))
+    return admm_(x1, F, M, mask, vsz, opts)
+
+
+def rts(f, mask, vsz, bdir=(0.0, 0.0, 1.0), opts=None):
+    """rts.m:1-118: per echo of a 3-D or 4-D f.  Returns (x, traces)."""
+    o = dict(RTS_DEFAULTS)
+    o.update({k: v for k, v in (opts or {}).items() if v is not None})
+    f = np.asarray(f)
+    if f.dtype not in (np.float32, np.float64):
+        f = f.astype(np.float64)
+    T = f.dtype.type
+    vsz = np.broadcast_to(np.asarray(vsz, dtype=np.float64), (3,))
+    D = dipole_kernel_k(np.shape(mask), vsz, bdir, T)
+    if f.ndim == 3:
+        return rts_(f, mask, D, vsz, o)
+    x = np.zeros(f.shape, T, order="F")
+    traces = []
+    for t in range(f.shape[3]):
+        x[..., t], tr = rts_(np.asfortranarray(f[..., t]), mask, D, vsz, o)
+        traces.append(tr)
+    return x, traces
 
 
 def tv(f, mask, vsz, bdir=(0.0, 0.0, 1.0), opts=None):

@@ -17,12 +17,21 @@
 // ((nx/2 + 1) ny nz complex).  Per iteration four launches touch HBM: the
 // right-hand side (gradAdj of rho v - l, 6 reads + 1 write per voxel), the
 // half-spectrum solve (five FFT passes, the k-space multipliers generated
-// in-kernel), the v/l update with its residual partial sums (7 reads + 9
-// writes), and the dual residual norms (6 reads).  Sums are accumulated in
-// double per block and reduced in a fixed order, so they are reproducible.
+// in-kernel from per-axis tables), the v/l update with its residual partial
+// sums (7 reads + 9 writes), and the dual residual norms (6 reads).  Sums
+// are accumulated in double per block and reduced in a fixed order, so
+// they are reproducible.
+//
+// The loop runs on the device: the stopping rule and the penalty update
+// (tv.m:279-306) are taken by a one-block kernel after each iteration's
+// reduction, from a control block in device memory that every kernel of
+// the next iteration reads (rho, 1/rho, stop).  The host enqueues maxit
+// iterations without waiting; once the rule stops the loop, the remaining
+// launches return at once.  One synchronisation per solve.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <limits>
 #include <vector>
 
 #include "fft_dev.h"
@@ -33,6 +42,30 @@ using namespace qsm::fft;
 namespace {
 
 constexpr int BX = 64, BY = 4, NT = BX * BY;
+// partial sums per block: [0..2] from the v-update (nr, na, nb), [3..4]
+// from the dual norms (||s||^2, ||rho gradAdj(l)||^2); the initial |v| sum
+// uses [0]
+constexpr int NPART = 5;
+// first-level reduction blocks (each sums a contiguous range of the
+// per-block partials; the decision kernel sums their RB results)
+constexpr int RB = 128;
+// iterations the device trace holds
+constexpr int MAX_ITERS = 65536;
+
+// tv.m's loop scalars, in device memory.
+struct Ctl {
+    double rho, irho;  // penalty and its inverse for the next iteration
+    double nr;         // last primal residual (nr_ of the next check)
+    int flag;          // penalty-update counter (tv.m:287-306)
+    int stop;          // set when the rule stops: later launches return
+    int iters;         // iterations done
+    int pad;
+};
+
+// Constants of the stopping rule and the penalty update.
+struct Rule {
+    double atolp, atold, rtol, rtau, stau, gamma;
+};
 
 struct G3 {
     int nx, ny, nz;
@@ -143,15 +176,17 @@ __global__ __launch_bounds__(NT) void init_kernel(Vec3<T> v, Vec3<T> lm, const T
         lm.z[l] = T(0);
         s[0] = double(sqrt((a * a + b * b) + d * d));
     }
-    block_sum<1>(s, part + flat_block());
+    block_sum<1>(s, part + NPART * flat_block());
 }
 
 // tv.m:196-200: r = gradAdj(rho v - l) (gradfp_adj_mex with -1/h), the
 // differences rho*v - l formed per sample as the reference forms r1..r3.
 template <typename T>
-__global__ __launch_bounds__(NT) void rhs_kernel(T *r, CVec3<T> v, CVec3<T> lm, G3 g, T rho,
-                                                 T hx, T hy, T hz)
+__global__ __launch_bounds__(NT) void rhs_kernel(T *r, CVec3<T> v, CVec3<T> lm, G3 g,
+                                                 const Ctl *ctl, T hx, T hy, T hz)
 {
+    if (ctl->stop) return;
+    const T rho = T(ctl->rho);
     int i, j, k, l;
     if (!vox(g, i, j, k, l)) return;
     int mx, my, mz;
@@ -174,8 +209,10 @@ __device__ __forceinline__ T sgn_of(T x)
 template <typename T, bool ISO>
 __global__ __launch_bounds__(NT) void shrink_kernel(Vec3<T> v, Vec3<T> lm, Vec3<T> dv,
                                                     const T *u, G3 g, T hx, T hy, T hz,
-                                                    T alpha, T oma, T rho, T irho, double *part)
+                                                    T alpha, T oma, const Ctl *ctl, double *part)
 {
+    if (ctl->stop) return;
+    const T rho = T(ctl->rho), irho = T(ctl->irho);
     int i, j, k, l;
     double s[3] = {0.0, 0.0, 0.0};
     if (vox(g, i, j, k, l)) {
@@ -219,15 +256,17 @@ __global__ __launch_bounds__(NT) void shrink_kernel(Vec3<T> v, Vec3<T> lm, Vec3<
         s[1] = double(sqrt((dx * dx + dy * dy) + dz * dz));
         s[2] = double(sqrt((n1 * n1 + n2 * n2) + n3 * n3));
     }
-    block_sum<3>(s, part + 3 * flat_block());
+    block_sum<3>(s, part + NPART * flat_block());
 }
 
 // tv.m:262-277: squared norms of rho gradAdj(dv) (dual residual) and of
 // rho gradAdj(l) (the dual tolerance's scale).
 template <typename T>
-__global__ __launch_bounds__(NT) void dual_kernel(CVec3<T> dv, CVec3<T> lm, G3 g, T rho, T hx,
-                                                  T hy, T hz, double *part)
+__global__ __launch_bounds__(NT) void dual_kernel(CVec3<T> dv, CVec3<T> lm, G3 g, const Ctl *ctl,
+                                                  T hx, T hy, T hz, double *part)
 {
+    if (ctl->stop) return;
+    const T rho = T(ctl->rho);
     int i, j, k, l;
     double s[2] = {0.0, 0.0};
     if (vox(g, i, j, k, l)) {
@@ -240,23 +279,104 @@ __global__ __launch_bounds__(NT) void dual_kernel(CVec3<T> dv, CVec3<T> lm, G3 g
         s[0] = double(a) * double(a);
         s[1] = double(b) * double(b);
     }
-    block_sum<2>(s, part + 2 * flat_block());
+    block_sum<2>(s, part + NPART * flat_block() + 3);
 }
 
-// Totals of NQ interleaved partial sums, in a fixed order.
-template <int NQ>
-__global__ __launch_bounds__(NT) void finish_kernel(const double *part, int nblk, double *out)
+// First reduction level: block b of RB sums the NPART partials of a
+// contiguous range of voxel blocks (a fixed order).
+__global__ __launch_bounds__(NT) void reduce1_kernel(const double *part, int nblk, double *out,
+                                                     const Ctl *ctl)
 {
-    double s[NQ];
+    if (ctl && ctl->stop) return;
+    const int chunk = (nblk + RB - 1) / RB;
+    const int b0 = int(blockIdx.x) * chunk, b1 = min(nblk, b0 + chunk);
+    double s[NPART];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q)
+    for (int q = 0; q < NPART; ++q)
         s[q] = 0.0;
     const int t = threadIdx.x + BX * threadIdx.y;
-    for (int b = t; b < nblk; b += NT)
+    for (int b = b0 + t; b < b1; b += NT)
 #pragma unroll
-        for (int q = 0; q < NQ; ++q)
-            s[q] += part[NQ * b + q];
-    block_sum<NQ>(s, out);
+        for (int q = 0; q < NPART; ++q)
+            s[q] += part[NPART * b + q];
+    block_sum<NPART>(s, out + NPART * blockIdx.x);
+}
+
+__device__ __forceinline__ void sum_level1(const double *p1, double (&tot)[NPART])
+{
+    double s[NPART];
+#pragma unroll
+    for (int q = 0; q < NPART; ++q)
+        s[q] = 0.0;
+    const int t = threadIdx.x + BX * threadIdx.y;
+    for (int b = t; b < RB; b += NT)
+#pragma unroll
+        for (int q = 0; q < NPART; ++q)
+            s[q] += p1[NPART * b + q];
+    __shared__ double res[NPART];
+    block_sum<NPART>(s, res);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NPART; ++q)
+        tot[q] = res[q];
+}
+
+// tv.m:141-149 scalars: nr = sum |grad f| (the init kernel's sums), rho and
+// the counters at their start values.
+__global__ __launch_bounds__(NT) void start_kernel(const double *p1, Ctl *ctl, double rho)
+{
+    double tot[NPART];
+    sum_level1(p1, tot);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        ctl->rho = rho;
+        ctl->irho = 1.0 / rho;
+        ctl->nr = tot[0];
+        ctl->flag = 0;
+        ctl->stop = 0;
+        ctl->iters = 0;
+    }
+}
+
+// tv.m:258-306 after iteration it: the residual norms, the stopping rule
+// and the penalty update; writes the trace row (||r||, eps_pri, ||s||,
+// eps_dual, rho after the update).
+__global__ __launch_bounds__(NT) void decide_kernel(const double *p1, Ctl *ctl, Rule r,
+                                                    double *trace, int it)
+{
+    if (ctl->stop) return;
+    double tot[NPART];
+    sum_level1(p1, tot);
+    if (threadIdx.x != 0 || threadIdx.y != 0) return;
+    const double nr_ = ctl->nr;
+    const double nr = tot[0], na = tot[1], nb = tot[2];
+    const double ns = sqrt(tot[3]);
+    const double ep = r.atolp + r.rtol * fmax(na, nb);
+    const double ed = r.atold + r.rtol * sqrt(tot[4]);
+    double rho = ctl->rho;
+    int flag = ctl->flag;
+    const bool stop = nr < ep && ns < ed;
+    if (!stop) {
+        if (nr > r.stau * ns || (nr > r.rtau * nr_ && flag < 3)) {
+            rho = r.gamma * rho;
+            if (flag > 5)
+                flag = 0;
+        } else if (nr < ns / r.stau) {
+            rho = rho / (r.gamma * r.gamma);
+            flag = flag + 1;
+        }
+    }
+    double *row = trace + 5 * (it - 1);
+    row[0] = nr;
+    row[1] = ep;
+    row[2] = ns;
+    row[3] = ed;
+    row[4] = rho;
+    ctl->nr = nr;
+    ctl->rho = rho;
+    ctl->irho = 1.0 / rho;
+    ctl->flag = flag;
+    ctl->iters = it;
+    ctl->stop = stop ? 1 : 0;
 }
 
 // tv.m:308: x = mask .* u.
@@ -268,50 +388,73 @@ __global__ __launch_bounds__(NT) void mask_kernel(T *x, const T *u, const uint8_
     x[l] = T(m[l] ? 1 : 0) * u[l];
 }
 
+template <typename T>
+__global__ __launch_bounds__(NT) void zero_kernel(T *x, G3 g)
+{
+    int i, j, k, l;
+    if (!vox(g, i, j, k, l)) return;
+    x[l] = T(0);
+}
+
 // The forward transform of f is kept as it comes from the x pass (doubled).
 template <typename T>
 struct KeepOp {
     __device__ Cx<T> operator()(int, int, int, Cx<T> v) const { return v; }
 };
 
-// k-space operation of the u-subproblem at half-spectrum sample (x, y, z),
-// tv.m:159-174 and :203-204: with iA(k) = 1/(mu D(k)^2 + rho L(k)) (0 where
-// not finite), the reference forms real(ifft3(iA .* (mu conj(D) fft3(f) +
-// fft3(r)))).  For real f and r that is the inverse transform of
-// A1s fft3(f) + A2s fft3(r), A1 = mu iA D, A2 = iA, As(k) = (A(k) + A(-k))/2;
-// As = A except on the Nyquist plane of an even axis with a tilted B0 (D
-// differs at k and -k there; L does not).  L(k) = sum_a (2 - 2 cos(2 pi m_a /
-// n_a)) / h_a^2 is the symbol of -laplaceKernel (laplaceKernel.m:14-24)
-// padded and circularly shifted to the origin (tv.m:163-165).  In double,
+// k-space operation of the u-subproblem at half-spectrum sample (x, y, z).
+// tv.m:159-174, :203-204: iA(k) = 1/(mu D(k)^2 + rho L(k)) (0 where not
+// finite) and u = real(ifft3(iA .* (mu conj(D) fft3(f) + fft3(r)))).
+// rts.m:145-157, :203-204 (RTS): M(k) = mu [|D(k)| > delta] (D in the class
+// of f), iA(k) = 1/(M(k) + rho L(k)) and x = real(ifft3(iA .* (M fft3(x1) +
+// fft3(r)))).  With the source s (f or x1) and r real, that is the inverse
+// transform of A1s fft3(s) + A2s fft3(r), A1 = iA mu D (tv) or iA M (rts),
+// A2 = iA, As(k) = (A(k) + A(-k))/2; As = A except on the Nyquist plane of
+// an even axis with a tilted B0 (D differs at k and -k there; L does not).
+// L(k) = sum_a (2 - 2 cos(2 pi m_a / n_a)) / h_a^2 is the symbol of
+// -laplaceKernel (laplaceKernel.m:14-24) padded and circularly shifted to
+// the origin (tv.m:163-165), with L(0) = 0 exactly.  The k coordinates and
+// the per-axis terms of L come from tables (tab: k and L along x for H
+// entries, then y for M1, then z for M2); the rest is formed in double and
 // rounded once to T.
-template <typename T>
+template <typename T, bool RTS>
 struct SolveOp {
-    const Cx<T> *F;  // 2 fft3(f) on the half spectrum
+    const Cx<T> *F;    // 2 fft3(source) on the half spectrum
+    const double *tab;
+    const Ctl *ctl;
     int M0, M1, M2, H;
-    double dk0, dk1, dk2, b0, b1, b2;
-    double mu, rho, lx, ly, lz, scale;
+    double b0, b1, b2, mu, delta, scale;
 
-    __device__ static double lap1(int m, int n, double ih2)
+    __device__ void weights(double D, double L, double rho, double &a1, double &a2) const
     {
-        const int mm = min(m, n - m);  // cos(2 pi m/n) = cos(2 pi (n-m)/n), evaluated once
-        return (2.0 - 2.0 * cospi(2.0 * double(mm) / double(n))) * ih2;
+        double K, g;
+        if constexpr (RTS) {
+            K = fabs(T(D)) > T(delta) ? mu : 0.0;
+            g = K;
+        } else {
+            K = mu * D * D;
+            g = mu * D;
+        }
+        double iA = 1.0 / (K + rho * L);
+        if (!isfinite(iA)) iA = 0.0;
+        a1 = iA * g;
+        a2 = iA;
     }
 
     __device__ Cx<T> operator()(int x, int y, int z, Cx<T> fr) const
     {
-        const double X = kcoord(x, M0, dk0), Y = kcoord(y, M1, dk1), Z = kcoord(z, M2, dk2);
-        const double L = (lap1(x, M0, lx) + lap1(y, M1, ly)) + lap1(z, M2, lz);
-        double D = dip(X, Y, Z, b0, b1, b2);
-        double iA = 1.0 / (mu * D * D + rho * L);
-        if (!isfinite(iA)) iA = 0.0;
-        double a1 = mu * iA * D, a2 = iA;
+        const double *ty = tab + 2 * H, *tz = ty + 2 * M1;
+        const double X = tab[x], Y = ty[y], Z = tz[z];
+        const double L = (tab[H + x] + ty[M1 + y]) + tz[M2 + z];
+        const double rho = ctl->rho;
+        double a1, a2;
+        weights(dip(X, Y, Z, b0, b1, b2), L, rho, a1, a2);
         const bool nx = 2 * x == M0, ny = 2 * y == M1, nz = 2 * z == M2;
         if (nx || ny || nz) {
-            const double Dm = dip(nx ? X : -X, ny ? Y : -Y, nz ? Z : -Z, b0, b1, b2);
-            double iAm = 1.0 / (mu * Dm * Dm + rho * L);
-            if (!isfinite(iAm)) iAm = 0.0;
-            a1 = 0.5 * (a1 + mu * iAm * Dm);
-            a2 = 0.5 * (a2 + iAm);
+            double m1, m2;
+            weights(dip(nx ? X : -X, ny ? Y : -Y, nz ? Z : -Z, b0, b1, b2), L, rho, m1, m2);
+            a1 = 0.5 * (a1 + m1);
+            a2 = 0.5 * (a2 + m2);
         }
         const T c1 = T(a1 * scale), c2 = T(a2 * scale);
         const Cx<T> ff = F[size_t(x) + size_t(H) * (size_t(y) + size_t(M1) * z)];
@@ -319,26 +462,45 @@ struct SolveOp {
     }
 };
 
+// Per-axis table of the solve: k coordinate (dipoleKernel.m grid followed
+// by ifftshift, the same formula as kcoord) then (2 - 2 cos(2 pi m/n))/h^2
+// (evaluated at min(m, n - m), so that m and n - m agree exactly).
+void axis_table(double *k, double *L, int count, int n, double h)
+{
+    const double dk = 1.0 / (double(n) * h), ih2 = 1.0 / (h * h);
+    const int low = n / 2;
+    for (int m = 0; m < count; ++m) {
+        k[m] = double(m < n - low ? m : m - n) * dk;
+        const int mm = std::min(m, n - m);
+        const long double a = 2.0L * 3.141592653589793238462643383279502884L * mm / n;
+        L[m] = mm == 0 ? 0.0 : double((2.0L - 2.0L * cosl(a)) * ih2);
+    }
+}
+
 // Work space, carved from one device buffer (256-byte aligned pieces).
 template <typename T>
 struct Work {
     Cx<T> *F, *W;
     T *r, *u;
     Vec3<T> v, l, dv;
-    double *part, *tot;
+    double *part, *p1, *tab, *trace;
+    Ctl *ctl;
+    // rts step 1 (LSMR on the full spectrum): u, v, h, hbar, x and D
+    Cx<T> *lu, *lv, *lh, *lhb, *lx;
+    T *D;
 };
 
 size_t align256(size_t n) { return (n + 255) / 256 * 256; }
 
 template <typename T>
-size_t work_layout(const size_t sz[3], Work<T> *w, char *base)
+size_t work_layout(const size_t sz[3], Work<T> *w, char *base, bool lsmr = false)
 {
     const size_t N = sz[0] * sz[1] * sz[2];
     const size_t M[3] = {sz[0], sz[1], sz[2]};
     const size_t spec = align256(real_conv_work_bytes(M, sizeof(T)));
     const size_t vol = align256(N * sizeof(T));
     const G3 g{int(sz[0]), int(sz[1]), int(sz[2])};
-    const size_t part = align256(size_t(vox_blocks(g)) * 3 * sizeof(double));
+    const size_t part = align256(size_t(vox_blocks(g)) * NPART * sizeof(double));
     size_t off = 0;
     auto take = [&](size_t n) { char *p = base ? base + off : nullptr; off += n; return p; };
     char *pF = take(spec), *pW = take(spec), *pr = take(vol), *pu = take(vol);
@@ -346,8 +508,22 @@ size_t work_layout(const size_t sz[3], Work<T> *w, char *base)
     for (auto &p : pv) p = take(vol);
     for (auto &p : pl) p = take(vol);
     for (auto &p : pd) p = take(vol);
-    char *pp = take(part), *pt = take(256);
+    char *pp = take(part), *p1 = take(align256(RB * NPART * sizeof(double)));
+    char *ptab = take(align256(2 * (sz[0] / 2 + 1 + sz[1] + sz[2]) * sizeof(double)));
+    char *ptr = take(align256(size_t(MAX_ITERS) * 5 * sizeof(double)));
+    char *pc = take(256);
+    char *pls[5] = {nullptr, nullptr, nullptr, nullptr, nullptr}, *pD = nullptr;
+    if (lsmr) {
+        for (auto &p : pls) p = take(align256(N * sizeof(Cx<T>)));
+        pD = take(vol);
+    }
     if (w) {
+        w->lu = reinterpret_cast<Cx<T> *>(pls[0]);
+        w->lv = reinterpret_cast<Cx<T> *>(pls[1]);
+        w->lh = reinterpret_cast<Cx<T> *>(pls[2]);
+        w->lhb = reinterpret_cast<Cx<T> *>(pls[3]);
+        w->lx = reinterpret_cast<Cx<T> *>(pls[4]);
+        w->D = reinterpret_cast<T *>(pD);
         w->F = reinterpret_cast<Cx<T> *>(pF);
         w->W = reinterpret_cast<Cx<T> *>(pW);
         w->r = reinterpret_cast<T *>(pr);
@@ -356,20 +532,140 @@ size_t work_layout(const size_t sz[3], Work<T> *w, char *base)
         w->l = {reinterpret_cast<T *>(pl[0]), reinterpret_cast<T *>(pl[1]), reinterpret_cast<T *>(pl[2])};
         w->dv = {reinterpret_cast<T *>(pd[0]), reinterpret_cast<T *>(pd[1]), reinterpret_cast<T *>(pd[2])};
         w->part = reinterpret_cast<double *>(pp);
-        w->tot = reinterpret_cast<double *>(pt);
+        w->p1 = reinterpret_cast<double *>(p1);
+        w->tab = reinterpret_cast<double *>(ptab);
+        w->trace = reinterpret_cast<double *>(ptr);
+        w->ctl = reinterpret_cast<Ctl *>(pc);
     }
     return off;
 }
 
-// Reads NQ totals back (the loop's decisions are taken on the host, as in
-// tv.m).
-template <int NQ>
-int read_totals(const double *part, int nblk, double *dtot, double *h, hipStream_t st)
+// The ADMM parameters shared by tv.m and rts.m (step 2).
+struct AdmmParams {
+    double mu, delta;  // delta: rts only
+    int tv, maxit, verbose;
+    double rtol, atol, rho, alpha, rtau, stau, gamma;
+};
+
+// B0 direction normalised as dipoleKernel.m:35-38 does.
+int unit_b(const double bdir[3], double b[3])
 {
-    finish_kernel<NQ><<<1, dim3(BX, BY), 0, st>>>(part, nblk, dtot);
+    const double bn = std::sqrt(bdir[0] * bdir[0] + bdir[1] * bdir[1] + bdir[2] * bdir[2]);
+    QSM_ARG(bn > 0.0, "B0 direction must be non-zero");
+    for (int a = 0; a < 3; ++a) b[a] = bn != 1.0 ? bdir[a] / bn : bdir[a];
+    return QSM_OK;
+}
+
+// The ADMM loop of tv.m:141-308 (RTS = false: source f) and rts.m:145-308
+// (RTS = true: source x1): F from the source, u = source, v = grad(u), l = 0,
+// then at most maxit iterations decided on the device, and x = mask .* u.
+// src may alias w.u.
+template <typename T, bool RTS>
+int admm_run(const T *src, const uint8_t *mask, const size_t sz[3], const double vsz[3],
+             const double b[3], const AdmmParams &o, T *x, Work<T> &w, double *trace,
+             int *iters, hipStream_t st)
+{
+    int rc;
+    const G3 g{int(sz[0]), int(sz[1]), int(sz[2])};
+    const size_t N = sz[0] * sz[1] * sz[2];
+    const dim3 grid = vox_grid(g), blk(BX, BY);
+    const int nblk = vox_blocks(g);
+
+    // gradient coefficients as gradfp_mex / gradfp_adj_mex form them
+    const T gx = T(1.0 / vsz[0]), gy = T(1.0 / vsz[1]), gz = T(1.0 / vsz[2]);
+    const T ax = T(-1.0 / vsz[0]), ay = T(-1.0 / vsz[1]), az = T(-1.0 / vsz[2]);
+
+    // per-axis tables of the solve
+    const int H = g.nx / 2 + 1;
+    std::vector<double> tab(2 * (H + g.ny + g.nz));
+    axis_table(tab.data(), tab.data() + H, H, g.nx, vsz[0]);
+    axis_table(tab.data() + 2 * H, tab.data() + 2 * H + g.ny, g.ny, g.ny, vsz[1]);
+    axis_table(tab.data() + 2 * H + 2 * g.ny, tab.data() + 2 * H + 2 * g.ny + g.nz, g.nz, g.nz,
+               vsz[2]);
+    QSM_HIP(hipMemcpyAsync(w.tab, tab.data(), tab.size() * sizeof(double),
+                           hipMemcpyHostToDevice, st));
+
+    const size_t M[3] = {sz[0], sz[1], sz[2]}, off[3] = {0, 0, 0};
+    // F: 2 fft3(source) on the half spectrum (the data-term factor is
+    // applied in the solve)
+    if ((rc = real_conv<T, false>(src, static_cast<T *>(nullptr), sz, M, off, w.F, KeepOp<T>{},
+                                  st)))
+        return rc;
+
+    SolveOp<T, RTS> op;
+    op.F = w.F;
+    op.tab = w.tab;
+    op.ctl = w.ctl;
+    op.M0 = g.nx;
+    op.M1 = g.ny;
+    op.M2 = g.nz;
+    op.H = H;
+    op.b0 = b[0];
+    op.b1 = b[1];
+    op.b2 = b[2];
+    op.mu = o.mu;
+    op.delta = o.delta;
+    // 1/(nx ny nz) of ifft3, 1/2 for the x pass's doubling
+    op.scale = 0.5 / (double(g.nx) * double(g.ny) * double(g.nz));
+
+    // tv.m:127-130 stopping tolerances, :279-306 penalty update
+    const Rule rule{std::sqrt(3.0 * double(N)) * o.atol, std::sqrt(double(N)) * o.atol, o.rtol,
+                    o.rtau, o.stau, o.gamma};
+
+    // v = grad(u), l = 0, nr = sum |v| (tv.m:141-149)
+    init_kernel<T><<<grid, blk, 0, st>>>(w.v, w.l, src, g, gx, gy, gz, w.part);
     QSM_HIP(hipGetLastError());
-    QSM_HIP(hipMemcpyAsync(h, dtot, NQ * sizeof(double), hipMemcpyDeviceToHost, st));
+    reduce1_kernel<<<RB, blk, 0, st>>>(w.part, nblk, w.p1, nullptr);
+    QSM_HIP(hipGetLastError());
+    start_kernel<<<1, blk, 0, st>>>(w.p1, w.ctl, o.rho);
+    QSM_HIP(hipGetLastError());
+
+    const CVec3<T> cv{w.v.x, w.v.y, w.v.z}, cl{w.l.x, w.l.y, w.l.z}, cd{w.dv.x, w.dv.y, w.dv.z};
+    const int *stop = &w.ctl->stop;
+    for (int it = 1; it <= o.maxit; ++it) {
+        // u-subproblem (tv.m:196-203)
+        rhs_kernel<T><<<grid, blk, 0, st>>>(w.r, cv, cl, g, w.ctl, ax, ay, az);
+        QSM_HIP(hipGetLastError());
+        if ((rc = real_conv<T, true>(w.r, w.u, sz, M, off, w.W, op, st, stop))) return rc;
+        // v-subproblem, multipliers, primal sums (tv.m:205-256)
+        if (o.tv == 1)
+            shrink_kernel<T, false><<<grid, blk, 0, st>>>(w.v, w.l, w.dv, w.u, g, gx, gy, gz,
+                                                         T(o.alpha), T(1.0 - o.alpha), w.ctl,
+                                                         w.part);
+        else
+            shrink_kernel<T, true><<<grid, blk, 0, st>>>(w.v, w.l, w.dv, w.u, g, gx, gy, gz,
+                                                        T(o.alpha), T(1.0 - o.alpha), w.ctl,
+                                                        w.part);
+        QSM_HIP(hipGetLastError());
+        // dual norms (tv.m:262-277), then the rule (tv.m:279-306)
+        dual_kernel<T><<<grid, blk, 0, st>>>(cd, cl, g, w.ctl, ax, ay, az, w.part);
+        QSM_HIP(hipGetLastError());
+        reduce1_kernel<<<RB, blk, 0, st>>>(w.part, nblk, w.p1, w.ctl);
+        QSM_HIP(hipGetLastError());
+        decide_kernel<<<1, blk, 0, st>>>(w.p1, w.ctl, rule, w.trace, it);
+        QSM_HIP(hipGetLastError());
+    }
+    // x = mask .* u (tv.m:308)
+    mask_kernel<T><<<grid, blk, 0, st>>>(x, w.u, mask, g);
+    QSM_HIP(hipGetLastError());
+
+    Ctl c;
+    QSM_HIP(hipMemcpyAsync(&c, w.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
     QSM_HIP(hipStreamSynchronize(st));
+    std::vector<double> tr(size_t(c.iters) * 5);
+    if (c.iters > 0)
+        QSM_HIP(hipMemcpy(tr.data(), w.trace, tr.size() * sizeof(double), hipMemcpyDeviceToHost));
+    if (o.verbose) {
+        std::printf("\niter\t\t||r||\t\teps_pri\t\t||s||\t\teps_dual\n");
+        for (int i = 0; i < c.iters; ++i)
+            std::printf("%3d/%d\t\t%.4e\t%.4e\t%.4e\t%.4e\n", i + 1, o.maxit, tr[5 * i],
+                        tr[5 * i + 1], tr[5 * i + 2], tr[5 * i + 3]);
+        std::printf("\n");
+    }
+    if (trace && c.iters > 0)
+        std::copy(tr.begin(), tr.end(), trace);
+    if (iters)
+        *iters = c.iters;
     return QSM_OK;
 }
 
@@ -378,144 +674,325 @@ int tv_dev_t(const T *f, const uint8_t *mask, const size_t sz[3], const double v
              const double bdir[3], const qsm_tv_opts &o, T *x, void *work, double *trace,
              int *iters, hipStream_t st)
 {
-    int rc;
-    const G3 g{int(sz[0]), int(sz[1]), int(sz[2])};
-    const size_t N = sz[0] * sz[1] * sz[2];
+    double b[3];
+    int rc = unit_b(bdir, b);
+    if (rc) return rc;
     Work<T> w;
     work_layout<T>(sz, &w, static_cast<char *>(work));
-    const dim3 grid = vox_grid(g), blk(BX, BY);
-    const int nblk = vox_blocks(g);
+    const AdmmParams p{o.mu, 0.0, o.tv, o.maxit, o.verbose, o.rtol, o.atol,
+                       o.rho, o.alpha, o.rtau, o.stau, o.gamma};
+    return admm_run<T, false>(f, mask, sz, vsz, b, p, x, w, trace, iters, st);
+}
 
-    // normalised b (dipoleKernel.m:35-38)
-    const double bn = std::sqrt(bdir[0] * bdir[0] + bdir[1] * bdir[1] + bdir[2] * bdir[2]);
-    QSM_ARG(bn > 0.0, "B0 direction must be non-zero");
-    double b[3] = {bdir[0], bdir[1], bdir[2]};
-    if (bn != 1.0)
-        for (double &e : b) e /= bn;
+// ---- rts.m step 1: LSMR (third_party/lsmr/lsmr.m) with A = diag(D) ------
 
-    // gradient coefficients as gradfp_mex / gradfp_adj_mex form them
-    const T gx = T(1.0 / vsz[0]), gy = T(1.0 / vsz[1]), gz = T(1.0 / vsz[2]);
-    const T ax = T(-1.0 / vsz[0]), ay = T(-1.0 / vsz[1]), az = T(-1.0 / vsz[2]);
+constexpr int LT = 256;  // threads per block of the 1-D LSMR kernels
 
-    const size_t M[3] = {sz[0], sz[1], sz[2]}, off[3] = {0, 0, 0};
-    // F: 2 fft3(f) on the half spectrum (the mu conj(D) factor is applied in
-    // the solve, tv.m:159)
-    if ((rc = real_conv<T, false>(f, static_cast<T *>(nullptr), sz, M, off, w.F, KeepOp<T>{}, st)))
-        return rc;
+__device__ __forceinline__ double abs2(Cx<float> a)
+{
+    return double(a.re) * double(a.re) + double(a.im) * double(a.im);
+}
+__device__ __forceinline__ double abs2(Cx<double> a) { return a.re * a.re + a.im * a.im; }
 
-    SolveOp<T> op;
-    op.F = w.F;
-    op.M0 = g.nx;
-    op.M1 = g.ny;
-    op.M2 = g.nz;
-    op.H = g.nx / 2 + 1;
-    op.dk0 = 1.0 / (double(g.nx) * vsz[0]);
-    op.dk1 = 1.0 / (double(g.ny) * vsz[1]);
-    op.dk2 = 1.0 / (double(g.nz) * vsz[2]);
-    op.b0 = b[0];
-    op.b1 = b[1];
-    op.b2 = b[2];
-    op.mu = o.mu;
-    op.lx = 1.0 / (vsz[0] * vsz[0]);
-    op.ly = 1.0 / (vsz[1] * vsz[1]);
-    op.lz = 1.0 / (vsz[2] * vsz[2]);
-    // 1/(nx ny nz) of ifft3, 1/2 for the x pass's doubling
-    op.scale = 0.5 / (double(g.nx) * double(g.ny) * double(g.nz));
-
-    // tv.m:127-130 stopping tolerances
-    const double atolp = std::sqrt(3.0 * double(N)) * o.atol;
-    const double atold = std::sqrt(double(N)) * o.atol;
-    double rho = o.rho, irho = 1.0 / rho;
-
-    // u = f, l = 0, v = grad(u), nr = sum |v| (tv.m:141-149)
-    init_kernel<T><<<grid, blk, 0, st>>>(w.v, w.l, f, g, gx, gy, gz, w.part);
-    QSM_HIP(hipGetLastError());
-    double nr;
-    if ((rc = read_totals<1>(w.part, nblk, w.tot, &nr, st))) return rc;
-
-    if (o.verbose)
-        std::printf("\niter\t\t||r||\t\teps_pri\t\t||s||\t\teps_dual\n");
-    int flag = 0, ii = 0;
-    bool stop = false;
-    const CVec3<T> cv{w.v.x, w.v.y, w.v.z}, cl{w.l.x, w.l.y, w.l.z}, cd{w.dv.x, w.dv.y, w.dv.z};
-    for (ii = 1; ii <= o.maxit && !stop; ++ii) {
-        // u-subproblem (tv.m:196-203)
-        rhs_kernel<T><<<grid, blk, 0, st>>>(w.r, cv, cl, g, T(rho), ax, ay, az);
-        QSM_HIP(hipGetLastError());
-        op.rho = rho;
-        if ((rc = real_conv<T, true>(w.r, w.u, sz, M, off, w.W, op, st))) return rc;
-        // v-subproblem, multipliers, primal sums (tv.m:205-256)
-        if (o.tv == 1)
-            shrink_kernel<T, false><<<grid, blk, 0, st>>>(w.v, w.l, w.dv, w.u, g, gx, gy, gz,
-                                                         T(o.alpha), T(1.0 - o.alpha), T(rho),
-                                                         T(irho), w.part);
-        else
-            shrink_kernel<T, true><<<grid, blk, 0, st>>>(w.v, w.l, w.dv, w.u, g, gx, gy, gz,
-                                                        T(o.alpha), T(1.0 - o.alpha), T(rho),
-                                                        T(irho), w.part);
-        QSM_HIP(hipGetLastError());
-        double p3[3];
-        if ((rc = read_totals<3>(w.part, nblk, w.tot, p3, st))) return rc;
-        dual_kernel<T><<<grid, blk, 0, st>>>(cd, cl, g, T(rho), ax, ay, az, w.part);
-        QSM_HIP(hipGetLastError());
-        double p2[2];
-        if ((rc = read_totals<2>(w.part, nblk, w.tot, p2, st))) return rc;
-
-        // convergence check (tv.m:258-285)
-        const double nr_ = nr;
-        nr = p3[0];
-        const double ns = std::sqrt(p2[0]);
-        const double na = p3[1], nb = p3[2];
-        const double ep = atolp + o.rtol * std::max(na, nb);
-        const double ed = atold + o.rtol * std::sqrt(p2[1]);
-        if (o.verbose)
-            std::printf("%3d/%d\t\t%.4e\t%.4e\t%.4e\t%.4e\n", ii, o.maxit, nr, ep, ns, ed);
-        if (nr < ep && ns < ed)
-            stop = true;
-        // parameter update (tv.m:287-306)
-        if (!stop) {
-            if (nr > o.stau * ns || (nr > o.rtau * nr_ && flag < 3)) {
-                rho = o.gamma * rho;
-                irho = 1.0 / rho;
-                if (flag > 5)
-                    flag = 0;
-            } else if (nr < ns / o.stau) {
-                rho = rho / (o.gamma * o.gamma);
-                irho = 1.0 / rho;
-                flag = flag + 1;
-            }
-        }
-        if (trace) {
-            double *t = trace + 5 * (ii - 1);
-            t[0] = nr;
-            t[1] = ep;
-            t[2] = ns;
-            t[3] = ed;
-            t[4] = rho;
-        }
-        if (stop)
-            break;
+// Sum of one double per thread of a LT-thread block into out[0] (slot 0 of
+// the block's NPART partials), fixed order.
+__device__ __forceinline__ void block_sum1(double v, double *out)
+{
+    __shared__ double red[LT];
+    const int t = threadIdx.x;
+    red[t] = v;
+    __syncthreads();
+    for (int s = LT / 2; s > 0; s >>= 1) {
+        if (t < s) red[t] += red[t + s];
+        __syncthreads();
     }
-    if (o.verbose)
-        std::printf("\n");
-    if (iters)
-        *iters = std::min(ii, o.maxit);
-    mask_kernel<T><<<grid, blk, 0, st>>>(x, w.u, mask, g);
+    if (t == 0) out[0] = red[0];
+}
+
+// dipoleKernel.m:56-66 on the full grid, cast to T: D(k) for every k of the
+// solver grid (x fastest), and u = (f, 0).
+template <typename T>
+__global__ __launch_bounds__(NT) void lsmr_setup_kernel(T *D, Cx<T> *u, const T *f, G3 g,
+                                                        double dk0, double dk1, double dk2,
+                                                        double b0, double b1, double b2)
+{
+    int i, j, k, l;
+    if (!vox(g, i, j, k, l)) return;
+    D[l] = T(dip(kcoord(i, g.nx, dk0), kcoord(j, g.ny, dk1), kcoord(k, g.nz, dk2), b0, b1, b2));
+    u[l] = Cx<T>{f[l], T(0)};
+}
+
+// lsmr.m: u = A(v) - alpha u (mode 1 with alpha; the first call, alpha = 0,
+// only measures u = b), partial sums of |u|^2.
+template <typename T>
+__global__ __launch_bounds__(LT) void lsmr_u_kernel(Cx<T> *u, const Cx<T> *v, const T *D,
+                                                    size_t n, T alpha, int with_av,
+                                                    double *part)
+{
+    const size_t l = size_t(blockIdx.x) * LT + threadIdx.x;
+    double s = 0.0;
+    if (l < n) {
+        Cx<T> a = u[l];
+        if (with_av) {
+            const T d = D[l];
+            const Cx<T> b = v[l];
+            a = Cx<T>{d * b.re - alpha * a.re, d * b.im - alpha * a.im};
+            u[l] = a;
+        }
+        s = abs2(a);
+    }
+    block_sum1(s, part + NPART * blockIdx.x);
+}
+
+// lsmr.m: u = (1/beta) u; v = A'(u) - beta v (beta_v = 0 on the first
+// call: v = A'(u)), partial sums of |v|^2.
+template <typename T>
+__global__ __launch_bounds__(LT) void lsmr_v_kernel(Cx<T> *u, Cx<T> *v, const T *D, size_t n,
+                                                    T ibeta, T beta_v, double *part)
+{
+    const size_t l = size_t(blockIdx.x) * LT + threadIdx.x;
+    double s = 0.0;
+    if (l < n) {
+        const Cx<T> a = u[l];
+        const Cx<T> un{ibeta * a.re, ibeta * a.im};
+        u[l] = un;
+        const T d = D[l];
+        Cx<T> vn{d * un.re, d * un.im};
+        if (beta_v != T(0)) {
+            const Cx<T> b = v[l];
+            vn = Cx<T>{vn.re - beta_v * b.re, vn.im - beta_v * b.im};
+        }
+        v[l] = vn;
+        s = abs2(vn);
+    }
+    block_sum1(s, part + NPART * blockIdx.x);
+}
+
+// lsmr.m: v = (1/alpha) v (ialpha = 1 when alpha = 0); first call (init):
+// h = v, hbar = 0, x = 0; later: hbar = h - c1 hbar; x = x + c2 hbar;
+// h = v - c3 h.  Partial sums of |x|^2.
+template <typename T>
+__global__ __launch_bounds__(LT) void lsmr_x_kernel(Cx<T> *v, Cx<T> *h, Cx<T> *hb, Cx<T> *x,
+                                                    size_t n, T ialpha, int init, T c1, T c2,
+                                                    T c3, double *part)
+{
+    const size_t l = size_t(blockIdx.x) * LT + threadIdx.x;
+    double s = 0.0;
+    if (l < n) {
+        const Cx<T> a = v[l];
+        const Cx<T> vn{ialpha * a.re, ialpha * a.im};
+        v[l] = vn;
+        if (init) {
+            h[l] = vn;
+            hb[l] = Cx<T>{T(0), T(0)};
+            x[l] = Cx<T>{T(0), T(0)};
+        } else {
+            const Cx<T> hh = h[l], hbo = hb[l], xo = x[l];
+            const Cx<T> hbn{hh.re - c1 * hbo.re, hh.im - c1 * hbo.im};
+            const Cx<T> xn{xo.re + c2 * hbn.re, xo.im + c2 * hbn.im};
+            hb[l] = hbn;
+            x[l] = xn;
+            h[l] = Cx<T>{vn.re - c3 * hh.re, vn.im - c3 * hh.im};
+            s = abs2(xn);
+        }
+    }
+    block_sum1(s, part + NPART * blockIdx.x);
+}
+
+// rts.m:139: x1 = mask .* real(ifft3(x)).
+template <typename T>
+__global__ __launch_bounds__(NT) void real_mask_kernel(T *x1, const Cx<T> *x, const uint8_t *m,
+                                                       G3 g)
+{
+    int i, j, k, l;
+    if (!vox(g, i, j, k, l)) return;
+    x1[l] = T(m[l] ? 1 : 0) * x[l].re;
+}
+
+// Total of slot 0 over nblk blocks' partials: two-level on the device, the
+// RB first-level results summed on the host in a fixed order.
+int lsmr_total(const double *part, int nblk, double *p1, double &out, hipStream_t st)
+{
+    reduce1_kernel<<<RB, dim3(BX, BY), 0, st>>>(part, nblk, p1, nullptr);
+    QSM_HIP(hipGetLastError());
+    double h[RB * NPART];
+    QSM_HIP(hipMemcpyAsync(h, p1, sizeof(h), hipMemcpyDeviceToHost, st));
+    QSM_HIP(hipStreamSynchronize(st));
+    double s = 0.0;
+    for (int b = 0; b < RB; ++b) s += h[NPART * b];
+    out = s;
+    return QSM_OK;
+}
+
+// lsmr(@(x,~) D.*x, fft3(f), 0, eps, eps, [], itnlim) (rts.m:131-139):
+// Golub-Kahan bidiagonalisation with A = A' = diag(D) on the full spectrum
+// (vector updates and norms on the device, the scalar recurrence of
+// lsmr.m on the host in double), then x1 = mask .* real(ifft3(x)).
+template <typename T>
+int rts_step1(const T *f, const uint8_t *mask, const size_t sz[3], const double vsz[3],
+              const double b[3], int itnlim, T *x1, Work<T> &w, hipStream_t st)
+{
+    int rc;
+    const G3 g{int(sz[0]), int(sz[1]), int(sz[2])};
+    const size_t n = sz[0] * sz[1] * sz[2];
+    const dim3 grid = vox_grid(g), blk(BX, BY);
+    const int nb = int((n + LT - 1) / LT);
+    const double eps = std::numeric_limits<T>::epsilon();
+    lsmr_setup_kernel<T><<<grid, blk, 0, st>>>(w.D, w.lu, f, g, 1.0 / (double(g.nx) * vsz[0]),
+                                               1.0 / (double(g.ny) * vsz[1]),
+                                               1.0 / (double(g.nz) * vsz[2]), b[0], b[1], b[2]);
+    QSM_HIP(hipGetLastError());
+    if ((rc = fft3_inplace<T>(w.lu, sz, 0, st))) return rc;  // u = b = fft3(f)
+
+    // u = b; beta = ||u||; u = u/beta; v = A'u; alpha = ||v||; v = v/alpha
+    double beta, alpha, x2;
+    lsmr_u_kernel<T><<<nb, LT, 0, st>>>(w.lu, w.lv, w.D, n, T(0), 0, w.part);
+    QSM_HIP(hipGetLastError());
+    if ((rc = lsmr_total(w.part, nb, w.p1, beta, st))) return rc;
+    beta = std::sqrt(beta);
+    lsmr_v_kernel<T><<<nb, LT, 0, st>>>(w.lu, w.lv, w.D, n, T(beta > 0 ? 1.0 / beta : 1.0), T(0),
+                                        w.part);
+    QSM_HIP(hipGetLastError());
+    if ((rc = lsmr_total(w.part, nb, w.p1, alpha, st))) return rc;
+    alpha = std::sqrt(alpha);
+    lsmr_x_kernel<T><<<nb, LT, 0, st>>>(w.lv, w.lh, w.lhb, w.lx, n,
+                                        T(alpha > 0 ? 1.0 / alpha : 1.0), 1, T(0), T(0), T(0),
+                                        w.part);
+    QSM_HIP(hipGetLastError());
+
+    // scalar state (lsmr.m, lambda = 0, atol = btol = eps, conlim = 1e8)
+    const double atol = eps, btol = eps, ctol = 1.0 / 1e8;
+    double zetabar = alpha * beta, alphabar = alpha, rho = 1, rhobar = 1, cbar = 1, sbar = 0;
+    double betadd = beta, betad = 0, rhodold = 1, tautildeold = 0, thetatilde = 0, zeta = 0;
+    double d = 0, normA2 = alpha * alpha, maxrbar = 0, minrbar = 1e100;
+    const double normb = beta;
+    if (alpha * beta != 0.0) {  // otherwise the exact solution is x = 0
+        for (int itn = 1; itn <= itnlim; ++itn) {
+            lsmr_u_kernel<T><<<nb, LT, 0, st>>>(w.lu, w.lv, w.D, n, T(alpha), 1, w.part);
+            QSM_HIP(hipGetLastError());
+            if ((rc = lsmr_total(w.part, nb, w.p1, beta, st))) return rc;
+            beta = std::sqrt(beta);
+            if (beta > 0) {
+                lsmr_v_kernel<T><<<nb, LT, 0, st>>>(w.lu, w.lv, w.D, n, T(1.0 / beta), T(beta),
+                                                    w.part);
+                QSM_HIP(hipGetLastError());
+                if ((rc = lsmr_total(w.part, nb, w.p1, alpha, st))) return rc;
+                alpha = std::sqrt(alpha);
+            }
+            // plane rotations (lambda = 0: alphahat = alphabar)
+            const double alphahat = alphabar, chat = 1.0, shat = 0.0;
+            const double rhoold = rho;
+            rho = std::hypot(alphahat, beta);
+            const double c = alphahat / rho, s = beta / rho;
+            const double thetanew = s * alpha;
+            alphabar = c * alpha;
+            const double rhobarold = rhobar, zetaold = zeta;
+            const double thetabar = sbar * rho, rhotemp = cbar * rho;
+            rhobar = std::hypot(cbar * rho, thetanew);
+            cbar = cbar * rho / rhobar;
+            sbar = thetanew / rhobar;
+            zeta = cbar * zetabar;
+            zetabar = -sbar * zetabar;
+            // hbar, x, h (with the new v scaled by 1/alpha first)
+            const double c1 = thetabar * rho / (rhoold * rhobarold);
+            const double c2 = zeta / (rho * rhobar);
+            const double c3 = thetanew / rho;
+            lsmr_x_kernel<T><<<nb, LT, 0, st>>>(w.lv, w.lh, w.lhb, w.lx, n,
+                                                T(beta > 0 && alpha > 0 ? 1.0 / alpha : 1.0), 0,
+                                                T(c1), T(c2), T(c3), w.part);
+            QSM_HIP(hipGetLastError());
+            if ((rc = lsmr_total(w.part, nb, w.p1, x2, st))) return rc;
+            // estimate of ||r||
+            const double betaacute = chat * betadd, betacheck = -shat * betadd;
+            const double betahat = c * betaacute;
+            betadd = -s * betaacute;
+            const double thetatildeold = thetatilde;
+            const double rhotildeold = std::hypot(rhodold, thetabar);
+            const double ctildeold = rhodold / rhotildeold, stildeold = thetabar / rhotildeold;
+            thetatilde = stildeold * rhobar;
+            rhodold = ctildeold * rhobar;
+            betad = -stildeold * betad + ctildeold * betahat;
+            tautildeold = (zetaold - thetatildeold * tautildeold) / rhotildeold;
+            const double taud = (zeta - thetatilde * tautildeold) / rhodold;
+            d = d + betacheck * betacheck;
+            const double normr = std::sqrt(d + (betad - taud) * (betad - taud) + betadd * betadd);
+            // estimates of ||A||, cond(A), ||A'r||
+            normA2 = normA2 + beta * beta;
+            const double normA = std::sqrt(normA2);
+            normA2 = normA2 + alpha * alpha;
+            maxrbar = std::max(maxrbar, rhobarold);
+            if (itn > 1) minrbar = std::min(minrbar, rhobarold);
+            const double condA = std::max(maxrbar, rhotemp) / std::min(minrbar, rhotemp);
+            const double normAr = std::fabs(zetabar), normx = std::sqrt(x2);
+            // stopping tests
+            const double test1 = normr / normb, test2 = normAr / (normA * normr);
+            const double test3 = 1.0 / condA;
+            const double t1 = test1 / (1 + normA * normx / normb);
+            const double rtol = btol + atol * normA * normx / normb;
+            int istop = 0;
+            if (itn >= itnlim) istop = 7;
+            if (1 + test3 <= 1) istop = 6;
+            if (1 + test2 <= 1) istop = 5;
+            if (1 + t1 <= 1) istop = 4;
+            if (test3 <= ctol) istop = 3;
+            if (test2 <= atol) istop = 2;
+            if (test1 <= rtol) istop = 1;
+            if (istop > 0) break;
+        }
+        if ((rc = fft3_inplace<T>(w.lx, sz, 1, st))) return rc;
+        real_mask_kernel<T><<<grid, blk, 0, st>>>(x1, w.lx, mask, g);
+    } else {
+        zero_kernel<T><<<grid, blk, 0, st>>>(x1, g);
+    }
     QSM_HIP(hipGetLastError());
     return QSM_OK;
 }
 
-int check_tv(int dtype, const size_t sz[3], const double vsz[3], const double bdir[3],
-             const qsm_tv_opts *o)
+template <typename T>
+int rts_dev_t(const T *f, const uint8_t *mask, const size_t sz[3], const double vsz[3],
+              const double bdir[3], const qsm_rts_opts &o, T *x, void *work, double *trace,
+              int *iters, hipStream_t st)
+{
+    double b[3];
+    int rc = unit_b(bdir, b);
+    if (rc) return rc;
+    Work<T> w;
+    work_layout<T>(sz, &w, static_cast<char *>(work), true);
+    // step 1 into the ADMM's u buffer, the source of step 2
+    if ((rc = rts_step1<T>(f, mask, sz, vsz, b, o.tolls, w.u, w, st))) return rc;
+    const AdmmParams p{o.mu, o.delta, o.tv, o.maxit, o.verbose, o.rtol, o.atol,
+                       o.rho, o.alpha, o.rtau, o.stau, o.gamma};
+    return admm_run<T, true>(w.u, mask, sz, vsz, b, p, x, w, trace, iters, st);
+}
+
+template <typename O>
+int check_admm(int dtype, const size_t sz[3], const double vsz[3], const double bdir[3],
+               const O *o)
 {
     QSM_ARG(dtype == QSM_F64 || dtype == QSM_F32, "bad dtype");
     QSM_ARG(sz && vsz && bdir && o, "null pointer");
     int rc = check_grid(sz, vsz, 1);
     if (rc) return rc;
     QSM_ARG(o->tv == 1 || o->tv == 2, "opts.tv must be 1 (anisotropic) or 2 (isotropic)");
-    QSM_ARG(o->maxit >= 1, "opts.maxit must be >= 1");
+    QSM_ARG(o->maxit >= 1 && o->maxit <= MAX_ITERS, "opts.maxit must be in [1, %d]", MAX_ITERS);
     QSM_ARG(o->rho > 0.0 && o->gamma > 0.0 && o->stau > 0.0, "opts.rho, gamma, stau must be > 0");
     QSM_ARG(std::isfinite(o->mu) && std::isfinite(o->alpha), "opts.mu and alpha must be finite");
+    return QSM_OK;
+}
+
+int check_tv(int dtype, const size_t sz[3], const double vsz[3], const double bdir[3],
+             const qsm_tv_opts *o)
+{
+    return check_admm(dtype, sz, vsz, bdir, o);
+}
+
+int check_rts(int dtype, const size_t sz[3], const double vsz[3], const double bdir[3],
+              const qsm_rts_opts *o)
+{
+    int rc = check_admm(dtype, sz, vsz, bdir, o);
+    if (rc) return rc;
+    QSM_ARG(o->tolls >= 0, "opts.tolls must be >= 0");
+    QSM_ARG(std::isfinite(o->delta), "opts.delta must be finite");
     return QSM_OK;
 }
 
@@ -578,6 +1055,68 @@ int qsm_tv(int dtype, const void *f, const uint8_t *mask, const size_t sz[3],
     QSM_HIP(hipMemcpy(dm.p, mask, N, hipMemcpyHostToDevice));
     if ((rc = qsm_tv_dev(dtype, df.p, static_cast<const uint8_t *>(dm.p), sz, vsz, bdir, opts,
                          dx.p, dw.p, trace, iters, nullptr)))
+        return rc;
+    QSM_HIP(hipMemcpy(x, dx.p, N * es, hipMemcpyDeviceToHost));
+    return QSM_OK;
+}
+
+void qsm_rts_opts_default(qsm_rts_opts *o)
+{
+    // rts.m:50-102
+    o->tolls = 4;
+    o->mu = 1e5;
+    o->delta = 0.15;
+    o->tv = 1;
+    o->rtol = 1e-1;
+    o->atol = 1e-2;
+    o->maxit = 20;
+    o->verbose = 1;
+    o->rho = 10.0;
+    o->alpha = 1.5;
+    o->rtau = 0.7;
+    o->stau = 1e2;
+    o->gamma = 3.0;
+}
+
+size_t qsm_rts_work_bytes(int dtype, const size_t sz[3])
+{
+    if (!sz) return 0;
+    return dtype == QSM_F32 ? work_layout<float>(sz, nullptr, nullptr, true)
+                            : work_layout<double>(sz, nullptr, nullptr, true);
+}
+
+int qsm_rts_dev(int dtype, const void *f, const uint8_t *mask, const size_t sz[3],
+                const double vsz[3], const double bdir[3], const qsm_rts_opts *opts, void *x,
+                void *work, double *trace, int *iters, void *stream)
+{
+    int rc = check_rts(dtype, sz, vsz, bdir, opts);
+    if (rc) return rc;
+    QSM_ARG(f && mask && x && work, "null pointer");
+    const hipStream_t st = as_stream(stream);
+    return dtype == QSM_F64
+        ? rts_dev_t<double>(static_cast<const double *>(f), mask, sz, vsz, bdir, *opts,
+                            static_cast<double *>(x), work, trace, iters, st)
+        : rts_dev_t<float>(static_cast<const float *>(f), mask, sz, vsz, bdir, *opts,
+                           static_cast<float *>(x), work, trace, iters, st);
+}
+
+int qsm_rts(int dtype, const void *f, const uint8_t *mask, const size_t sz[3],
+            const double vsz[3], const double bdir[3], const qsm_rts_opts *opts, void *x,
+            double *trace, int *iters)
+{
+    int rc = check_rts(dtype, sz, vsz, bdir, opts);
+    if (rc) return rc;
+    QSM_ARG(f && mask && x, "null pointer");
+    if ((rc = ensure_device())) return rc;
+    const size_t N = sz[0] * sz[1] * sz[2], es = dtype_size(dtype);
+    DevBuf df, dm, dx, dw;
+    if ((rc = df.alloc(N * es)) || (rc = dm.alloc(N)) || (rc = dx.alloc(N * es))
+        || (rc = dw.alloc(qsm_rts_work_bytes(dtype, sz))))
+        return rc;
+    QSM_HIP(hipMemcpy(df.p, f, N * es, hipMemcpyHostToDevice));
+    QSM_HIP(hipMemcpy(dm.p, mask, N, hipMemcpyHostToDevice));
+    if ((rc = qsm_rts_dev(dtype, df.p, static_cast<const uint8_t *>(dm.p), sz, vsz, bdir, opts,
+                          dx.p, dw.p, trace, iters, nullptr)))
         return rc;
     QSM_HIP(hipMemcpy(x, dx.p, N * es, hipMemcpyDeviceToHost));
     return QSM_OK;

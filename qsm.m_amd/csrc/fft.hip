@@ -138,34 +138,6 @@ template int make_axis<double>(AxisPlan &, size_t, const char *);
 
 namespace {
 
-template <typename T>
-int fft3_dev_t(void *data, const size_t Msz[3], int inverse, hipStream_t st)
-{
-    AxisPlan ap[3];
-    int rc;
-    for (int a = 0; a < 3; ++a)
-        if ((rc = make_axis<T>(ap[a], Msz[a], "FFT length"))) return rc;
-    if ((rc = allow_lds<&pass_kernel<T>>())) return rc;
-    const Vol v{int(Msz[0]), int(Msz[1]), int(Msz[2])};
-    const T sgn = inverse ? T(1) : T(-1);
-    // fft3.m:5 and ifft3.m:5 apply axis 3, then 2, then 1
-    for (int a = 2; a >= 0; --a) {
-        const Plan &pl = ap[a].pl;
-        if (pl.M == 1) continue;
-        const Pen pn = pencils(pl.M, sizeof(Cx<T>));
-        const int P = 1 << pn.logP;
-        const size_t lds = lds_bytes(pn, pl.M, sizeof(Cx<T>));
-        int nblk;
-        if (a == 0) nblk = (v.M1 * v.M2 + P - 1) / P;
-        else nblk = ((v.M0 + P - 1) / P) * (a == 1 ? v.M2 : v.M1);
-        const T scale = inverse ? T(1.0 / double(pl.M)) : T(1);
-        pass_kernel<T><<<nblk, FFT_THREADS, lds, st>>>(
-            static_cast<Cx<T> *>(data), v, a, pn, ap[a].dpl, tw_of<T>(ap[a]), sgn, scale);
-        QSM_HIP(hipGetLastError());
-    }
-    return QSM_OK;
-}
-
 // Multiplier of the dipole convolution at half-spectrum sample (x, y, z):
 // Ds(k) * scale.  The reference keeps real(ifftn(D .* fftn(x))); for a real
 // x that is the inverse transform of fftn(x) .* Ds with Ds(k) = (D(k) +
@@ -256,8 +228,9 @@ int qsm_fft3_dev(int dtype, void *data, const size_t M[3], int inverse, void *st
 {
     QSM_ARG(data && M, "null pointer");
     QSM_ARG(dtype == QSM_F64 || dtype == QSM_F32, "bad dtype");
-    return dtype == QSM_F64 ? fft3_dev_t<double>(data, M, inverse, as_stream(stream))
-                            : fft3_dev_t<float>(data, M, inverse, as_stream(stream));
+    return dtype == QSM_F64
+        ? fft3_inplace<double>(static_cast<Cx<double> *>(data), M, inverse, as_stream(stream))
+        : fft3_inplace<float>(static_cast<Cx<float> *>(data), M, inverse, as_stream(stream));
 }
 
 size_t qsm_nextfastfft(size_t n) { return n == 0 ? 1 : next7(n); }

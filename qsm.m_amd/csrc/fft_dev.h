@@ -468,8 +468,11 @@ struct RealGeom {
 // scaling) into the half-spectrum volume (H, M1, M2).
 template <typename T>
 __global__ __launch_bounds__(FFT_THREADS) void rx_fwd(
-    Cx<T> *data, const T *in, RealGeom g, Pen pn, const Plan *pl, const Cx<T> *tw)
+    Cx<T> *data, const T *in, RealGeom g, Pen pn, const Plan *pl, const Cx<T> *tw,
+    const int *skip)
 {
+    if (skip && *skip)
+        return;
     extern __shared__ unsigned char smem[];
     Cx<T> *A = reinterpret_cast<Cx<T> *>(smem);
     const int P = 1 << pn.logP, PP = pn.PP;
@@ -517,8 +520,10 @@ __global__ __launch_bounds__(FFT_THREADS) void rx_fwd(
 template <typename T>
 __global__ __launch_bounds__(FFT_THREADS) void cy_pass(
     Cx<T> *data, Vol v, int k0, Pen pn, const Plan *pl, const Cx<T> *tw, T sgn,
-    int ylo, int yhi, int slo, int shi)
+    int ylo, int yhi, int slo, int shi, const int *skip)
 {
+    if (skip && *skip)
+        return;
     extern __shared__ unsigned char smem[];
     Cx<T> *A = reinterpret_cast<Cx<T> *>(smem);
     const int P = 1 << pn.logP, PP = pn.PP;
@@ -554,8 +559,11 @@ __global__ __launch_bounds__(FFT_THREADS) void cy_pass(
 // plane of the spectrum is stored.
 template <typename T, bool ROUND, typename Op>
 __global__ __launch_bounds__(FFT_THREADS) void z_pass(
-    Cx<T> *data, Vol v, Pen pn, const Plan *pl, const Cx<T> *tw, int zlo, int zhi, Op op)
+    Cx<T> *data, Vol v, Pen pn, const Plan *pl, const Cx<T> *tw, int zlo, int zhi, Op op,
+    const int *skip)
 {
+    if (skip && *skip)
+        return;
     extern __shared__ unsigned char smem[];
     Cx<T> *A = reinterpret_cast<Cx<T> *>(smem);
     const int P = 1 << pn.logP, PP = pn.PP;
@@ -601,8 +609,11 @@ __global__ __launch_bounds__(FFT_THREADS) void z_pass(
 // into out (nx, ny, nz).
 template <typename T>
 __global__ __launch_bounds__(FFT_THREADS) void rx_inv(
-    const Cx<T> *data, T *out, RealGeom g, Pen pn, const Plan *pl, const Cx<T> *tw)
+    const Cx<T> *data, T *out, RealGeom g, Pen pn, const Plan *pl, const Cx<T> *tw,
+    const int *skip)
 {
+    if (skip && *skip)
+        return;
     extern __shared__ unsigned char smem[];
     Cx<T> *A = reinterpret_cast<Cx<T> *>(smem);
     const int P = 1 << pn.logP, PP = pn.PP;
@@ -706,10 +717,11 @@ int make_axis(AxisPlan &ap, size_t M, const char *what);
 // result is transformed back without the 1/(M0 M1 M2)); work holds
 // (M0/2 + 1) M1 M2 complex values and needs no clearing.  With
 // ROUND = false the passes stop after op: work then holds op(2 FFT3(pad(in)))
-// on every plane, and out is not touched.
+// on every plane, and out is not touched.  skip (device flag, may be NULL):
+// the passes do nothing when it is set at their turn on the stream.
 template <typename T, bool ROUND, typename Op>
 int real_conv(const T *in, T *out, const size_t sz[3], const size_t M[3], const size_t off[3],
-              Cx<T> *work, Op op, hipStream_t st)
+              Cx<T> *work, Op op, hipStream_t st, const int *skip = nullptr)
 {
     int rc;
     AxisPlan ap[3];
@@ -729,35 +741,65 @@ int real_conv(const T *in, T *out, const size_t sz[3], const size_t M[3], const 
         const Pen pn = pencils(M0, cs);
         const int P2 = 2 << pn.logP;
         rx_fwd<T><<<(ny * nz + P2 - 1) / P2, FFT_THREADS, lds_bytes(pn, M0, cs), st>>>(
-            work, in, g, pn, ap[0].dpl, tw_of<T>(ap[0]));
+            work, in, g, pn, ap[0].dpl, tw_of<T>(ap[0]), skip);
         QSM_HIP(hipGetLastError());
     }
     const Pen pny = pencils(M1, cs);
     const int Py = 1 << pny.logP;
     {   // y forward on data planes
         cy_pass<T><<<((H + Py - 1) / Py) * nz, FFT_THREADS, lds_bytes(pny, M1, cs), st>>>(
-            work, v, g.o2, pny, ap[1].dpl, tw_of<T>(ap[1]), T(-1), g.o1, g.o1 + ny, 0, M1);
+            work, v, g.o2, pny, ap[1].dpl, tw_of<T>(ap[1]), T(-1), g.o1, g.o1 + ny, 0, M1, skip);
         QSM_HIP(hipGetLastError());
     }
     {   // z forward, op (, z inverse)
         const Pen pn = pencils(M2, cs);
         const int P = 1 << pn.logP;
         z_pass<T, ROUND, Op><<<((H + P - 1) / P) * M1, FFT_THREADS, lds_bytes(pn, M2, cs), st>>>(
-            work, v, pn, ap[2].dpl, tw_of<T>(ap[2]), g.o2, g.o2 + nz, op);
+            work, v, pn, ap[2].dpl, tw_of<T>(ap[2]), g.o2, g.o2 + nz, op, skip);
         QSM_HIP(hipGetLastError());
     }
     if (!ROUND)
         return QSM_OK;
     {   // y inverse on output planes
         cy_pass<T><<<((H + Py - 1) / Py) * nz, FFT_THREADS, lds_bytes(pny, M1, cs), st>>>(
-            work, v, g.o2, pny, ap[1].dpl, tw_of<T>(ap[1]), T(1), 0, M1, g.o1, g.o1 + ny);
+            work, v, g.o2, pny, ap[1].dpl, tw_of<T>(ap[1]), T(1), 0, M1, g.o1, g.o1 + ny, skip);
         QSM_HIP(hipGetLastError());
     }
     {   // x inverse on output rows, two per transform, cropped
         const Pen pn = pencils(M0, cs);
         const int P2 = 2 << pn.logP;
         rx_inv<T><<<(ny * nz + P2 - 1) / P2, FFT_THREADS, lds_bytes(pn, M0, cs), st>>>(
-            work, out, g, pn, ap[0].dpl, tw_of<T>(ap[0]));
+            work, out, g, pn, ap[0].dpl, tw_of<T>(ap[0]), skip);
+        QSM_HIP(hipGetLastError());
+    }
+    return QSM_OK;
+}
+
+// In-place complex 3D transform of data (M[0] x M[1] x M[2], x fastest):
+// fft3.m:5 (inverse = 0) or ifft3.m:5 (inverse = 1, 1/M per axis), axis 3
+// then 2 then 1 as fftn.  Asynchronous on the stream.
+template <typename T>
+int fft3_inplace(Cx<T> *data, const size_t Msz[3], int inverse, hipStream_t st)
+{
+    AxisPlan ap[3];
+    int rc;
+    for (int a = 0; a < 3; ++a)
+        if ((rc = make_axis<T>(ap[a], Msz[a], "FFT length"))) return rc;
+    if ((rc = allow_lds<&pass_kernel<T>>())) return rc;
+    const Vol v{int(Msz[0]), int(Msz[1]), int(Msz[2])};
+    const T sgn = inverse ? T(1) : T(-1);
+    for (int a = 2; a >= 0; --a) {
+        const Plan &pl = ap[a].pl;
+        if (pl.M == 1) continue;
+        const Pen pn = pencils(pl.M, sizeof(Cx<T>));
+        const int P = 1 << pn.logP;
+        const size_t lds = lds_bytes(pn, pl.M, sizeof(Cx<T>));
+        int nblk;
+        if (a == 0) nblk = (v.M1 * v.M2 + P - 1) / P;
+        else nblk = ((v.M0 + P - 1) / P) * (a == 1 ? v.M2 : v.M1);
+        const T scale = inverse ? T(1.0 / double(pl.M)) : T(1);
+        pass_kernel<T><<<nblk, FFT_THREADS, lds, st>>>(data, v, a, pn, ap[a].dpl, tw_of<T>(ap[a]),
+                                                       sgn, scale);
         QSM_HIP(hipGetLastError());
     }
     return QSM_OK;

@@ -89,3 +89,11 @@ def test_tv_argument_checks_without_gpu(bad, msg):
     with pytest.raises(QsmError, match=msg) as e:
         gtv.tv(f, np.ones(f.shape), (1, 1, 1), None, dict(bad, verbose=0))
     assert e.value.code == _lib.QSM_EARG
+
+
+def test_rts_opts_defaults_are_rts_m():
+    from qsm_amd import rts as grts
+
+    o = grts.make_opts(None)
+    assert (o.tolls, o.mu, o.delta, o.tv, o.rtol, o.atol, o.maxit) == (4, 1e5, 0.15, 1, 1e-1, 1e-2, 20)
+    assert (o.verbose, o.rho, o.alpha, o.rtau, o.stau, o.gamma) == (1, 10.0, 1.5, 0.7, 1e2, 3.0)

@@ -1,4 +1,4 @@
-"""Pins of the tv.m restatement (oracle/tv_admm.py) by what the mathematics
+"""Pins of the tv.m / rts.m restatement (oracle/tv_admm.py) by what the mathematics
 fixes, independent of the oracle's own construction:
 
 * the u-subproblem (tv.m:196-203: an FFT-diagonalised solve) equals the
@@ -12,7 +12,10 @@ fixes, independent of the oracle's own construction:
   generates in-kernel;
 * soft thresholding (tv.m:226-242) on hand-worked cases;
 * dipoleKernel.m special values: D(0) = 0, -2/3 along B0, 1/3 across;
-* a zero field gives x = 0 and stops at the first iteration.
+* a zero field gives x = 0 and stops at the first iteration;
+* rts.m: the LSMR of step 1 (third_party/lsmr/lsmr.m) against scipy's
+  implementation of the same published algorithm, and step 2's
+  x-subproblem against its normal equations with M = mu [|D| > delta].
 """
 import numpy as np
 import pytest
@@ -44,7 +47,7 @@ def test_u_step_is_the_normal_equation_solution():
     lm = [np.asfortranarray(rng.standard_normal(sz)) for _ in range(3)]
     # oracle step
     F = (mu * np.conj(D)) * tva.fft3(f, np.float64)
-    iA = tva.inv_a(mu * np.abs(D) ** 2, tva.laplace_symbol(sz, vsz), rho, np.float64)
+    iA = tva.inv_a(mu * np.abs(D) ** 2, tva.laplace_symbol(sz, vsz), rho)
     u = tva.u_step(F, iA, v, lm, rho, vsz, np.float64)
     # dense: x fastest (Fortran vec) -> kron(z, y, x)
     W = np.kron(_dft(nz), np.kron(_dft(ny), _dft(nx)))
@@ -100,3 +103,50 @@ def test_zero_field_stops_at_once():
     sz = (8, 7, 6)
     x, tr = tva.tv(np.zeros(sz), np.ones(sz), (1, 1, 1))
     assert np.all(x == 0) and len(tr) == 1 and tr[0][0] == 0.0 and tr[0][2] == 0.0
+
+
+def test_lsmr_matches_scipy():
+    """The LSMR restatement (third_party/lsmr/lsmr.m) against scipy's
+    implementation of the same published algorithm (Fong and Saunders),
+    diagonal complex operator, a few iterations and to convergence."""
+    from scipy.sparse.linalg import LinearOperator
+    from scipy.sparse.linalg import lsmr as sp_lsmr
+
+    rng = np.random.default_rng(5)
+    n = 300
+    d = rng.uniform(-2.0 / 3.0, 1.0 / 3.0, n)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    A = LinearOperator((n, n), matvec=lambda x: d * x, rmatvec=lambda x: d * x, dtype=complex)
+    for it in (1, 2, 4, 12):
+        x, itn, _ = tva.lsmr(d, b, 1e-14, 1e-14, it)
+        want = sp_lsmr(A, b, damp=0.0, atol=1e-14, btol=1e-14, conlim=1e8, maxiter=it)[0]
+        assert itn == it
+        np.testing.assert_allclose(x, want, rtol=1e-10, atol=1e-12 * np.abs(want).max())
+
+
+def test_rts_step2_u_step_is_the_normal_equation_solution():
+    """rts.m's x-subproblem: (M + rho G'G) x = M x1 + G'(rho y - p) with
+    M = mu [|D| > delta] as a Fourier multiplier (B0 along an axis)."""
+    rng = np.random.default_rng(8)
+    sz, vsz = (4, 5, 3), (1.0, 0.9, 1.2)
+    nx, ny, nz = sz
+    mu, rho, delta = 1e5, 10.0, 0.15
+    D = tva.dipole_kernel_k(sz, vsz, (0, 0, 1), np.float64)
+    x1 = np.asfortranarray(rng.standard_normal(sz))
+    y = [np.asfortranarray(rng.standard_normal(sz)) for _ in range(3)]
+    p = [np.asfortranarray(rng.standard_normal(sz)) for _ in range(3)]
+    Mk = mu * (np.abs(D) > delta).astype(np.float64)
+    F = Mk * tva.fft3(x1, np.float64)
+    iA = tva.inv_a(Mk, tva.laplace_symbol(sz, vsz), rho)
+    x = tva.u_step(F, iA, y, p, rho, vsz, np.float64)
+    W = np.kron(_dft(nz), np.kron(_dft(ny), _dft(nx)))
+    vec = lambda a: a.reshape(-1, order="F")
+    Mop = np.real(np.linalg.solve(W, np.diag(vec(Mk)) @ W))
+    Ix, Iy, Iz = np.eye(nx), np.eye(ny), np.eye(nz)
+    G = np.vstack([np.kron(Iz, np.kron(Iy, _fdiff(nx, vsz[0]))),
+                   np.kron(Iz, np.kron(_fdiff(ny, vsz[1]), Ix)),
+                   np.kron(_fdiff(nz, vsz[2]), np.kron(Iy, Ix))])
+    A = Mop + rho * G.T @ G
+    rhs = Mop @ vec(x1) + G.T @ np.concatenate([rho * vec(y[c]) - vec(p[c]) for c in range(3)])
+    want = np.linalg.lstsq(A, rhs, rcond=1e-12)[0]
+    np.testing.assert_allclose(vec(x), want, rtol=0, atol=1e-9 * np.abs(want).max())
