@@ -345,69 +345,74 @@ def dipole_leg(shape=(256, 256, 176), reps=10):
     return out
 
 
-def tv_leg(cfg, field, mask, iters=20, cpu_iters=2):
-    """tv.m (TV inversion by ADMM) at the C3 size: the phantom's local field
-    and mask, both precisions, `iters` iterations (tv.m's default maxit)
-    with the stopping tolerances at 0 so that every run does them all; timed
-    with HIP events on its stream after one warm-up solve.  The host reads
-    five residual sums back once per iteration (tv.m decides stopping and
-    the penalty update there).
+def admm_leg(cfg, field, mask, iters=20, cpu_iters=2):
+    """tv.m (TV inversion by ADMM) and rts.m (LSMR in k-space, then the same
+    ADMM) at the C3 size: the phantom's local field and mask, both
+    precisions, `iters` ADMM iterations (the default maxit) with the
+    stopping tolerances at 0 so that every run does them all; one warm-up
+    solve, then one solve timed with HIP events on its stream (the loop is
+    decided on the device: one synchronisation per solve).
 
-    hbm_bytes per iteration: right-hand side 7 N s (v, l read at -1
-    neighbours, r written), the half-spectrum solve 2 N s + 9 S (S = H ny
-    nz c, the spectrum; passes: x 1+1, y 2, z 3 with F, y 2, x 1+1), the
-    v/l update 16 N s (u, v, l read; v, l, dv written), the dual norms 6 N
-    s.  cpu: the oracle (numpy, one thread for its FFTs) on cpu_iters
-    iterations of the same input."""
+    hbm_bytes per ADMM iteration: right-hand side 7 N s (v, l read with -1
+    neighbours, r written), the half-spectrum solve 2 N s + 9 S (S = H ny nz
+    c, the spectrum; passes: x 1+1, y 2, z 3 with F, y 2, x 1+1), the v/l
+    update 16 N s (u, v, l read; v, l, dv written), the dual norms 6 N s.
+    rts adds its step 1 once (4 LSMR iterations and two full complex 3D
+    FFTs).  cpu: the oracle (numpy restatement, one thread for its FFTs) on
+    cpu_iters ADMM iterations of the same input, setup included."""
     import torch
+    from qsm_amd import rts as grts
     from qsm_amd import tv as gtv
 
     shape = field.shape
     N = int(np.prod(shape))
     vsz, bdir = cfg["spec"].vsz, cfg["spec"].bdir
     opts = {"maxit": iters, "rtol": 0.0, "atol": 0.0, "verbose": 0}
-    out = {"shape": list(shape), "iterations": iters, "opts": "tv.m defaults, rtol = atol = 0"}
+    out = {"shape": list(shape), "iterations": iters,
+           "opts": "tv.m / rts.m defaults, rtol = atol = 0"}
     st = torch.cuda.Stream()
-    for dt, name in ((torch.float32, "f32"), (torch.float64, "f64")):
-        s = 4 if dt == torch.float32 else 8
-        with torch.cuda.stream(st):
-            f = torch.from_numpy(np.ascontiguousarray(field.reshape(-1, order="F"))).to("cuda", dt)
-            f = f.view(shape[::-1]).permute(2, 1, 0)
-            m = torch.from_numpy(np.ascontiguousarray(mask.reshape(-1, order="F"))).to("cuda")
-            m = m.view(shape[::-1]).permute(2, 1, 0)
-            x = torch.empty_like(f)
-            work = torch.empty(gtv.work_bytes(dt, shape), dtype=torch.uint8, device="cuda")
-        st.synchronize()
-        gtv.tv_dev(f, m, vsz, bdir, opts, x, work=work, stream=st)  # warm-up (plans)
-        st.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(st)
-        it = gtv.tv_dev(f, m, vsz, bdir, opts, x, work=work, stream=st)
-        e1.record(st)
-        e1.synchronize()
-        t = e0.elapsed_time(e1) / 1e3
-        H = shape[0] // 2 + 1
-        S = H * shape[1] * shape[2] * 2 * s
-        hbm = (7 + 2 + 16 + 6) * N * s + 9 * S
-        per = t / it
-        out[name] = {"ms_total": t * 1e3, "ms_per_iter": per * 1e3,
-                     "voxel_iters_per_s": N * it / t, "hbm_bytes_per_iter": hbm,
-                     "hbm_gbs": hbm / per / 1e9}
-        del f, m, x, work
-    torch.cuda.empty_cache()
-    if cpu_iters > 0:
-        from oracle import tv_admm as tva
+    for kind, mod, run in (("tv", gtv, gtv.tv_dev), ("rts", grts, grts.rts_dev)):
+        res = {}
+        for dt, name in ((torch.float32, "f32"), (torch.float64, "f64")):
+            s = 4 if dt == torch.float32 else 8
+            with torch.cuda.stream(st):
+                f = torch.from_numpy(np.ascontiguousarray(field.reshape(-1, order="F")))
+                f = f.to("cuda", dt).view(shape[::-1]).permute(2, 1, 0)
+                m = torch.from_numpy(np.ascontiguousarray(mask.reshape(-1, order="F"))).to("cuda")
+                m = m.view(shape[::-1]).permute(2, 1, 0)
+                x = torch.empty_like(f)
+                work = torch.empty(mod.work_bytes(dt, shape), dtype=torch.uint8, device="cuda")
+            st.synchronize()
+            run(f, m, vsz, bdir, opts, x, work=work, stream=st)  # warm-up (plans)
+            st.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            it = run(f, m, vsz, bdir, opts, x, work=work, stream=st)
+            e1.record(st)
+            e1.synchronize()
+            t = e0.elapsed_time(e1) / 1e3
+            H = shape[0] // 2 + 1
+            S = H * shape[1] * shape[2] * 2 * s
+            hbm = (7 + 2 + 16 + 6) * N * s + 9 * S
+            res[name] = {"ms_total": t * 1e3, "ms_per_iter": t / it * 1e3,
+                         "voxel_iters_per_s": N * it / t, "hbm_bytes_per_iter": hbm,
+                         "hbm_gbs_per_iter": hbm / (t / it) / 1e9}
+            del f, m, x, work
+        torch.cuda.empty_cache()
+        if cpu_iters > 0:
+            from oracle import tv_admm as tva
 
-        o = dict(opts, maxit=cpu_iters)
-        t = time.perf_counter()
-        tva.tv(field.astype(np.float32), mask, vsz, bdir, o)
-        c = time.perf_counter() - t
-        out["cpu"] = {"value": N * cpu_iters / c, "unit": "voxel-iters/s", "cores": 1,
-                      "kind": "port",
-                      "sample": (f"oracle tv (numpy restatement of tv.m, float32, numpy FFT), "
-                                 f"{shape[0]}x{shape[1]}x{shape[2]}, {cpu_iters} iterations "
-                                 f"incl. setup, {c:.2f}s")}
+            o = dict(opts, maxit=cpu_iters)
+            t = time.perf_counter()
+            (tva.tv if kind == "tv" else tva.rts)(field.astype(np.float32), mask, vsz, bdir, o)
+            c = time.perf_counter() - t
+            res["cpu"] = {"value": N * cpu_iters / c, "unit": "voxel-iters/s", "cores": 1,
+                          "kind": "port",
+                          "sample": (f"oracle {kind} (numpy restatement of {kind}.m, float32, "
+                                     f"numpy FFT), {shape[0]}x{shape[1]}x{shape[2]}, "
+                                     f"{cpu_iters} ADMM iterations incl. setup, {c:.2f}s")}
+        out[kind] = res
     return out
 
 
@@ -425,8 +430,8 @@ def main(argv=None) -> int:
     ap.add_argument("--serve-requests", type=int, default=40,
                     help="requests of the C5 serving leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
-    ap.add_argument("--tv-iters", type=int, default=20,
-                    help="ADMM iterations of the tv.m leg (0: skip)")
+    ap.add_argument("--admm-iters", type=int, default=20,
+                    help="ADMM iterations of the tv.m / rts.m leg (0: skip)")
     ap.add_argument("--no-dipole", dest="dipole", action="store_false",
                     help="skip the finite-difference and FFT dipole convolution legs")
     a = ap.parse_args(argv)
@@ -524,10 +529,11 @@ def main(argv=None) -> int:
         out["dipole_conv"] = dipole_leg()
         log(f"[bench] dipole conv f32 {out['dipole_conv']['f32']['ms']:.3f} ms, "
             f"f64 {out['dipole_conv']['f64']['ms']:.3f} ms")
-    if a.tv_iters > 0:
-        out["tv_admm"] = tv_leg(cfg, field, mask, a.tv_iters, 2 if a.cpu_iters > 0 else 0)
-        log(f"[bench] tv (ADMM) f32 {out['tv_admm']['f32']['ms_per_iter']:.3f} ms/iter, "
-            f"f64 {out['tv_admm']['f64']['ms_per_iter']:.3f} ms/iter")
+    if a.admm_iters > 0:
+        out["admm"] = admm_leg(cfg, field, mask, a.admm_iters, 2 if a.cpu_iters > 0 else 0)
+        for k in ("tv", "rts"):
+            log(f"[bench] {k}.m (ADMM) f32 {out['admm'][k]['f32']['ms_total']:.2f} ms, "
+                f"f64 {out['admm'][k]['f64']['ms_total']:.2f} ms for {a.admm_iters} iterations")
     if a.batch_iters > 0:
         out["batch"] = batch_leg(a.batch_iters)
         log(f"[bench] C4 batch: {out['batch']['voxel_iters_per_s']:.3e} voxel-iters/s")
