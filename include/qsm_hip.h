@@ -415,6 +415,42 @@ int qsm_rts(int dtype, const void *f, const uint8_t *mask, const size_t sz[3],
             void *x, double *trace, int *iters);
 
 /* ------------------------------------------------------------------------
+ * Background field removal (src/bgremove/sharp.m, vsharp.m)
+ * ------------------------------------------------------------------------ */
+
+/* Bytes of device work space qsm_sharp_dev / qsm_vsharp_dev need for a
+ * cropped volume szc, voxel size vsz and the largest radius rmax. */
+size_t qsm_sharp_work_bytes(int dtype, const size_t szc[3], const double vsz[3], double rmax);
+
+/* Replaces sharp.m:47-84 after its crop: f and mask already cropped to the
+ * mask's non-empty slices (cropIndices.m), szc that crop, nechoes echoes
+ * stored one after another.  padfastfft(., ceil(r/min(vsz))) is implicit;
+ * writes the local field flc (same layout) and the eroded mask mask_out.
+ * Device pointers, asynchronous on the stream. */
+int qsm_sharp_dev(int dtype, const void *fc, const uint8_t *mc, const size_t szc[3],
+                  size_t nechoes, const double vsz[3], double r, double thr, void *flc,
+                  uint8_t *mask_out, void *work, void *stream);
+
+/* Replaces vsharp.m:47-153 after its crop, as qsm_sharp_dev, with the radii
+ * rs[0..nrs-1] (any order, at least two; vsharp.m's default is
+ * 9:-2*max(vsz):2*max(vsz)).  mask_out is the reference's returned mask:
+ * the eroded mask of the first radius of the last pair (second smallest
+ * radius for an even count) or of the last radius (odd count). */
+int qsm_vsharp_dev(int dtype, const void *fc, const uint8_t *mc, const size_t szc[3],
+                   size_t nechoes, const double vsz[3], const double *rs, size_t nrs,
+                   double thr, void *flc, uint8_t *mask_out, void *work, void *stream);
+
+/* Host-pointer forms of the whole functions sharp.m / vsharp.m: crop with
+ * cropIndices.m, run, uncrop into zeros (fl: sz x nechoes, mask_out: sz;
+ * may be NULL).  Synchronous. */
+int qsm_sharp(int dtype, const void *f, const uint8_t *mask, const size_t sz[3],
+              size_t nechoes, const double vsz[3], double r, double thr, void *fl,
+              uint8_t *mask_out);
+int qsm_vsharp(int dtype, const void *f, const uint8_t *mask, const size_t sz[3],
+               size_t nechoes, const double vsz[3], const double *rs, size_t nrs,
+               double thr, void *fl, uint8_t *mask_out);
+
+/* ------------------------------------------------------------------------
  * Device utilities
  * ------------------------------------------------------------------------ */
 

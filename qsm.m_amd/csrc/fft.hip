@@ -187,39 +187,6 @@ int dipole_conv_dev_t(const void *chi, const size_t sz[3], const double h[3],
                               static_cast<Cx<T> *>(work), op, st);
 }
 
-// padfastfft.m:34-51 ('both'): padded length and leading pad of one axis.
-size_t next7(size_t n)
-{
-    // smallest 2,3,5,7-smooth number >= n (nextfastfft.m semantics for
-    // n <= 2^20; above that the value is halved, looked up and doubled)
-    size_t red = 0;
-    double v = double(n);
-    while (v > 1048576.0) { v /= 2.0; ++red; }
-    size_t best = 0;
-    for (size_t p2 = 1; p2 <= (1u << 21); p2 *= 2)
-        for (size_t p3 = p2; p3 <= (1u << 21); p3 *= 3)
-            for (size_t p5 = p3; p5 <= (1u << 21); p5 *= 5)
-                for (size_t p7 = p5; p7 <= (1u << 21); p7 *= 7)
-                    if (double(p7) >= v && (best == 0 || p7 < best))
-                        best = p7;
-    return best << red;
-}
-
-void pad_plan(const size_t sz[3], const size_t pad[3], size_t M[3], size_t off[3])
-{
-    for (int a = 0; a < 3; ++a) {
-        if (pad[a] == 0) {
-            M[a] = sz[a];
-            off[a] = 0;
-            continue;
-        }
-        M[a] = next7(sz[a] + 2 * pad[a]);
-        const size_t tot = M[a] - sz[a];
-        const size_t oddpre = (tot & 1) & (sz[a] & 1);
-        off[a] = tot / 2 + oddpre;
-    }
-}
-
 }  // namespace
 
 extern "C" {

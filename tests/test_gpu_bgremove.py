@@ -1,0 +1,85 @@
+"""SHARP / V-SHARP (src/bgremove/sharp.m, vsharp.m) on the GPU against the
+oracle's restatement (oracle/bgremove.py): eroded masks equal, local fields
+to 1e-10 (FP64) / 1e-4 (FP32) relative -- the FFTs make bitwise parity
+impossible (MATLAB's FFT is closed).  Includes anisotropic voxels, odd and
+even crop sizes, a mask with an empty interior slice (cropIndices skips
+it), 4-D echoes, an odd number of V-SHARP radii (the returned-mask rule)
+and the harmonic-background property on the device."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from oracle import bgremove as obg
+from qsm_amd import QsmError
+from qsm_amd import bgremove as gbg
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(shape, seed, hole=False):
+    rng = np.random.default_rng(seed)
+    c = [(n - 1) / 2 for n in shape]
+    g = np.meshgrid(*(np.arange(n) - c[a] for a, n in enumerate(shape)), indexing="ij")
+    r = min(shape) / 2 - 3
+    mask = (g[0] ** 2 + (1.1 * g[1]) ** 2 + (0.9 * g[2]) ** 2) <= r * r
+    if hole:
+        mask[:, :, shape[2] // 2] = False
+    f = np.zeros(shape)
+    for _ in range(6):
+        p = rng.uniform(-r / 2, r / 2, 3)
+        f += rng.uniform(-0.1, 0.1) * np.exp(-sum((g[a] - p[a]) ** 2 for a in range(3)) / 6.0)
+    f += 0.01 * (g[0] ** 2 - g[1] ** 2) + 0.02 * g[2] + 0.004 * rng.standard_normal(shape)
+    return np.asfortranarray(f), np.asfortranarray(mask)
+
+
+CASES = [((40, 36, 30), (1.0, 1.0, 1.0), 5, False), ((37, 33, 28), (0.9, 1.0, 1.2), 4, False),
+         ((34, 34, 32), (1.0, 1.0, 1.0), 3, True)]
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_sharp_matches_oracle(cuda, case, dt):
+    shape, vsz, r, hole = CASES[case]
+    f, m = _case(shape, case, hole)
+    f = f.astype(dt)
+    wf, wm = obg.sharp(f, m, vsz, r)
+    gf, gm = gbg.sharp(f, m, vsz, r)
+    np.testing.assert_array_equal(gm, wm)
+    assert gf.dtype == f.dtype
+    assert rel_l2(gf, wf) < (1e-10 if dt == np.float64 else 1e-4)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("rs", [None, [6, 4, 2]])
+def test_vsharp_matches_oracle(cuda, dt, rs):
+    shape, vsz, _, _ = CASES[0]
+    f, m = _case(shape, 9)
+    f = f.astype(dt)
+    if rs is None:
+        rs_used = gbg.default_radii(vsz)
+        assert list(rs_used) == [9.0, 7.0, 5.0, 3.0]
+    wf, wm = obg.vsharp(f, m, vsz, rs)
+    gf, gm = gbg.vsharp(f, m, vsz, rs)
+    np.testing.assert_array_equal(gm, wm)
+    assert rel_l2(gf, wf) < (1e-10 if dt == np.float64 else 1e-4)
+
+
+def test_sharp_multi_echo_and_harmonic_background(cuda):
+    shape, vsz, r, _ = CASES[1]
+    f, m = _case(shape, 2)
+    c = [(n - 1) / 2 for n in shape]
+    g = np.meshgrid(*(np.arange(n) - c[a] for a, n in enumerate(shape)), indexing="ij")
+    b = 0.02 * (g[0] ** 2 - g[2] ** 2) + 0.01 * g[0] * g[1] + 0.03 * g[2]  # harmonic
+    f4 = np.stack([f, f + b], axis=3)
+    fl, mo = gbg.sharp(f4, m, (1.0, 1.0, 1.0), r)
+    for t in range(2):
+        one, _ = gbg.sharp(f4[..., t], m, (1.0, 1.0, 1.0), r)
+        np.testing.assert_array_equal(fl[..., t], one)
+    # the harmonic part is removed: the two echoes agree to rounding
+    assert np.abs(fl[..., 1] - fl[..., 0]).max() < 1e-12 * np.abs(b).max()
+
+
+def test_vsharp_rejects_a_single_radius(cuda):
+    f, m = _case((20, 20, 20), 1)
+    with pytest.raises(QsmError, match="two radii"):
+        gbg.vsharp(f, m, (1, 1, 1), [3])
