@@ -416,6 +416,68 @@ def admm_leg(cfg, field, mask, iters=20, cpu_iters=2):
     return out
 
 
+def bgremove_leg(cfg, field, mask, reps=3, cpu=True):
+    """SHARP (sharp.m, r = 9) and V-SHARP (vsharp.m, radii 9:-2:2) at the C3
+    size on the phantom's field and mask, both precisions.  device_ms: the
+    device form on the cropped volume resident in HBM, HIP events on its
+    stream; host_ms: the drop-in's form (host arrays in and out: crop,
+    copies, device convolutions, uncrop) timed on the host; both over
+    `reps` calls after a warm-up.  cpu: the oracle (numpy restatement, one
+    thread for its FFTs) on the same input, FP64."""
+    import torch
+    from qsm_amd import bgremove as gbg
+
+    vsz = cfg["spec"].vsz
+    rs = gbg.default_radii(vsz)
+    out = {"shape": list(field.shape), "mask_voxels": int(np.count_nonzero(mask)),
+           "vsharp_radii": [float(r) for r in rs]}
+    ix, iy, iz = (np.nonzero(np.asarray(mask).any(axis=a))[0] for a in ((1, 2), (0, 2), (0, 1)))
+    crop = np.ix_(ix, iy, iz)
+    st = torch.cuda.Stream()
+    for name, fn, dev in (("sharp", lambda f: gbg.sharp(f, mask, vsz, 9),
+                           lambda fc, mc, o, mo, w: gbg.sharp_dev(fc, mc, vsz, 9, 0.05, o, mo, w, st)),
+                          ("vsharp", lambda f: gbg.vsharp(f, mask, vsz),
+                           lambda fc, mc, o, mo, w: gbg.vsharp_dev(fc, mc, vsz, rs, 0.05, o, mo, w, st))):
+        res = {}
+        for dt, tag in ((np.float32, "f32"), (np.float64, "f64")):
+            f = np.asfortranarray(field.astype(dt))
+            fn(f)
+            t = time.perf_counter()
+            for _ in range(reps):
+                fn(f)
+            host_ms = (time.perf_counter() - t) / reps * 1e3
+            fcrop = np.asfortranarray(f[crop])
+            shp = fcrop.shape
+            with torch.cuda.stream(st):
+                fc = torch.from_numpy(np.ascontiguousarray(fcrop.reshape(-1, order="F"))).to("cuda")
+                fc = fc.view(shp[::-1]).permute(2, 1, 0)
+                mcp = np.ascontiguousarray(np.asarray(mask)[crop].astype(np.uint8).reshape(-1, order="F"))
+                mc = torch.from_numpy(mcp).to("cuda").view(shp[::-1]).permute(2, 1, 0)
+                o = torch.empty_like(fc)
+                mo = torch.empty_like(mc)
+            w = dev(fc, mc, o, mo, None)
+            st.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            for _ in range(reps):
+                dev(fc, mc, o, mo, w)
+            e1.record(st)
+            e1.synchronize()
+            res[tag] = {"device_ms": e0.elapsed_time(e1) / reps, "host_ms": host_ms}
+            del fc, mc, o, mo, w
+        torch.cuda.empty_cache()
+        if cpu:
+            from oracle import bgremove as obg
+
+            t = time.perf_counter()
+            (obg.sharp(field, mask, vsz, 9) if name == "sharp" else obg.vsharp(field, mask, vsz))
+            res["cpu"] = {"ms": (time.perf_counter() - t) * 1e3, "cores": 1, "kind": "port",
+                          "sample": f"oracle {name} (numpy restatement), float64, same input"}
+        out[name] = res
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[1])
     ap.add_argument("--gpus", type=int, default=1)
@@ -430,6 +492,8 @@ def main(argv=None) -> int:
     ap.add_argument("--serve-requests", type=int, default=40,
                     help="requests of the C5 serving leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--no-bgremove", dest="bgremove", action="store_false",
+                    help="skip the SHARP / V-SHARP leg")
     ap.add_argument("--admm-iters", type=int, default=20,
                     help="ADMM iterations of the tv.m / rts.m leg (0: skip)")
     ap.add_argument("--no-dipole", dest="dipole", action="store_false",
@@ -534,6 +598,10 @@ def main(argv=None) -> int:
         for k in ("tv", "rts"):
             log(f"[bench] {k}.m (ADMM) f32 {out['admm'][k]['f32']['ms_total']:.2f} ms, "
                 f"f64 {out['admm'][k]['f64']['ms_total']:.2f} ms for {a.admm_iters} iterations")
+    if a.bgremove:
+        out["bgremove"] = bgremove_leg(cfg, field, mask, cpu=a.cpu_iters > 0)
+        log(f"[bench] sharp f32 {out['bgremove']['sharp']['f32']['device_ms']:.2f} ms on device, "
+            f"vsharp f32 {out['bgremove']['vsharp']['f32']['device_ms']:.2f} ms")
     if a.batch_iters > 0:
         out["batch"] = batch_leg(a.batch_iters)
         log(f"[bench] C4 batch: {out['batch']['voxel_iters_per_s']:.3e} voxel-iters/s")

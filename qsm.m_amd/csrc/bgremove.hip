@@ -357,16 +357,24 @@ int host_form(int dtype, const void *f, const uint8_t *mask, const size_t sz[3],
     const size_t n = szc[0] * szc[1] * szc[2];
     std::vector<char> fc(n * nech * es), flc(n * nech * es);
     std::vector<uint8_t> mc(n), mo(n);
+    // runs of consecutive x indices are copied whole
+    std::vector<std::pair<size_t, size_t>> runs;  // (first x, length)
+    for (size_t a = 0; a < ix[0].size();) {
+        size_t b = a + 1;
+        while (b < ix[0].size() && ix[0][b] == ix[0][b - 1] + 1) ++b;
+        runs.emplace_back(ix[0][a], b - a);
+        a = b;
+    }
     size_t c = 0;
     for (size_t k : ix[2])
         for (size_t j : ix[1])
-            for (size_t i : ix[0]) {
-                const size_t l = i + sz[0] * (j + sz[1] * k);
-                mc[c] = mask[l] ? 1 : 0;
+            for (const auto &rn : runs) {
+                const size_t l = rn.first + sz[0] * (j + sz[1] * k);
+                for (size_t t = 0; t < rn.second; ++t) mc[c + t] = mask[l + t] ? 1 : 0;
                 for (size_t e = 0; e < nech; ++e)
                     std::memcpy(&fc[(e * n + c) * es],
-                                static_cast<const char *>(f) + (e * N + l) * es, es);
-                ++c;
+                                static_cast<const char *>(f) + (e * N + l) * es, rn.second * es);
+                c += rn.second;
             }
     int rc;
     if ((rc = ensure_device())) return rc;
@@ -385,13 +393,13 @@ int host_form(int dtype, const void *f, const uint8_t *mask, const size_t sz[3],
     c = 0;
     for (size_t k : ix[2])
         for (size_t j : ix[1])
-            for (size_t i : ix[0]) {
-                const size_t l = i + sz[0] * (j + sz[1] * k);
-                if (mask_out) mask_out[l] = mo[c];
+            for (const auto &rn : runs) {
+                const size_t l = rn.first + sz[0] * (j + sz[1] * k);
+                if (mask_out) std::memcpy(mask_out + l, &mo[c], rn.second);
                 for (size_t e = 0; e < nech; ++e)
                     std::memcpy(static_cast<char *>(fl) + (e * N + l) * es,
-                                &flc[(e * n + c) * es], es);
-                ++c;
+                                &flc[(e * n + c) * es], rn.second * es);
+                c += rn.second;
             }
     return QSM_OK;
 }

@@ -74,3 +74,43 @@ def vsharp(f, mask, vsz, rs=None, thr=0.05):
                            h3(vsz), rs.ctypes.data_as(_dp), rs.size, float(thr), ptr(fl),
                            ptr(mo)))
     return fl, mo.astype(bool)
+
+
+def work_bytes(dtype, szc, vsz, rmax) -> int:
+    code = _lib.F32 if str(dtype).endswith("float32") else _lib.F64
+    return int(lib().qsm_sharp_work_bytes(code, (_sz * 3)(*szc), h3(vsz), float(rmax)))
+
+
+def _dev_args(f, mask, work, rmax, vsz, stream):
+    import torch
+
+    shape = tuple(int(s) for s in f.shape[:3])
+    ne = 1 if f.dim() == 3 else int(f.shape[3])
+    code = _lib.F64 if f.dtype == torch.float64 else _lib.F32
+    nb = work_bytes(f.dtype, shape, vsz, rmax)
+    if work is None:
+        work = torch.empty(nb, dtype=torch.uint8, device=f.device)
+    elif work.numel() * work.element_size() < nb:
+        raise ValueError(f"work space too small: {nb} bytes needed")
+    sp = stream.cuda_stream if stream is not None else None
+    return shape, ne, code, work, sp
+
+
+def sharp_dev(fc, mc, vsz, r, thr, flc, mask_out, work=None, stream=None):
+    """Device form on an already cropped volume (torch tensors, Fortran-order
+    views; echoes along a fourth axis): qsm_sharp_dev, asynchronous."""
+    shape, ne, code, work, sp = _dev_args(fc, mc, work, r, vsz, stream)
+    check(lib().qsm_sharp_dev(code, fc.data_ptr(), mc.data_ptr(), (_sz * 3)(*shape), ne, h3(vsz),
+                              float(r), float(thr), flc.data_ptr(), mask_out.data_ptr(),
+                              work.data_ptr(), sp))
+    return work
+
+
+def vsharp_dev(fc, mc, vsz, rs, thr, flc, mask_out, work=None, stream=None):
+    """Device form of vsharp on an already cropped volume: qsm_vsharp_dev."""
+    rs = np.ascontiguousarray(np.asarray(rs, dtype=np.float64).reshape(-1))
+    shape, ne, code, work, sp = _dev_args(fc, mc, work, float(rs.max()), vsz, stream)
+    check(lib().qsm_vsharp_dev(code, fc.data_ptr(), mc.data_ptr(), (_sz * 3)(*shape), ne,
+                               h3(vsz), rs.ctypes.data_as(_dp), rs.size, float(thr),
+                               flc.data_ptr(), mask_out.data_ptr(), work.data_ptr(), sp))
+    return work
