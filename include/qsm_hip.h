@@ -451,6 +451,60 @@ int qsm_vsharp(int dtype, const void *f, const uint8_t *mask, const size_t sz[3]
                double thr, void *fl, uint8_t *mask_out);
 
 /* ------------------------------------------------------------------------
+ * Masked Poisson solver and Laplacian unwrapping
+ * (src/utils/poisson_solver/mgpcg.m, src/unwrap/unwrapLaplacian.m)
+ * ------------------------------------------------------------------------ */
+
+/* mgopts of mgpcg.m:11-52 (nlevels <= 0: max(1, floor(log2(min(sz))) - 3)). */
+typedef struct qsm_mg_opts {
+    double tol;     /* multigrid tolerance; < 0: fixed cycles (-1) */
+    int maxit;      /* V-cycles per preconditioner application (1) */
+    int mu;         /* 1 V-cycle, 2 W-cycle (1) */
+    int npre;       /* pre-smoothing sweeps (1) */
+    int npost;      /* post-smoothing sweeps (1) */
+    int nboundary;  /* extra sweeps on the mask's boundary band (2) */
+    int nlevels;    /* grid levels (0: the default) */
+} qsm_mg_opts;
+
+void qsm_mg_opts_default(qsm_mg_opts *o);
+
+/* Bytes of device work space of qsm_mgpcg_dev / qsm_unwrap_laplacian_dev. */
+size_t qsm_mgpcg_work_bytes(int dtype, const size_t sz[3], int nlevels);
+
+/* Replaces mgpcg.m:1-58 / mgpcg_mex.c: v solves -Lap v = f on the mask
+ * (7-point Laplacian of voxel size vsz, v = 0 off the mask and outside the
+ * volume) by CG preconditioned with mgopts->maxit multigrid V-cycles,
+ * stopping at ||r|| <= tol ||f|| or maxit iterations.  The smoother is
+ * red-black Gauss-Seidel (the reference's GAUSS_SEIDEL_RED_BLACK variant)
+ * and mgopts->tol is not used (fixed cycles, the reference's default).
+ * iters: iterations done; resnorm (may be NULL): maxit doubles, ||r|| per
+ * iteration.  Device pointers; synchronises the stream (the CG scalars are
+ * read on the host). */
+int qsm_mgpcg_dev(int dtype, const void *f, const uint8_t *mask, const size_t sz[3],
+                  const double vsz[3], double tol, int maxit, const qsm_mg_opts *mgopts,
+                  void *v, void *work, int *iters, double *resnorm, void *stream);
+
+/* Host-pointer form of qsm_mgpcg_dev (synchronous). */
+int qsm_mgpcg(int dtype, const void *f, const uint8_t *mask, const size_t sz[3],
+              const double vsz[3], double tol, int maxit, const qsm_mg_opts *mgopts,
+              void *v, int *iters, double *resnorm);
+
+/* Replaces unwrapLaplacian.m:78-98 (unwrap_) for one echo of a volume
+ * already cropped to the mask's box +1 voxel: d2u = lapw(u, vsz), uphas =
+ * mgpcg(-mask .* d2u, mask, vsz, 1e-8, ceil(sqrt(numel(mask))), mgopts
+ * with maxit = npre = npost = nboundary = 2).  Device pointers; work:
+ * qsm_mgpcg_work_bytes(dtype, sz, 0). */
+int qsm_unwrap_laplacian_dev(int dtype, const void *u, const uint8_t *mask,
+                             const size_t sz[3], const double vsz[3], void *uphas, void *d2u,
+                             void *work, int *iters, void *stream);
+
+/* Host-pointer form of unwrapLaplacian.m:1-76: crop to the mask's box
+ * grown by one voxel, per echo qsm_unwrap_laplacian_dev, uncrop into zeros
+ * (uphas and, if not NULL, d2phas: sz x nechoes).  Synchronous. */
+int qsm_unwrap_laplacian(int dtype, const void *phas, const uint8_t *mask, const size_t sz[3],
+                         size_t nechoes, const double vsz[3], void *uphas, void *d2phas);
+
+/* ------------------------------------------------------------------------
  * Device utilities
  * ------------------------------------------------------------------------ */
 

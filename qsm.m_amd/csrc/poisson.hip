@@ -1,0 +1,832 @@
+// poisson.hip -- multigrid-preconditioned CG for the masked Poisson problem
+// (src/utils/poisson_solver/mgpcg.m, mex/mgpcg_mex.c) and Laplacian phase
+// unwrapping (src/unwrap/unwrapLaplacian.m) on gfx950.
+//
+// The system (lapmg_mex.c): on the voxels of the mask G, -Lap_h v = f with
+// the 7-point Laplacian, v = 0 off the mask and on a one-voxel border added
+// around the volume (mgpcg_mex.c: mx_pad_boundary).  PCG (mgpcg_mex.c:
+// mx_pcg) stops when ||r|| <= tol ||f|| or after maxit iterations; its
+// preconditioner is mgopts.maxit V-cycles (mx_mg, mx_cycle): pre-smoothing
+// on G and on the boundary band B (2^l nboundary sweeps on level l), the
+// residual, full-weighting restriction onto the coarse mask, the coarse
+// cycle, trilinear correction, the mirrored post-smoothing; the coarsest
+// level is solved to a relative residual of 1e-8 (double) / 1e-4 (single).
+// The grid hierarchy (the voxel size doubles per level), the masks (a coarse
+// voxel is active iff its 8 fine children are: coarsen_grid_mex.c), the
+// band (boundary_mask_mex.c) and the transfer weights (restrict_mex.c,
+// correct_mex.c) are the reference's.  The smoother is red-black
+// Gauss-Seidel (the reference's GAUSS_SEIDEL_RED_BLACK variant) with the
+// reference's colour order in its reverse sweeps (gauss_seidel_mex.c:
+// 141-163: the first colour there is the one of parity nx + 1).  Two
+// choices differ from the reference and change only the preconditioner,
+// hence how fast PCG converges, not what it converges to: every voxel of a
+// colour is updated (the reference's loops skip the row end of the second
+// colour on rows with j = k mod 2), and the coarsest level is solved by
+// conjugate gradients in one workgroup instead of by rounds of 256
+// Gauss-Seidel sweeps (gauss-seidel needs O(n^2) sweeps on an n^3 coarse
+// grid, CG O(n) iterations).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+
+using namespace qsm;
+
+namespace {
+
+constexpr int BX = 64, BY = 4, NT = BX * BY;
+constexpr int MAX_LEVELS = 16;
+constexpr int COARSE_T = 1024;    // threads of the coarsest-level solve
+constexpr int COARSE_MAXIT = 4096;
+
+struct Dim {
+    int nx, ny, nz;
+    __host__ __device__ size_t n() const { return size_t(nx) * ny * nz; }
+};
+
+dim3 grid3(Dim d) { return dim3((d.nx + BX - 1) / BX, (d.ny + BY - 1) / BY, d.nz); }
+
+__device__ __forceinline__ bool vox(Dim d, int &i, int &j, int &k, size_t &l)
+{
+    i = blockIdx.x * BX + threadIdx.x;
+    j = blockIdx.y * BY + threadIdx.y;
+    k = blockIdx.z;
+    l = size_t(i) + size_t(d.nx) * (size_t(j) + size_t(d.ny) * k);
+    return i < d.nx && j < d.ny;
+}
+
+__device__ __forceinline__ bool interior(Dim d, int i, int j, int k)
+{
+    return i > 0 && j > 0 && k > 0 && i < d.nx - 1 && j < d.ny - 1 && k < d.nz - 1;
+}
+
+// Laplacian coefficients as lapmg_mex.c forms them (in double): 1 / h^2 per
+// axis, h the voxel size times 2^level (mgpcg_mex.c: mx_init).
+struct Lap {
+    double hx, hy, hz;
+};
+
+Lap lap_at(const double vsz[3], int level)
+{
+    const double s = std::ldexp(1.0, level);
+    const double h[3] = {vsz[0] * s, vsz[1] * s, vsz[2] * s};
+    return Lap{1.0 / (h[0] * h[0]), 1.0 / (h[1] * h[1]), 1.0 / (h[2] * h[2])};
+}
+
+// ---- padding -------------------------------------------------------------
+
+// out (nx+2, ny+2, nz+2) = in with a zero border; G likewise from a mask.
+template <typename T>
+__global__ __launch_bounds__(NT) void pad_kernel(T *out, const T *in, uint8_t *G,
+                                                 const uint8_t *m, Dim dp, T sgn_mask)
+{
+    int i, j, k;
+    size_t l;
+    if (!vox(dp, i, j, k, l)) return;
+    const bool in_box = interior(dp, i, j, k);
+    const size_t li = size_t(i - 1) + size_t(dp.nx - 2) * (size_t(j - 1) + size_t(dp.ny - 2) * (k - 1));
+    if (out) {
+        T v = T(0);
+        // sgn_mask != 0: out = sgn_mask * mask .* in (unwrapLaplacian's -m.*d2u)
+        if (in_box) v = (sgn_mask != T(0)) ? sgn_mask * (T(m[li] ? 1 : 0) * in[li]) : in[li];
+        out[l] = v;
+    }
+    if (G) G[l] = in_box ? (m[li] ? 1 : 0) : 0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void unpad_kernel(T *out, const T *in, Dim dp)
+{
+    int i, j, k;
+    size_t l;
+    if (!vox(dp, i, j, k, l)) return;
+    if (!interior(dp, i, j, k)) return;
+    out[size_t(i - 1) + size_t(dp.nx - 2) * (size_t(j - 1) + size_t(dp.ny - 2) * (k - 1))] = in[l];
+}
+
+// ---- masks of the hierarchy ----------------------------------------------
+
+// coarsen_grid_mex.c: coarse (i2, j2, k2) is active iff the fine voxel
+// (2 i2 - 1, ...) and its +1 neighbours in x, y, z (those inside the
+// interior) are all active.
+__global__ __launch_bounds__(NT) void coarsen_kernel(uint8_t *G2, const uint8_t *G, Dim d2, Dim d)
+{
+    int i2, j2, k2;
+    size_t l2;
+    if (!vox(d2, i2, j2, k2, l2)) return;
+    uint8_t g = 0;
+    if (interior(d2, i2, j2, k2)) {
+        const int i = 2 * i2 - 1, j = 2 * j2 - 1, k = 2 * k2 - 1;
+        const size_t sx = 1, sy = size_t(d.nx), sz = size_t(d.nx) * d.ny;
+        const size_t l = size_t(i) + sy * (size_t(j) + size_t(d.ny) * k);
+        const bool bi = !(i < d.nx - 2), bj = !(j < d.ny - 2), bk = !(k < d.nz - 2);
+        g = G[l] && (bi || G[l + sx]) && (bj || G[l + sy]) && (bk || G[l + sz]) &&
+            (bi || bj || G[l + sx + sy]) && (bi || bk || G[l + sx + sz]) &&
+            (bj || bk || G[l + sy + sz]) && (bi || bj || bk || G[l + sx + sy + sz]);
+    }
+    G2[l2] = g;
+}
+
+// boundary_mask_mex.c, first two steps: b = G on the interior's outer
+// layer, and on every 2x2x2 block (starting at odd indices) that is not
+// fully active.
+__global__ __launch_bounds__(NT) void band_seed_kernel(uint8_t *b, const uint8_t *G, Dim d)
+{
+    int i, j, k;
+    size_t l;
+    if (!vox(d, i, j, k, l)) return;
+    uint8_t v = 0;
+    if (interior(d, i, j, k)) {
+        if (i == 1 || j == 1 || k == 1 || i == d.nx - 2 || j == d.ny - 2 || k == d.nz - 2)
+            v = G[l];
+        // the block that contains (i, j, k): its corner at odd indices
+        const int ic = i - ((i - 1) & 1), jc = j - ((j - 1) & 1), kc = k - ((k - 1) & 1);
+        const size_t sy = size_t(d.nx), sz = size_t(d.nx) * d.ny;
+        const size_t lc = size_t(ic) + sy * (size_t(jc) + size_t(d.ny) * kc);
+        // the reference's block loop visits corners up to the interior's last
+        // index and reads the +1 neighbours, which may be the zero border
+        bool full = G[lc] && G[lc + 1] && G[lc + sy] && G[lc + sz] && G[lc + 1 + sy] &&
+                    G[lc + 1 + sz] && G[lc + sy + sz] && G[lc + 1 + sy + sz];
+        if (!full) v = G[l];
+    }
+    b[l] = v;
+}
+
+// boundary_mask_mex.c, last step: B = G .* (b or any of its 6 neighbours).
+__global__ __launch_bounds__(NT) void band_kernel(uint8_t *B, const uint8_t *b, const uint8_t *G,
+                                                  Dim d)
+{
+    int i, j, k;
+    size_t l;
+    if (!vox(d, i, j, k, l)) return;
+    uint8_t v = 0;
+    if (interior(d, i, j, k) && G[l]) {
+        const size_t sy = size_t(d.nx), sz = size_t(d.nx) * d.ny;
+        v = b[l - sz] || b[l - sy] || b[l - 1] || b[l] || b[l + 1] || b[l + sy] || b[l + sz];
+    }
+    B[l] = v;
+}
+
+// ---- smoothing, residual, transfers --------------------------------------
+
+// One colour of a red-black Gauss-Seidel sweep (gauss_seidel_mex.c):
+// v = (f + hx (v-x + v+x) + hy (...) + hz (...)) / (2 (hx + hy + hz)) on the
+// voxels of M with (i + j + k) % 2 == colour.
+template <typename T>
+__global__ __launch_bounds__(NT) void gs_kernel(T *v, const T *f, const uint8_t *M, Dim d, Lap c,
+                                                int colour)
+{
+    int i, j, k;
+    size_t l;
+    if (!vox(d, i, j, k, l)) return;
+    if (((i + j + k) & 1) != colour || !interior(d, i, j, k) || !M[l]) return;
+    const size_t sy = size_t(d.nx), sz = size_t(d.nx) * d.ny;
+    const T hh = T(1.0 / (2.0 * (c.hx + c.hy + c.hz)));
+    v[l] = hh * (f[l] + ((T(c.hx) * (v[l - 1] + v[l + 1]) + T(c.hy) * (v[l - sy] + v[l + sy])) +
+                         T(c.hz) * (v[l - sz] + v[l + sz])));
+}
+
+// residual_mex.c: r = f + Lap_h v on G, 0 elsewhere.
+template <typename T>
+__global__ __launch_bounds__(NT) void residual_kernel(T *r, const T *f, const T *v,
+                                                      const uint8_t *G, Dim d, Lap c)
+{
+    int i, j, k;
+    size_t l;
+    if (!vox(d, i, j, k, l)) return;
+    T out = T(0);
+    if (interior(d, i, j, k) && G[l]) {
+        const size_t sy = size_t(d.nx), sz = size_t(d.nx) * d.ny;
+        const T hh = T(-2.0 * (c.hx + c.hy + c.hz));
+        out = f[l] + (((hh * v[l] + T(c.hx) * (v[l - 1] + v[l + 1])) +
+                       T(c.hy) * (v[l - sy] + v[l + sy])) + T(c.hz) * (v[l - sz] + v[l + sz]));
+    }
+    r[l] = out;
+}
+
+// lapmg_mex.c with the sign of A: q = -Lap_h p on G, 0 elsewhere.
+template <typename T>
+__global__ __launch_bounds__(NT) void apply_kernel(T *q, const T *p, const uint8_t *G, Dim d,
+                                                   Lap c)
+{
+    int i, j, k;
+    size_t l;
+    if (!vox(d, i, j, k, l)) return;
+    T out = T(0);
+    if (interior(d, i, j, k) && G[l]) {
+        const size_t sy = size_t(d.nx), sz = size_t(d.nx) * d.ny;
+        const T hh = T(-2.0 * (c.hx + c.hy + c.hz));
+        out = -((((hh * p[l] + T(c.hx) * (p[l - 1] + p[l + 1])) + T(c.hy) * (p[l - sy] + p[l + sy])) +
+                 T(c.hz) * (p[l - sz] + p[l + sz])));
+    }
+    q[l] = out;
+}
+
+// restrict_mex.c: full weighting (1 2 1 / 2 4 2 / 1 2 1 per plane, x2 in
+// the middle plane, / 64) around the fine (2 i2 - 1, ...), onto G2; v2 = 0.
+template <typename T>
+__global__ __launch_bounds__(NT) void restrict_kernel(T *f2, T *v2, const T *r, const uint8_t *G2,
+                                                      Dim d2, Dim d)
+{
+    int i2, j2, k2;
+    size_t l2;
+    if (!vox(d2, i2, j2, k2, l2)) return;
+    v2[l2] = T(0);
+    T out = T(0);
+    if (interior(d2, i2, j2, k2) && G2[l2]) {
+        const int i = 2 * i2 - 1, j = 2 * j2 - 1, k = 2 * k2 - 1;
+        const size_t sy = size_t(d.nx), sz = size_t(d.nx) * d.ny;
+        const size_t l = size_t(i) + sy * (size_t(j) + size_t(d.ny) * k);
+        double s = 0.0;
+        for (int dk = -1; dk <= 1; ++dk)
+            for (int dj = -1; dj <= 1; ++dj)
+                for (int di = -1; di <= 1; ++di) {
+                    const int w = (2 - (di < 0 ? -di : di)) * (2 - (dj < 0 ? -dj : dj)) *
+                                  (2 - (dk < 0 ? -dk : dk));
+                    s += double(w) * double(r[l + di + ptrdiff_t(dj) * ptrdiff_t(sy) +
+                                              ptrdiff_t(dk) * ptrdiff_t(sz)]);
+                }
+        out = T(s / 64.0);
+    }
+    f2[l2] = out;
+}
+
+// correct_mex.c: v += trilinear interpolation of v2 on G (each coarse
+// voxel serves the 2x2x2 fine block at (2 i2 - 1, ...)).
+template <typename T>
+__global__ __launch_bounds__(NT) void correct_kernel(T *v, const T *v2, const uint8_t *G, Dim d,
+                                                     Dim d2)
+{
+    int i2, j2, k2;
+    size_t l2;
+    if (!vox(d2, i2, j2, k2, l2)) return;
+    if (!interior(d2, i2, j2, k2)) return;
+    const size_t sy2 = size_t(d2.nx), sz2 = size_t(d2.nx) * d2.ny;
+    const size_t sy = size_t(d.nx), sz = size_t(d.nx) * d.ny;
+    const int i = 2 * i2 - 1, j = 2 * j2 - 1, k = 2 * k2 - 1;
+    const size_t l = size_t(i) + sy * (size_t(j) + size_t(d.ny) * k);
+    for (int dk = 0; dk <= 1; ++dk)
+        for (int dj = 0; dj <= 1; ++dj)
+            for (int di = 0; di <= 1; ++di) {
+                const int fi = i + di, fj = j + dj, fk = k + dk;
+                if (!interior(d, fi, fj, fk)) continue;
+                const size_t lf = l + di + dj * sy + dk * sz;
+                if (!G[lf]) continue;
+                // mean of the coarse values at the corners this fine voxel
+                // lies between (1, 2, 4 or 8 of them)
+                double s = 0.0;
+                for (int ek = 0; ek <= dk; ++ek)
+                    for (int ej = 0; ej <= dj; ++ej)
+                        for (int ei = 0; ei <= di; ++ei)
+                            s += double(v2[l2 + ei + ej * sy2 + ek * sz2]);
+                const int cnt = (1 + di) * (1 + dj) * (1 + dk);
+                v[lf] += T(s / double(cnt));
+            }
+}
+
+// Sum over the workgroup (COARSE_T threads), the same on every thread,
+// in a fixed order.
+__device__ double block_sum(double x, double *red)
+{
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+    __syncthreads();
+    double s = 0.0;
+    for (int w = 0; w < COARSE_T / 64; ++w) s += red[w];
+    __syncthreads();
+    return s;
+}
+
+// The coarsest level: -Lap v = f on G by conjugate gradients from the
+// current v, in one workgroup, until ||f + Lap v|| < tol ||f|| (the
+// reference's criterion for its coarse sweeps, mgpcg_mex.c: mx_cycle) or
+// maxit iterations.  r, p, q: scratch of the level's size.
+template <typename T>
+__global__ __launch_bounds__(COARSE_T) void coarse_kernel(T *v, const T *f, T *r, T *p, T *q,
+                                                          const uint8_t *G, Dim d, Lap c,
+                                                          double tol, int maxit)
+{
+    __shared__ double red[COARSE_T / 64];
+    const size_t n = d.n();
+    const size_t sy = size_t(d.nx), sz = size_t(d.nx) * d.ny;
+    const T hh = T(-2.0 * (c.hx + c.hy + c.hz));
+    auto active = [&](size_t l) {
+        const int i = int(l % d.nx), j = int((l / d.nx) % d.ny), k = int(l / sz);
+        return interior(d, i, j, k) && G[l];
+    };
+    // -Lap_h x at l (an active voxel)
+    auto negl = [&](const T *x, size_t l) {
+        return -(((hh * x[l] + T(c.hx) * (x[l - 1] + x[l + 1])) + T(c.hy) * (x[l - sy] + x[l + sy])) +
+                 T(c.hz) * (x[l - sz] + x[l + sz]));
+    };
+    double ff = 0.0, rr = 0.0;
+    for (size_t l = threadIdx.x; l < n; l += COARSE_T) {
+        const T fl = f[l];
+        ff += double(fl) * double(fl);
+        const T rl = active(l) ? T(fl - negl(v, l)) : T(0);
+        r[l] = rl;
+        p[l] = T(0);
+        rr += double(rl) * double(rl);
+    }
+    ff = block_sum(ff, red);
+    rr = block_sum(rr, red);
+    const double stop = tol * tol * ff;
+    double rho_prev = 1.0;
+    for (int it = 0; it < maxit && rr > 0.0 && rr >= stop; ++it) {
+        const T beta = T(it == 0 ? 0.0 : rr / rho_prev);
+        for (size_t l = threadIdx.x; l < n; l += COARSE_T) p[l] = r[l] + beta * p[l];
+        __syncthreads();
+        double pq = 0.0;
+        for (size_t l = threadIdx.x; l < n; l += COARSE_T) {
+            const T ql = active(l) ? negl(p, l) : T(0);
+            q[l] = ql;
+            pq += double(p[l]) * double(ql);
+        }
+        pq = block_sum(pq, red);
+        if (!(pq > 0.0)) break;
+        const T alpha = T(rr / pq);
+        rho_prev = rr;
+        double rn = 0.0;
+        for (size_t l = threadIdx.x; l < n; l += COARSE_T) {
+            v[l] = v[l] + alpha * p[l];
+            const T rl = r[l] - alpha * q[l];
+            r[l] = rl;
+            rn += double(rl) * double(rl);
+        }
+        rr = block_sum(rn, red);
+    }
+}
+
+// ---- PCG vector kernels ----------------------------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(NT) void zero_kernel(T *a, size_t n)
+{
+    const size_t l = size_t(blockIdx.x) * NT + threadIdx.x;
+    if (l < n) a[l] = T(0);
+}
+
+// p = q + beta p
+template <typename T>
+__global__ __launch_bounds__(NT) void xpby_kernel(T *p, const T *q, T beta, size_t n)
+{
+    const size_t l = size_t(blockIdx.x) * NT + threadIdx.x;
+    if (l < n) p[l] = beta * p[l] + q[l];
+}
+
+// x += a p; r -= a q
+template <typename T>
+__global__ __launch_bounds__(NT) void update_kernel(T *x, T *r, const T *p, const T *q, T a,
+                                                    size_t n)
+{
+    const size_t l = size_t(blockIdx.x) * NT + threadIdx.x;
+    if (l < n) {
+        x[l] = x[l] + a * p[l];
+        r[l] = r[l] - a * q[l];
+    }
+}
+
+// per-block partial sums of a[l] * b[l] (fixed order)
+template <typename T>
+__global__ __launch_bounds__(NT) void dot_kernel(const T *a, const T *b, size_t n, double *part)
+{
+    __shared__ double red[NT];
+    const size_t l = size_t(blockIdx.x) * NT + threadIdx.x;
+    red[threadIdx.x] = l < n ? double(a[l]) * double(b[l]) : 0.0;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(NT) void sum_kernel(const double *part, int nb, double *out)
+{
+    __shared__ double red[NT];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += NT) s += part[b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = NT / 2; k > 0; k >>= 1) {
+        if (int(threadIdx.x) < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+unsigned g1(size_t n) { return unsigned((n + NT - 1) / NT); }
+
+size_t al(size_t n) { return (n + 255) / 256 * 256; }
+
+// ---- the solver --------------------------------------------------------------
+
+template <typename T>
+struct Level {
+    Dim d;
+    Lap c;
+    T *v, *f, *r;
+    uint8_t *G, *B;
+};
+
+template <typename T>
+struct Solver {
+    int L;
+    Level<T> lv[MAX_LEVELS];
+    T *x, *p;           // PCG iterate and direction (level-0 size)
+    T *cp, *cq;         // coarsest-level CG scratch
+    uint8_t *tmp;       // band seed scratch (level-0 size)
+    double *part, *sum; // reductions
+};
+
+// Level sizes: the padded grid, then ((n - 2 + 1) >> 1) + 2 per level
+// (mgpcg_mex.c: mx_init).
+int level_dims(const size_t sz[3], int L, Dim *d)
+{
+    d[0] = Dim{int(sz[0]) + 2, int(sz[1]) + 2, int(sz[2]) + 2};
+    for (int i = 1; i < L; ++i)
+        d[i] = Dim{((d[i - 1].nx - 2 + 1) >> 1) + 2, ((d[i - 1].ny - 2 + 1) >> 1) + 2,
+                   ((d[i - 1].nz - 2 + 1) >> 1) + 2};
+    return QSM_OK;
+}
+
+template <typename T>
+size_t solver_layout(const size_t sz[3], const double vsz[3], int L, Solver<T> *s, char *base)
+{
+    Dim d[MAX_LEVELS];
+    level_dims(sz, L, d);
+    size_t off = 0;
+    auto take = [&](size_t b) { char *p = base ? base + off : nullptr; off += al(b); return p; };
+    for (int i = 0; i < L; ++i) {
+        const size_t n = d[i].n();
+        char *pv = take(n * sizeof(T)), *pf = take(n * sizeof(T)), *pr = take(n * sizeof(T));
+        char *pG = take(n), *pB = take(n);
+        if (s) s->lv[i] = Level<T>{d[i], lap_at(vsz, i), reinterpret_cast<T *>(pv), reinterpret_cast<T *>(pf),
+                                   reinterpret_cast<T *>(pr), reinterpret_cast<uint8_t *>(pG),
+                                   reinterpret_cast<uint8_t *>(pB)};
+    }
+    const size_t n0 = d[0].n();
+    char *px = take(n0 * sizeof(T)), *pp = take(n0 * sizeof(T)), *pt = take(n0);
+    const size_t nc = d[L - 1].n();
+    char *pcp = take(nc * sizeof(T)), *pcq = take(nc * sizeof(T));
+    char *ppart = take(g1(n0) * sizeof(double)), *psum = take(sizeof(double));
+    if (s) {
+        s->L = L;
+        s->x = reinterpret_cast<T *>(px);
+        s->p = reinterpret_cast<T *>(pp);
+        s->cp = reinterpret_cast<T *>(pcp);
+        s->cq = reinterpret_cast<T *>(pcq);
+        s->tmp = reinterpret_cast<uint8_t *>(pt);
+        s->part = reinterpret_cast<double *>(ppart);
+        s->sum = reinterpret_cast<double *>(psum);
+    }
+    return off;
+}
+
+template <typename T>
+int dot(const T *a, const T *b, size_t n, Solver<T> &s, double &out, hipStream_t st)
+{
+    dot_kernel<T><<<g1(n), NT, 0, st>>>(a, b, n, s.part);
+    QSM_HIP(hipGetLastError());
+    sum_kernel<<<1, NT, 0, st>>>(s.part, int(g1(n)), s.sum);
+    QSM_HIP(hipGetLastError());
+    QSM_HIP(hipMemcpyAsync(&out, s.sum, sizeof(double), hipMemcpyDeviceToHost, st));
+    QSM_HIP(hipStreamSynchronize(st));
+    return QSM_OK;
+}
+
+// n red-black sweeps on mask M.  Forward sweeps take the even colour
+// ((i + j + k) even) first; reverse sweeps start from the volume's far
+// corner, whose colour is that of parity nx + 1 (gauss_seidel_mex.c:141-163).
+template <typename T>
+int smooth(Level<T> &L, const uint8_t *M, int n, bool reverse, hipStream_t st)
+{
+    const dim3 g = grid3(L.d), b(BX, BY);
+    const int first = reverse ? ((L.d.nx + 1) & 1) : 0;
+    for (int s = 0; s < n; ++s)
+        for (int h = 0; h < 2; ++h) {
+            gs_kernel<T><<<g, b, 0, st>>>(L.v, L.f, M, L.d, L.c, first ^ h);
+            QSM_HIP(hipGetLastError());
+        }
+    return QSM_OK;
+}
+
+struct MgParams {
+    double tol;
+    int cycles, mu, npre, npost, nboundary;
+};
+
+// mgpcg_mex.c: mx_cycle on level l.
+template <typename T>
+int cycle(Solver<T> &s, int l, const MgParams &m, hipStream_t st)
+{
+    int rc;
+    Level<T> &L = s.lv[l];
+    const dim3 b(BX, BY);
+    if (l < s.L - 1) {
+        const int bs = (1 << l) * m.nboundary;
+        if ((rc = smooth(L, L.G, m.npre, false, st))) return rc;
+        if ((rc = smooth(L, L.B, bs, false, st))) return rc;
+        residual_kernel<T><<<grid3(L.d), b, 0, st>>>(L.r, L.f, L.v, L.G, L.d, L.c);
+        QSM_HIP(hipGetLastError());
+        Level<T> &C = s.lv[l + 1];
+        restrict_kernel<T><<<grid3(C.d), b, 0, st>>>(C.f, C.v, L.r, C.G, C.d, L.d);
+        QSM_HIP(hipGetLastError());
+        for (int i = 0; i < m.mu; ++i) {
+            if ((rc = cycle(s, l + 1, m, st))) return rc;
+            if (l + 1 == s.L - 1) break;
+        }
+        correct_kernel<T><<<grid3(C.d), b, 0, st>>>(L.v, C.v, L.G, L.d, C.d);
+        QSM_HIP(hipGetLastError());
+        if ((rc = smooth(L, L.B, bs, true, st))) return rc;
+        if ((rc = smooth(L, L.G, m.npost, true, st))) return rc;
+    } else {
+        const double tol = sizeof(T) == 4 ? 1e-4 : 1e-8;
+        coarse_kernel<T><<<1, COARSE_T, 0, st>>>(L.v, L.f, L.r, s.cp, s.cq, L.G, L.d, L.c, tol,
+                                                 COARSE_MAXIT);
+        QSM_HIP(hipGetLastError());
+    }
+    return QSM_OK;
+}
+
+// Solves -Lap v = f on the mask (sz, unpadded, device), v out (sz).  src:
+// f itself, or with unwrap != 0 the wrapped Laplacian d2u whose right-hand
+// side is -m .* d2u (unwrapLaplacian.m:84).
+template <typename T>
+int mgpcg_t(const T *src, bool unwrap, const uint8_t *mask, const size_t sz[3],
+            const double vsz[3], double tol, int maxit, const MgParams &m, int nlevels, T *v,
+            void *work, int *iters, double *resnorm, hipStream_t st)
+{
+    int rc;
+    Solver<T> s;
+    solver_layout<T>(sz, vsz, nlevels, &s, static_cast<char *>(work));
+    Level<T> &L0 = s.lv[0];
+    const size_t n0 = L0.d.n();
+    const dim3 b(BX, BY);
+    // level 0: r (held in f) = padded right-hand side, G; masks of the hierarchy
+    pad_kernel<T><<<grid3(L0.d), b, 0, st>>>(L0.f, src, L0.G, mask, L0.d, unwrap ? T(-1) : T(0));
+    QSM_HIP(hipGetLastError());
+    for (int l = 0; l < s.L; ++l) {
+        Level<T> &L = s.lv[l];
+        if (l > 0) {
+            coarsen_kernel<<<grid3(L.d), b, 0, st>>>(L.G, s.lv[l - 1].G, L.d, s.lv[l - 1].d);
+            QSM_HIP(hipGetLastError());
+        }
+        band_seed_kernel<<<grid3(L.d), b, 0, st>>>(s.tmp, L.G, L.d);
+        band_kernel<<<grid3(L.d), b, 0, st>>>(L.B, s.tmp, L.G, L.d);
+        QSM_HIP(hipGetLastError());
+    }
+    // PCG (mgpcg_mex.c: mx_pcg), x = 0 so r = f
+    zero_kernel<T><<<g1(n0), NT, 0, st>>>(s.x, n0);
+    zero_kernel<T><<<g1(n0), NT, 0, st>>>(s.p, n0);
+    QSM_HIP(hipGetLastError());
+    T *r = L0.f, *q = L0.v;
+    double res;
+    if ((rc = dot(r, r, n0, s, res, st))) return rc;
+    res = std::sqrt(res);
+    const double reltol = tol * res;
+    double rho = 1.0, prevrho;
+    int it = 0;
+    for (; it < maxit; ++it) {
+        if (res <= reltol) break;
+        prevrho = rho;
+        // q = M \ r: m.cycles V-cycles from q = 0 (mgpcg_mex.c: mx_mg; with
+        // mgopts.tol > 0 they stop once ||r - A q|| < mgopts.tol ||r||)
+        zero_kernel<T><<<g1(n0), NT, 0, st>>>(q, n0);
+        QSM_HIP(hipGetLastError());
+        const double mgtol = m.tol > 0.0 ? m.tol * res : 0.0;
+        for (int k = 0; k < m.cycles; ++k) {
+            if (m.tol > 0.0) {
+                residual_kernel<T><<<grid3(L0.d), b, 0, st>>>(L0.r, L0.f, L0.v, L0.G, L0.d, L0.c);
+                QSM_HIP(hipGetLastError());
+                double rn;
+                if ((rc = dot(L0.r, L0.r, n0, s, rn, st))) return rc;
+                if (std::sqrt(rn) < mgtol) break;
+            }
+            if ((rc = cycle(s, 0, m, st))) return rc;
+        }
+        if ((rc = dot(r, q, n0, s, rho, st))) return rc;
+        // breakdown: the residual is exactly zero on the mask (the rule
+        // above never stops a right-hand side that is nonzero off the mask,
+        // since ||r|| counts the whole array), so no direction is left
+        if (!(rho > 0.0)) break;
+        const double beta = rho / prevrho;
+        xpby_kernel<T><<<g1(n0), NT, 0, st>>>(s.p, q, T(beta), n0);
+        apply_kernel<T><<<grid3(L0.d), b, 0, st>>>(q, s.p, L0.G, L0.d, L0.c);
+        QSM_HIP(hipGetLastError());
+        double pq;
+        if ((rc = dot(s.p, q, n0, s, pq, st))) return rc;
+        if (!(pq > 0.0)) break;
+        const double a = rho / pq;
+        update_kernel<T><<<g1(n0), NT, 0, st>>>(s.x, r, s.p, q, T(a), n0);
+        QSM_HIP(hipGetLastError());
+        if ((rc = dot(r, r, n0, s, res, st))) return rc;
+        res = std::sqrt(res);
+        if (resnorm) resnorm[it] = res;
+    }
+    if (iters) *iters = it;
+    unpad_kernel<T><<<grid3(L0.d), b, 0, st>>>(v, s.x, L0.d);
+    QSM_HIP(hipGetLastError());
+    return QSM_OK;
+}
+
+int default_levels(const size_t sz[3])
+{
+    // mgpcg.m:51: max(1, floor(log2(min(size(f)))) - 3)
+    const size_t mn = std::min(sz[0], std::min(sz[1], sz[2]));
+    return std::max(1, int(std::floor(std::log2(double(mn)))) - 3);
+}
+
+int check_mg(int dtype, const size_t sz[3], const double vsz[3], const qsm_mg_opts *o, int &L)
+{
+    QSM_ARG(dtype == QSM_F64 || dtype == QSM_F32, "bad dtype");
+    QSM_ARG(sz && vsz && o, "null pointer");
+    int rc = check_grid(sz, vsz, 1);
+    if (rc) return rc;
+    L = o->nlevels > 0 ? o->nlevels : default_levels(sz);
+    QSM_ARG(L >= 1 && L < MAX_LEVELS, "mgopts.nlevels %d out of range", L);
+    Dim d[MAX_LEVELS];
+    level_dims(sz, L, d);
+    QSM_ARG(d[L - 1].nx > 2 && d[L - 1].ny > 2 && d[L - 1].nz > 2,
+            "mgopts.nlevels %d too many for this size", L);
+    QSM_ARG(o->maxit >= 1 && o->mu >= 1 && o->npre >= 0 && o->npost >= 0 && o->nboundary >= 0 &&
+            std::isfinite(o->tol), "invalid mgopts");
+    return QSM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void qsm_mg_opts_default(qsm_mg_opts *o)
+{
+    // mgpcg.m:11-52
+    o->tol = -1.0;
+    o->maxit = 1;
+    o->mu = 1;
+    o->npre = 1;
+    o->npost = 1;
+    o->nboundary = 2;
+    o->nlevels = 0;
+}
+
+size_t qsm_mgpcg_work_bytes(int dtype, const size_t sz[3], int nlevels)
+{
+    if (!sz) return 0;
+    const int L = nlevels > 0 ? nlevels : default_levels(sz);
+    if (L < 1 || L >= MAX_LEVELS) return 0;
+    const double one[3] = {1.0, 1.0, 1.0};
+    return dtype == QSM_F32 ? solver_layout<float>(sz, one, L, nullptr, nullptr)
+                            : solver_layout<double>(sz, one, L, nullptr, nullptr);
+}
+
+int qsm_mgpcg_dev(int dtype, const void *f, const uint8_t *mask, const size_t sz[3],
+                  const double vsz[3], double tol, int maxit, const qsm_mg_opts *mgopts, void *v,
+                  void *work, int *iters, double *resnorm, void *stream)
+{
+    int L, rc = check_mg(dtype, sz, vsz, mgopts, L);
+    if (rc) return rc;
+    QSM_ARG(f && mask && v && work, "null pointer");
+    QSM_ARG(maxit >= 1 && std::isfinite(tol), "tol must be finite and maxit >= 1");
+    const MgParams m{mgopts->tol, mgopts->maxit, mgopts->mu, mgopts->npre, mgopts->npost,
+                     mgopts->nboundary};
+    const hipStream_t st = as_stream(stream);
+    return dtype == QSM_F64
+        ? mgpcg_t<double>(static_cast<const double *>(f), false, mask, sz, vsz, tol, maxit, m, L,
+                          static_cast<double *>(v), work, iters, resnorm, st)
+        : mgpcg_t<float>(static_cast<const float *>(f), false, mask, sz, vsz, tol, maxit, m, L,
+                         static_cast<float *>(v), work, iters, resnorm, st);
+}
+
+int qsm_mgpcg(int dtype, const void *f, const uint8_t *mask, const size_t sz[3],
+              const double vsz[3], double tol, int maxit, const qsm_mg_opts *mgopts, void *v,
+              int *iters, double *resnorm)
+{
+    int L, rc = check_mg(dtype, sz, vsz, mgopts, L);
+    if (rc) return rc;
+    QSM_ARG(f && mask && v, "null pointer");
+    QSM_ARG(maxit >= 1 && std::isfinite(tol), "tol must be finite and maxit >= 1");
+    if ((rc = ensure_device())) return rc;
+    const size_t N = sz[0] * sz[1] * sz[2], es = dtype_size(dtype);
+    DevBuf df, dm, dv, dw;
+    if ((rc = df.alloc(N * es)) || (rc = dm.alloc(N)) || (rc = dv.alloc(N * es))
+        || (rc = dw.alloc(qsm_mgpcg_work_bytes(dtype, sz, L))))
+        return rc;
+    QSM_HIP(hipMemcpy(df.p, f, N * es, hipMemcpyHostToDevice));
+    QSM_HIP(hipMemcpy(dm.p, mask, N, hipMemcpyHostToDevice));
+    if ((rc = qsm_mgpcg_dev(dtype, df.p, static_cast<const uint8_t *>(dm.p), sz, vsz, tol, maxit,
+                            mgopts, dv.p, dw.p, iters, resnorm, nullptr)))
+        return rc;
+    QSM_HIP(hipStreamSynchronize(nullptr));
+    QSM_HIP(hipMemcpy(v, dv.p, N * es, hipMemcpyDeviceToHost));
+    return QSM_OK;
+}
+
+int qsm_unwrap_laplacian_dev(int dtype, const void *u, const uint8_t *mask, const size_t sz[3],
+                             const double vsz[3], void *uphas, void *d2u, void *work,
+                             int *iters, void *stream)
+{
+    // unwrapLaplacian.m:78-98 on an already cropped volume: d2u = lapw(u,
+    // h), then mgpcg(-m .* d2u, m, h, 1e-8, ceil(sqrt(numel(m))), mgopts
+    // with maxit = npre = npost = nboundary = 2)
+    qsm_mg_opts o;
+    qsm_mg_opts_default(&o);
+    o.maxit = 2;
+    o.npre = 2;
+    o.npost = 2;
+    o.nboundary = 2;
+    int L, rc = check_mg(dtype, sz, vsz, &o, L);
+    if (rc) return rc;
+    QSM_ARG(u && mask && uphas && d2u && work, "null pointer");
+    const size_t N = sz[0] * sz[1] * sz[2];
+    const hipStream_t st = as_stream(stream);
+    // lapw leaves the faces untouched: they are zero in the reference
+    QSM_HIP(hipMemsetAsync(d2u, 0, N * dtype_size(dtype), st));
+    if ((rc = qsm_lapw_dev(dtype, d2u, u, sz, vsz, stream))) return rc;
+    const int maxit = int(std::ceil(std::sqrt(double(N))));
+    const MgParams m{o.tol, o.maxit, o.mu, o.npre, o.npost, o.nboundary};
+    return dtype == QSM_F64
+        ? mgpcg_t<double>(static_cast<const double *>(d2u), true, mask, sz, vsz, 1e-8, maxit, m,
+                          L, static_cast<double *>(uphas), work, iters, nullptr, st)
+        : mgpcg_t<float>(static_cast<const float *>(d2u), true, mask, sz, vsz, 1e-8, maxit, m, L,
+                         static_cast<float *>(uphas), work, iters, nullptr, st);
+}
+
+int qsm_unwrap_laplacian(int dtype, const void *phas, const uint8_t *mask, const size_t sz[3],
+                         size_t nechoes, const double vsz[3], void *uphas, void *d2phas)
+{
+    // unwrapLaplacian.m:60-75: crop to the mask's box grown by one voxel
+    // (clamped to the volume), per echo unwrap_, uncrop into zeros
+    QSM_ARG(dtype == QSM_F64 || dtype == QSM_F32, "bad dtype");
+    QSM_ARG(phas && mask && sz && vsz && uphas && nechoes >= 1, "null pointer or no echo");
+    int rc = check_grid(sz, vsz, 1);
+    if (rc) return rc;
+    const size_t es = dtype_size(dtype), N = sz[0] * sz[1] * sz[2];
+    std::memset(uphas, 0, N * nechoes * es);
+    if (d2phas) std::memset(d2phas, 0, N * nechoes * es);
+    size_t lo[3] = {sz[0], sz[1], sz[2]}, hi[3] = {0, 0, 0};
+    size_t l = 0;
+    for (size_t k = 0; k < sz[2]; ++k)
+        for (size_t j = 0; j < sz[1]; ++j)
+            for (size_t i = 0; i < sz[0]; ++i, ++l)
+                if (mask[l]) {
+                    const size_t x[3] = {i, j, k};
+                    for (int a = 0; a < 3; ++a) {
+                        lo[a] = std::min(lo[a], x[a]);
+                        hi[a] = std::max(hi[a], x[a]);
+                    }
+                }
+    if (lo[0] > hi[0]) return QSM_OK;  // empty mask
+    size_t c0[3], cs[3];
+    for (int a = 0; a < 3; ++a) {
+        c0[a] = lo[a] > 0 ? lo[a] - 1 : 0;
+        const size_t c1 = std::min(sz[a] - 1, hi[a] + 1);
+        cs[a] = c1 - c0[a] + 1;
+    }
+    qsm_mg_opts o;
+    qsm_mg_opts_default(&o);
+    int L;
+    if ((rc = check_mg(dtype, cs, vsz, &o, L))) return rc;
+    if ((rc = ensure_device())) return rc;
+    const size_t n = cs[0] * cs[1] * cs[2];
+    std::vector<char> buf(n * es), out(n * es), d2(n * es);
+    std::vector<uint8_t> mc(n);
+    auto crop_copy = [&](const char *src, char *dst, bool to_crop) {
+        for (size_t k = 0; k < cs[2]; ++k)
+            for (size_t j = 0; j < cs[1]; ++j) {
+                const size_t lf = c0[0] + sz[0] * ((c0[1] + j) + sz[1] * (c0[2] + k));
+                const size_t lc = cs[0] * (j + cs[1] * k);
+                if (to_crop) std::memcpy(dst + lc * es, src + lf * es, cs[0] * es);
+                else std::memcpy(dst + lf * es, src + lc * es, cs[0] * es);
+            }
+    };
+    for (size_t k = 0; k < cs[2]; ++k)
+        for (size_t j = 0; j < cs[1]; ++j)
+            for (size_t i = 0; i < cs[0]; ++i)
+                mc[i + cs[0] * (j + cs[1] * k)] =
+                    mask[(c0[0] + i) + sz[0] * ((c0[1] + j) + sz[1] * (c0[2] + k))] ? 1 : 0;
+    DevBuf du, dm, dout, dd2, dw;
+    if ((rc = du.alloc(n * es)) || (rc = dm.alloc(n)) || (rc = dout.alloc(n * es))
+        || (rc = dd2.alloc(n * es)) || (rc = dw.alloc(qsm_mgpcg_work_bytes(dtype, cs, L))))
+        return rc;
+    QSM_HIP(hipMemcpy(dm.p, mc.data(), n, hipMemcpyHostToDevice));
+    for (size_t e = 0; e < nechoes; ++e) {
+        crop_copy(static_cast<const char *>(phas) + e * N * es, buf.data(), true);
+        QSM_HIP(hipMemcpy(du.p, buf.data(), n * es, hipMemcpyHostToDevice));
+        if ((rc = qsm_unwrap_laplacian_dev(dtype, du.p, static_cast<const uint8_t *>(dm.p), cs,
+                                           vsz, dout.p, dd2.p, dw.p, nullptr, nullptr)))
+            return rc;
+        QSM_HIP(hipStreamSynchronize(nullptr));
+        QSM_HIP(hipMemcpy(out.data(), dout.p, n * es, hipMemcpyDeviceToHost));
+        crop_copy(out.data(), static_cast<char *>(uphas) + e * N * es, false);
+        if (d2phas) {
+            QSM_HIP(hipMemcpy(d2.data(), dd2.p, n * es, hipMemcpyDeviceToHost));
+            crop_copy(d2.data(), static_cast<char *>(d2phas) + e * N * es, false);
+        }
+    }
+    return QSM_OK;
+}
+
+}  // extern "C"

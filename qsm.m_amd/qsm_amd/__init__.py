@@ -10,6 +10,10 @@ MEX/MATLAB interface for that path:
   sstgv.sstgv                        src/inversion/sstgv.m
   dipole.dipole_conv                 dipoleKernel.m + fft3/ifft3 (+ padfastfft)
   fastfft.nextfastfft/padfastfft     third_party/nextfastfft.m, padfastfft.m
+  tv.tv / rts.rts                    src/inversion/tv.m, rts.m (ADMM)
+  bgremove.sharp / bgremove.vsharp   src/bgremove/sharp.m, vsharp.m
+  unwrap.unwrap_laplacian            src/unwrap/unwrapLaplacian.m
+  unwrap.mgpcg                       src/utils/poisson_solver/mgpcg.m
 """
 from ._lib import F32, F64, TGV, TV, QsmAdmissionError, QsmError, lib  # noqa: F401
 

@@ -97,3 +97,41 @@ def test_rts_opts_defaults_are_rts_m():
     o = grts.make_opts(None)
     assert (o.tolls, o.mu, o.delta, o.tv, o.rtol, o.atol, o.maxit) == (4, 1e5, 0.15, 1, 1e-1, 1e-2, 20)
     assert (o.verbose, o.rho, o.alpha, o.rtau, o.stau, o.gamma) == (1, 10.0, 1.5, 0.7, 1e2, 3.0)
+
+
+def test_mg_opts_defaults_are_mgpcg_m():
+    """qsm_mg_opts_default fills mgpcg.m:71-101's defaults (nlevels 0: the
+    size rule max(1, floor(log2(min(size))) - 3), applied by the library)."""
+    from qsm_amd import unwrap as uw
+
+    o = uw.make_mgopts(None)
+    assert (o.tol, o.maxit, o.mu, o.npre, o.npost, o.nboundary, o.nlevels) == (-1.0, 1, 1, 1, 1, 2, 0)
+    o = uw.make_mgopts({"npre": 2, "nlevels": 3, "tol": []})
+    assert (o.npre, o.nlevels, o.tol) == (2, 3, -1.0)
+
+
+@pytest.mark.parametrize("kw,msg", [({"mgopts": {"nlevels": 16}}, "nlevels"),
+                                    ({"mgopts": {"mu": 0}}, "mgopts"),
+                                    ({"maxit": 0}, "maxit")])
+def test_mgpcg_argument_checks_without_gpu(kw, msg):
+    """More levels than the library holds, invalid mgopts and maxit are
+    refused before any device work."""
+    import numpy as np
+
+    from qsm_amd import QsmError
+    from qsm_amd import unwrap as uw
+
+    f = np.zeros((16, 16, 16))
+    with pytest.raises(QsmError, match=msg) as e:
+        uw.mgpcg(f, np.ones(f.shape), (1, 1, 1), **kw)
+    assert e.value.code == _lib.QSM_EARG
+
+
+def test_mgpcg_work_bytes_cover_the_hierarchy():
+    """Work space: per level v, f, r and two mask bytes; PCG x, p; coarse
+    CG scratch; band seed; reduction partials."""
+    n = [(34, 30, 26), (18, 16, 14), (10, 9, 8)]      # 32x28x24 padded, 3 levels
+    vox = [a * b * c for a, b, c in n]
+    w = _lib.lib().qsm_mgpcg_work_bytes(_lib.F64, (C.c_size_t * 3)(32, 28, 24), 3)
+    lo = sum(3 * 8 * v + 2 * v for v in vox) + 2 * 8 * vox[0] + 2 * 8 * vox[-1] + vox[0]
+    assert lo <= w <= lo + 64 * 1024
