@@ -478,6 +478,112 @@ def bgremove_leg(cfg, field, mask, reps=3, cpu=True):
     return out
 
 
+def _wrapped_phase(field, mask):
+    """A wrapped phase on the volume: the field scaled to 8 rad peak inside
+    the mask plus a smooth quadratic background of about 40 rad across the
+    volume, wrapped to (-pi, pi]."""
+    shp = field.shape
+    g = np.meshgrid(*(np.linspace(-1.0, 1.0, n) for n in shp), indexing="ij")
+    m = np.asarray(mask) != 0
+    peak = float(np.abs(field[m]).max()) if m.any() else 1.0
+    phi = 8.0 * field / (peak if peak > 0 else 1.0) + 20.0 * (g[0] ** 2 + 0.5 * g[1] * g[2] - 0.3 * g[2])
+    return np.asfortranarray(np.angle(np.exp(1j * phi)))
+
+
+def unwrap_leg(cfg, field, mask, reps=3, cpu=True):
+    """unwrapLaplacian.m at the C3 size on a wrapped phase made from the
+    phantom's field (_wrapped_phase) with the phantom's mask, both
+    precisions.  device_ms: the device form (lapw + MGPCG) on the crop (the
+    mask's box grown by one voxel) resident in HBM, HIP events on its
+    stream; host_ms: the drop-in's form (host arrays in and out); iterations:
+    PCG iterations of the device form.  cpu: the oracle's numpy port of the
+    reference's algorithm (mgpcg_mex.c's MGPCG, FP64, unwrapLaplacian's
+    options) for a fixed 4 PCG iterations on the centre 128^3 block with
+    the mask restricted to it, per iteration, beside the device form's time
+    per iteration on that same block.  A fixed count: on a mask that
+    touches the volume's faces (the phantom's covers the whole volume) the
+    reference's red-black loops, which skip a row end of the second colour,
+    make its V-cycle a poor preconditioner and PCG would run for hundreds of
+    iterations; the port's residual after the 4 is reported."""
+    import torch
+    from qsm_amd import unwrap as uw
+
+    vsz = cfg["spec"].vsz
+    t = time.perf_counter()
+    phase = _wrapped_phase(field, mask)
+    m = np.asarray(mask) != 0
+    log(f"[bench] unwrap: wrapped phase ready in {time.perf_counter() - t:.1f}s")
+    out = {"shape": list(field.shape), "mask_voxels": int(np.count_nonzero(m))}
+    nz = np.argwhere(m)
+    lo = np.maximum(nz.min(axis=0) - 1, 0)
+    hi = np.minimum(nz.max(axis=0) + 1, np.array(m.shape) - 1)
+    crop = tuple(slice(a, b + 1) for a, b in zip(lo, hi))
+    st = torch.cuda.Stream()
+
+    def on_device(u, mk, dt):
+        shp = u.shape
+        with torch.cuda.stream(st):
+            tu = torch.from_numpy(np.ascontiguousarray(u.astype(dt).reshape(-1, order="F"))).to("cuda")
+            tu = tu.view(shp[::-1]).permute(2, 1, 0)
+            tm = torch.from_numpy(np.ascontiguousarray(mk.astype(np.uint8).reshape(-1, order="F")))
+            tm = tm.to("cuda").view(shp[::-1]).permute(2, 1, 0)
+            up, d2 = torch.empty_like(tu), torch.empty_like(tu)
+            w = torch.empty(uw.work_bytes(tu.dtype, shp), dtype=torch.uint8, device="cuda")
+        it = uw.unwrap_laplacian_dev(tu, tm, vsz, up, d2, w, st)
+        st.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            uw.unwrap_laplacian_dev(tu, tm, vsz, up, d2, w, st)
+        e1.record(st)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        del tu, tm, up, d2, w
+        return ms, it
+
+    for dt, tag in ((np.float32, "f32"), (np.float64, "f64")):
+        p = np.asfortranarray(phase.astype(dt))
+        t = time.perf_counter()
+        uw.unwrap_laplacian(p, m, vsz)
+        log(f"[bench] unwrap {tag}: host form warm-up {time.perf_counter() - t:.2f}s")
+        t = time.perf_counter()
+        for _ in range(reps):
+            uw.unwrap_laplacian(p, m, vsz)
+        host_ms = (time.perf_counter() - t) / reps * 1e3
+        dev_ms, it = on_device(np.asfortranarray(p[crop]), m[crop], dt)
+        log(f"[bench] unwrap {tag}: host form {host_ms:.1f} ms, device form {dev_ms:.1f} ms, "
+            f"{it} PCG iterations")
+        out[tag] = {"device_ms": dev_ms, "host_ms": host_ms, "iterations": it,
+                    "crop": [int(b - a + 1) for a, b in zip(lo, hi)]}
+    torch.cuda.empty_cache()
+    if cpu:
+        from oracle import poisson as opo
+
+        c = [n // 2 for n in field.shape]
+        blk = tuple(slice(x - 64, x + 64) for x in c)
+        pb, mb = np.asfortranarray(phase[blk]), m[blk]
+        from oracle import oracle as orc
+
+        rhs = np.asfortranarray(-(mb * orc.lapw(pb, vsz)))
+        t = time.perf_counter()
+        _, cpu_it, hist = opo.mgpcg_port(rhs, mb, vsz, 1e-8, 4,
+                                         {"maxit": 2, "npre": 2, "npost": 2, "nboundary": 2})
+        cpu_ms = (time.perf_counter() - t) * 1e3
+        log(f"[bench] unwrap cpu port (128^3 block): {cpu_ms / 1e3:.1f}s for {cpu_it} PCG iterations")
+        dev_ms, dev_it = on_device(pb, mb, np.float64)
+        fn = float(np.linalg.norm(rhs))
+        out["cpu"] = {"ms_per_iteration": cpu_ms / max(cpu_it, 1), "iterations_run": cpu_it,
+                      "rel_residual_after": float(hist[-1] / fn) if cpu_it and fn > 0 else 0.0,
+                      "cores": 1, "kind": "port",
+                      "device_ms_same_block_f64": dev_ms, "device_iterations_same_block": dev_it,
+                      "device_ms_per_iteration_same_block": dev_ms / max(dev_it, 1),
+                      "sample": "oracle port of mgpcg_mex.c (numpy, float64), unwrapLaplacian's "
+                                "options, 4 PCG iterations on the centre 128^3 block of the same "
+                                "phase and mask"}
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[1])
     ap.add_argument("--gpus", type=int, default=1)
@@ -494,6 +600,8 @@ def main(argv=None) -> int:
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--no-bgremove", dest="bgremove", action="store_false",
                     help="skip the SHARP / V-SHARP leg")
+    ap.add_argument("--no-unwrap", dest="unwrap", action="store_false",
+                    help="skip the unwrapLaplacian (lapw + MGPCG) leg")
     ap.add_argument("--admm-iters", type=int, default=20,
                     help="ADMM iterations of the tv.m / rts.m leg (0: skip)")
     ap.add_argument("--no-dipole", dest="dipole", action="store_false",
@@ -602,6 +710,11 @@ def main(argv=None) -> int:
         out["bgremove"] = bgremove_leg(cfg, field, mask, cpu=a.cpu_iters > 0)
         log(f"[bench] sharp f32 {out['bgremove']['sharp']['f32']['device_ms']:.2f} ms on device, "
             f"vsharp f32 {out['bgremove']['vsharp']['f32']['device_ms']:.2f} ms")
+    if a.unwrap:
+        out["unwrap"] = unwrap_leg(cfg, field, mask, cpu=a.cpu_iters > 0)
+        log(f"[bench] unwrapLaplacian f32 {out['unwrap']['f32']['device_ms']:.2f} ms on device "
+            f"({out['unwrap']['f32']['iterations']} PCG iterations), "
+            f"f64 {out['unwrap']['f64']['device_ms']:.2f} ms")
     if a.batch_iters > 0:
         out["batch"] = batch_leg(a.batch_iters)
         log(f"[bench] C4 batch: {out['batch']['voxel_iters_per_s']:.3e} voxel-iters/s")

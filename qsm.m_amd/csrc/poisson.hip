@@ -9,8 +9,8 @@
 // preconditioner is mgopts.maxit V-cycles (mx_mg, mx_cycle): pre-smoothing
 // on G and on the boundary band B (2^l nboundary sweeps on level l), the
 // residual, full-weighting restriction onto the coarse mask, the coarse
-// cycle, trilinear correction, the mirrored post-smoothing; the coarsest
-// level is solved to a relative residual of 1e-8 (double) / 1e-4 (single).
+// cycle, trilinear correction, the mirrored post-smoothing; on the
+// coarsest level the reference sweeps to a relative residual of 1e-8.
 // The grid hierarchy (the voxel size doubles per level), the masks (a coarse
 // voxel is active iff its 8 fine children are: coarsen_grid_mex.c), the
 // band (boundary_mask_mex.c) and the transfer weights (restrict_mex.c,
@@ -21,10 +21,15 @@
 // choices differ from the reference and change only the preconditioner,
 // hence how fast PCG converges, not what it converges to: every voxel of a
 // colour is updated (the reference's loops skip the row end of the second
-// colour on rows with j = k mod 2), and the coarsest level is solved by
-// conjugate gradients in one workgroup instead of by rounds of 256
-// Gauss-Seidel sweeps (gauss-seidel needs O(n^2) sweeps on an n^3 coarse
-// grid, CG O(n) iterations).
+// colour on rows with j = k mod 2, which on a mask that touches the
+// volume's faces makes the V-cycle a poor preconditioner), and the coarsest
+// level takes a fixed-degree Chebyshev iteration on the Jacobi-scaled
+// operator instead of rounds of 256 Gauss-Seidel sweeps.  Its degree
+// reduces the coarse error by 1e3 over the spectrum bound of the level's
+// box (the masked operator is a principal submatrix of the box's, so the
+// bound holds for any mask); each step is one stencil pass over the level
+// by many workgroups, with no reduction, and a fixed polynomial keeps the
+// V-cycle a fixed linear map, as PCG requires of its preconditioner.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -38,8 +43,9 @@ namespace {
 
 constexpr int BX = 64, BY = 4, NT = BX * BY;
 constexpr int MAX_LEVELS = 16;
-constexpr int COARSE_T = 1024;    // threads of the coarsest-level solve
-constexpr int COARSE_MAXIT = 4096;
+constexpr int CHEB_MAX = 512;     // Chebyshev degree cap on the coarsest level
+constexpr double CHEB_EPS = 1e-3; // coarse error reduction it aims for
+constexpr int DOT_BLOCKS = 1024;  // fixed grid of the dot products
 
 struct Dim {
     int nx, ny, nz;
@@ -286,77 +292,49 @@ __global__ __launch_bounds__(NT) void correct_kernel(T *v, const T *v2, const ui
             }
 }
 
-// Sum over the workgroup (COARSE_T threads), the same on every thread,
-// in a fixed order.
-__device__ double block_sum(double x, double *red)
+// Coarsest level, Chebyshev iteration on hh A (A = -Lap_h, hh = 1 /
+// (2 (hx + hy + hz)), spectrum in [lmin, lmax]; Saad, Iterative Methods for
+// Sparse Linear Systems, 2nd ed., Algorithm 12.1).  Start: r = f - A v on
+// the active voxels, d = s0 hh r with s0 = 1 / theta.
+template <typename T>
+__global__ __launch_bounds__(NT) void cheb_start_kernel(T *r, T *dn, const T *v, const T *f,
+                                                        const uint8_t *G, Dim d, Lap c, T s0)
 {
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
-    __syncthreads();
-    double s = 0.0;
-    for (int w = 0; w < COARSE_T / 64; ++w) s += red[w];
-    __syncthreads();
-    return s;
+    int i, j, k;
+    size_t l;
+    if (!vox(d, i, j, k, l)) return;
+    T rl = T(0);
+    if (interior(d, i, j, k) && G[l]) {
+        const size_t sy = size_t(d.nx), sz = size_t(d.nx) * d.ny;
+        const T hh = T(-2.0 * (c.hx + c.hy + c.hz));
+        rl = f[l] + (((hh * v[l] + T(c.hx) * (v[l - 1] + v[l + 1])) + T(c.hy) * (v[l - sy] + v[l + sy])) +
+                     T(c.hz) * (v[l - sz] + v[l + sz]));
+    }
+    r[l] = rl;
+    dn[l] = s0 * T(1.0 / (2.0 * (c.hx + c.hy + c.hz))) * rl;
 }
 
-// The coarsest level: -Lap v = f on G by conjugate gradients from the
-// current v, in one workgroup, until ||f + Lap v|| < tol ||f|| (the
-// reference's criterion for its coarse sweeps, mgpcg_mex.c: mx_cycle) or
-// maxit iterations.  r, p, q: scratch of the level's size.
+// One step: v += d; r -= A d; dn = c1 d + c2 hh r (active voxels; dn = 0
+// elsewhere).  d is read with its neighbours, so dn is another buffer.
 template <typename T>
-__global__ __launch_bounds__(COARSE_T) void coarse_kernel(T *v, const T *f, T *r, T *p, T *q,
-                                                          const uint8_t *G, Dim d, Lap c,
-                                                          double tol, int maxit)
+__global__ __launch_bounds__(NT) void cheb_step_kernel(T *v, T *r, const T *dd, T *dn,
+                                                       const uint8_t *G, Dim d, Lap c, T c1, T c2)
 {
-    __shared__ double red[COARSE_T / 64];
-    const size_t n = d.n();
-    const size_t sy = size_t(d.nx), sz = size_t(d.nx) * d.ny;
-    const T hh = T(-2.0 * (c.hx + c.hy + c.hz));
-    auto active = [&](size_t l) {
-        const int i = int(l % d.nx), j = int((l / d.nx) % d.ny), k = int(l / sz);
-        return interior(d, i, j, k) && G[l];
-    };
-    // -Lap_h x at l (an active voxel)
-    auto negl = [&](const T *x, size_t l) {
-        return -(((hh * x[l] + T(c.hx) * (x[l - 1] + x[l + 1])) + T(c.hy) * (x[l - sy] + x[l + sy])) +
-                 T(c.hz) * (x[l - sz] + x[l + sz]));
-    };
-    double ff = 0.0, rr = 0.0;
-    for (size_t l = threadIdx.x; l < n; l += COARSE_T) {
-        const T fl = f[l];
-        ff += double(fl) * double(fl);
-        const T rl = active(l) ? T(fl - negl(v, l)) : T(0);
+    int i, j, k;
+    size_t l;
+    if (!vox(d, i, j, k, l)) return;
+    T out = T(0);
+    if (interior(d, i, j, k) && G[l]) {
+        const size_t sy = size_t(d.nx), sz = size_t(d.nx) * d.ny;
+        const T hh = T(-2.0 * (c.hx + c.hy + c.hz));
+        const T ad = -(((hh * dd[l] + T(c.hx) * (dd[l - 1] + dd[l + 1])) + T(c.hy) * (dd[l - sy] + dd[l + sy])) +
+                       T(c.hz) * (dd[l - sz] + dd[l + sz]));
+        v[l] = v[l] + dd[l];
+        const T rl = r[l] - ad;
         r[l] = rl;
-        p[l] = T(0);
-        rr += double(rl) * double(rl);
+        out = c1 * dd[l] + c2 * T(1.0 / (2.0 * (c.hx + c.hy + c.hz))) * rl;
     }
-    ff = block_sum(ff, red);
-    rr = block_sum(rr, red);
-    const double stop = tol * tol * ff;
-    double rho_prev = 1.0;
-    for (int it = 0; it < maxit && rr > 0.0 && rr >= stop; ++it) {
-        const T beta = T(it == 0 ? 0.0 : rr / rho_prev);
-        for (size_t l = threadIdx.x; l < n; l += COARSE_T) p[l] = r[l] + beta * p[l];
-        __syncthreads();
-        double pq = 0.0;
-        for (size_t l = threadIdx.x; l < n; l += COARSE_T) {
-            const T ql = active(l) ? negl(p, l) : T(0);
-            q[l] = ql;
-            pq += double(p[l]) * double(ql);
-        }
-        pq = block_sum(pq, red);
-        if (!(pq > 0.0)) break;
-        const T alpha = T(rr / pq);
-        rho_prev = rr;
-        double rn = 0.0;
-        for (size_t l = threadIdx.x; l < n; l += COARSE_T) {
-            v[l] = v[l] + alpha * p[l];
-            const T rl = r[l] - alpha * q[l];
-            r[l] = rl;
-            rn += double(rl) * double(rl);
-        }
-        rr = block_sum(rn, red);
-    }
+    dn[l] = out;
 }
 
 // ---- PCG vector kernels ----------------------------------------------------
@@ -388,13 +366,16 @@ __global__ __launch_bounds__(NT) void update_kernel(T *x, T *r, const T *p, cons
     }
 }
 
-// per-block partial sums of a[l] * b[l] (fixed order)
+// partial sums of a[l] * b[l] over a fixed grid of DOT_BLOCKS blocks (each
+// thread strides the array; fixed order, so the result is reproducible)
 template <typename T>
 __global__ __launch_bounds__(NT) void dot_kernel(const T *a, const T *b, size_t n, double *part)
 {
     __shared__ double red[NT];
-    const size_t l = size_t(blockIdx.x) * NT + threadIdx.x;
-    red[threadIdx.x] = l < n ? double(a[l]) * double(b[l]) : 0.0;
+    double acc = 0.0;
+    for (size_t l = size_t(blockIdx.x) * NT + threadIdx.x; l < n; l += size_t(DOT_BLOCKS) * NT)
+        acc += double(a[l]) * double(b[l]);
+    red[threadIdx.x] = acc;
     __syncthreads();
     for (int s = NT / 2; s > 0; s >>= 1) {
         if (int(threadIdx.x) < s) red[threadIdx.x] += red[threadIdx.x + s];
@@ -471,7 +452,7 @@ size_t solver_layout(const size_t sz[3], const double vsz[3], int L, Solver<T> *
     char *px = take(n0 * sizeof(T)), *pp = take(n0 * sizeof(T)), *pt = take(n0);
     const size_t nc = d[L - 1].n();
     char *pcp = take(nc * sizeof(T)), *pcq = take(nc * sizeof(T));
-    char *ppart = take(g1(n0) * sizeof(double)), *psum = take(sizeof(double));
+    char *ppart = take(DOT_BLOCKS * sizeof(double)), *psum = take(sizeof(double));
     if (s) {
         s->L = L;
         s->x = reinterpret_cast<T *>(px);
@@ -488,9 +469,9 @@ size_t solver_layout(const size_t sz[3], const double vsz[3], int L, Solver<T> *
 template <typename T>
 int dot(const T *a, const T *b, size_t n, Solver<T> &s, double &out, hipStream_t st)
 {
-    dot_kernel<T><<<g1(n), NT, 0, st>>>(a, b, n, s.part);
+    dot_kernel<T><<<DOT_BLOCKS, NT, 0, st>>>(a, b, n, s.part);
     QSM_HIP(hipGetLastError());
-    sum_kernel<<<1, NT, 0, st>>>(s.part, int(g1(n)), s.sum);
+    sum_kernel<<<1, NT, 0, st>>>(s.part, DOT_BLOCKS, s.sum);
     QSM_HIP(hipGetLastError());
     QSM_HIP(hipMemcpyAsync(&out, s.sum, sizeof(double), hipMemcpyDeviceToHost, st));
     QSM_HIP(hipStreamSynchronize(st));
@@ -510,6 +491,52 @@ int smooth(Level<T> &L, const uint8_t *M, int n, bool reverse, hipStream_t st)
             gs_kernel<T><<<g, b, 0, st>>>(L.v, L.f, M, L.d, L.c, first ^ h);
             QSM_HIP(hipGetLastError());
         }
+    return QSM_OK;
+}
+
+// Spectrum bounds of hh A on the level's box (interior nx-2 x ny-2 x nz-2,
+// Dirichlet): lmin = hh sum_a 2 h_a (1 - cos(pi / (m_a + 1))), lmax = 2 -
+// lmin; the Chebyshev degree that reduces the error by CHEB_EPS over them.
+struct Cheb {
+    double lmin, lmax;
+    int K;
+};
+
+Cheb cheb_of(const Dim &d, const Lap &c)
+{
+    const double pi = 3.14159265358979323846;
+    const double hh = 1.0 / (2.0 * (c.hx + c.hy + c.hz));
+    const double w[3] = {c.hx, c.hy, c.hz};
+    const int m[3] = {d.nx - 2, d.ny - 2, d.nz - 2};
+    double lmin = 0.0;
+    for (int a = 0; a < 3; ++a) lmin += hh * 2.0 * w[a] * (1.0 - std::cos(pi / (m[a] + 1)));
+    const double lmax = 2.0 - lmin;
+    const double sigma = (lmax + lmin) / (lmax - lmin);
+    const int K = int(std::ceil(std::acosh(1.0 / CHEB_EPS) / std::acosh(sigma)));
+    return Cheb{lmin, lmax, std::max(1, std::min(K, CHEB_MAX))};
+}
+
+// The coarsest level: K Chebyshev steps from the current v (restriction
+// zeroed it, except on a one-level hierarchy's later cycles).
+template <typename T>
+int coarse_solve(Solver<T> &s, Level<T> &L, hipStream_t st)
+{
+    const Cheb ch = cheb_of(L.d, L.c);
+    const double theta = 0.5 * (ch.lmax + ch.lmin), delta = 0.5 * (ch.lmax - ch.lmin);
+    const double sigma = theta / delta;
+    const dim3 g = grid3(L.d), b(BX, BY);
+    T *d0 = s.cp, *d1 = s.cq;
+    cheb_start_kernel<T><<<g, b, 0, st>>>(L.r, d0, L.v, L.f, L.G, L.d, L.c, T(1.0 / theta));
+    QSM_HIP(hipGetLastError());
+    double rho = 1.0 / sigma;
+    for (int k = 0; k < ch.K; ++k) {
+        const double rho1 = 1.0 / (2.0 * sigma - rho);
+        cheb_step_kernel<T><<<g, b, 0, st>>>(L.v, L.r, d0, d1, L.G, L.d, L.c, T(rho1 * rho),
+                                             T(2.0 * rho1 / delta));
+        QSM_HIP(hipGetLastError());
+        rho = rho1;
+        std::swap(d0, d1);
+    }
     return QSM_OK;
 }
 
@@ -543,10 +570,7 @@ int cycle(Solver<T> &s, int l, const MgParams &m, hipStream_t st)
         if ((rc = smooth(L, L.B, bs, true, st))) return rc;
         if ((rc = smooth(L, L.G, m.npost, true, st))) return rc;
     } else {
-        const double tol = sizeof(T) == 4 ? 1e-4 : 1e-8;
-        coarse_kernel<T><<<1, COARSE_T, 0, st>>>(L.v, L.f, L.r, s.cp, s.cq, L.G, L.d, L.c, tol,
-                                                 COARSE_MAXIT);
-        QSM_HIP(hipGetLastError());
+        if ((rc = coarse_solve(s, L, st))) return rc;
     }
     return QSM_OK;
 }

@@ -66,3 +66,29 @@ def test_masked_solution_residual():
     lap = (hx * (P[2:, 1:-1, 1:-1] + P[:-2, 1:-1, 1:-1]) + hy * (P[1:-1, 2:, 1:-1] + P[1:-1, :-2, 1:-1])
            + hz * (P[1:-1, 1:-1, 2:] + P[1:-1, 1:-1, :-2]) - 2 * (hx + hy + hz) * v)
     np.testing.assert_allclose(-lap[m], f[m], atol=1e-10)
+
+
+@pytest.mark.parametrize("mg", [{}, {"nlevels": 1}, {"mu": 2, "npre": 2, "npost": 2},
+                                {"tol": 1e-3, "maxit": 3}])
+def test_port_of_the_reference_mgpcg_converges_to_the_exact_solution(mg):
+    """The numpy restatement of mgpcg_mex.c (used to time the reference's
+    algorithm on the CPU) reaches the exact discrete solution, in few
+    iterations: a working multigrid preconditioner."""
+    rng = np.random.default_rng(2)
+    shape = (26, 22, 20)
+    c = [(n - 1) / 2 for n in shape]
+    g = np.meshgrid(*(np.arange(n) - c[a] for a, n in enumerate(shape)), indexing="ij")
+    m = ((g[0] ** 2 + g[1] ** 2 + g[2] ** 2) <= 81) & ~(((g[0] - 2) ** 2 + g[1] ** 2) < 4)
+    f = rng.standard_normal(shape) * m
+    v, it, res = po.mgpcg_port(f, m, (1.0, 1.2, 0.9), 1e-12, 100, mg)
+    want = po.solve(f, m, (1.0, 1.2, 0.9))
+    assert np.linalg.norm(v - want) <= 1e-9 * np.linalg.norm(want)
+    assert it <= (40 if mg.get("nlevels") == 1 else 20), it
+    assert np.all(np.diff(np.log(res)) < 0.5)
+
+
+def test_port_unwrap_recovers_a_wrapped_phase():
+    shape = (24, 22, 20)
+    phi = sine_phase(shape, 10.0)
+    up, _, it = po.unwrap_laplacian_port(np.angle(np.exp(1j * phi)), interior(shape), (1, 1, 1))
+    np.testing.assert_allclose(up, phi, atol=1e-6)
