@@ -584,6 +584,39 @@ def unwrap_leg(cfg, field, mask, reps=3, cpu=True):
     return out
 
 
+def nifti_leg(field, mask, vsz, reps=3):
+    """saveNii.m / loadNii.m at the C3 size (host IO, the request payloads):
+    the field as float64 .nii and float32 .nii.gz, the mask as uint8 .nii,
+    written and read back in a temporary folder; MB/s of image data."""
+    import shutil
+    import tempfile
+
+    from qsm_amd.nifti import loadNii, saveNii
+
+    out = {}
+    d = tempfile.mkdtemp(prefix="qsm_nii_")
+    try:
+        for tag, arr, name in (("f64_nii", field.astype(np.float64), "f.nii"),
+                               ("f32_nii_gz", field.astype(np.float32), "f.nii.gz"),
+                               ("mask_u8_nii", (np.asarray(mask) != 0).astype(np.uint8), "m.nii")):
+            fn = os.path.join(d, name)
+            t = time.perf_counter()
+            for _ in range(reps):
+                saveNii(fn, arr, vsz)
+            w = (time.perf_counter() - t) / reps
+            t = time.perf_counter()
+            for _ in range(reps):
+                img = loadNii(fn)["img"]
+            r = (time.perf_counter() - t) / reps
+            assert img.shape == arr.shape
+            mb = arr.nbytes / 1e6
+            out[tag] = {"mb": mb, "file_mb": os.path.getsize(fn) / 1e6, "save_ms": w * 1e3,
+                        "load_ms": r * 1e3, "save_mb_s": mb / w, "load_mb_s": mb / r}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[1])
     ap.add_argument("--gpus", type=int, default=1)
@@ -600,6 +633,8 @@ def main(argv=None) -> int:
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--no-bgremove", dest="bgremove", action="store_false",
                     help="skip the SHARP / V-SHARP leg")
+    ap.add_argument("--no-nifti", dest="nifti", action="store_false",
+                    help="skip the NIfTI IO leg (saveNii / loadNii at C3)")
     ap.add_argument("--no-unwrap", dest="unwrap", action="store_false",
                     help="skip the unwrapLaplacian (lapw + MGPCG) leg")
     ap.add_argument("--admm-iters", type=int, default=20,
@@ -715,6 +750,10 @@ def main(argv=None) -> int:
         log(f"[bench] unwrapLaplacian f32 {out['unwrap']['f32']['device_ms']:.2f} ms on device "
             f"({out['unwrap']['f32']['iterations']} PCG iterations), "
             f"f64 {out['unwrap']['f64']['device_ms']:.2f} ms")
+    if a.nifti:
+        out["nifti"] = nifti_leg(field, mask, cfg["spec"].vsz)
+        log("[bench] NIfTI IO: " + ", ".join(f"{k} load {v['load_mb_s']:.0f} MB/s save "
+                                             f"{v['save_mb_s']:.0f} MB/s" for k, v in out["nifti"].items()))
     if a.batch_iters > 0:
         out["batch"] = batch_leg(a.batch_iters)
         log(f"[bench] C4 batch: {out['batch']['voxel_iters_per_s']:.3e} voxel-iters/s")

@@ -150,6 +150,37 @@ class Server:
         return getattr(self.b, "budget", 1 << 62)
 
 
+def request_from_nii(rid: int, arrival: float, phase_file, mask_file, kind: int, lam: float,
+                     alpha: float, maxiter: int, tol: float = 0.0, dtype=np.float32) -> Request:
+    """A request whose payload is a pair of NIfTI files, as a user of
+    sstgv.m holds them: the local-field phase and its mask (loadNii.m), the
+    voxel size from the phase's header.  d2f = -lapw(phase) on the device
+    (sstgv.m:67), in the solver's precision."""
+    from . import fd
+    from .nifti import loadNii
+
+    ph = loadNii(phase_file)
+    mk = loadNii(mask_file)
+    phase = np.asfortranarray(np.asarray(ph["img"]).astype(dtype))
+    if phase.ndim != 3 or mk["img"].shape != phase.shape:
+        raise ValueError("phase must be 3-D and the mask of the same size")
+    vsz = tuple(float(v) for v in ph["hdr"]["dime"]["pixdim"][1:4])
+    d2f = -fd.lapw(phase, vsz)
+    mask = np.asfortranarray(np.asarray(mk["img"]) != 0).astype(np.uint8)
+    return Request(rid=rid, arrival=arrival, kind=kind, d2f=np.asfortranarray(d2f), mask=mask,
+                   vsz=vsz, lam=lam, alpha=alpha, maxiter=maxiter, tol=tol)
+
+
+def save_chi(r: Request, filename):
+    """Write a finished request's susceptibility map (saveNii.m) with its
+    voxel size."""
+    from .nifti import saveNii
+
+    if r.chi is None:
+        raise ValueError(f"request {r.rid} holds no chi (not finished, or keep_chi off)")
+    return saveNii(filename, r.chi, r.vsz)
+
+
 def poisson_arrivals(rate: float, n: int, rng) -> np.ndarray:
     """n arrival times of a Poisson process of the given rate (1/s)."""
     return np.cumsum(rng.exponential(1.0 / rate, size=n))

@@ -63,3 +63,39 @@ def test_serving_with_live_voxel_cap(cuda):
         else:
             ref = orc.pd_tv(r.d2f, r.mask, r.vsz, r.lam, 0.0, r.maxiter)[0]
         np.testing.assert_array_equal(r.chi, ref, err_msg=f"request {r.rid}")
+
+
+def test_serving_from_nifti_files(cuda, tmp_path):
+    """Payloads as NIfTI files (loadNii.m): phase and mask written with
+    saveNii, requests built from the files (d2f = -lapw(phase) on the device,
+    the voxel size from the header), served, and chi written back
+    (saveNii.m); the chi read from the file is bitwise the oracle's solve of
+    the oracle's -lapw of the same phase."""
+    from qsm_amd.nifti import loadNii, saveNii
+    from qsm_amd.serve import request_from_nii, save_chi
+
+    rng = np.random.default_rng(9)
+    reqs, want = [], []
+    for i, (shape, vsz) in enumerate((((26, 22, 18), (1.0, 1.0, 1.0)), ((24, 28, 20), (0.9, 1.1, 1.2)))):
+        g = np.meshgrid(*(np.linspace(-1, 1, n) for n in shape), indexing="ij")
+        phase = np.asfortranarray((0.8 * np.sin(3 * g[0]) * g[1] + 0.3 * g[2] ** 2
+                                   + 0.01 * rng.standard_normal(shape)).astype(np.float32))
+        m = brainish_mask(shape, seed=70 + i, frac=0.9)
+        saveNii(str(tmp_path / f"phase{i}.nii.gz"), phase, vsz)
+        saveNii(str(tmp_path / f"mask{i}.nii"), m.astype(np.uint8), vsz)
+        r = request_from_nii(i, 0.01 * i, str(tmp_path / f"phase{i}.nii.gz"),
+                             str(tmp_path / f"mask{i}.nii"), _lib.TGV, 2e-3, 2.0, 40)
+        vs = tuple(float(np.float32(v)) for v in vsz)
+        assert r.vsz == vs and r.d2f.dtype == np.float32
+        d2f = -orc.lapw(phase, vs)
+        np.testing.assert_array_equal(r.d2f, d2f)
+        want.append(orc.pd_tgv(d2f, m.astype(np.uint8), vs, 2e-3, 2.0, 0.0, 40)[0])
+        reqs.append(r)
+    with Batch(np.float32, max_volumes=4) as b:
+        Server(b, chunk=10).run(reqs)
+    for r, w in zip(reqs, want):
+        fn = str(tmp_path / f"chi{r.rid}.nii")
+        save_chi(r, fn)
+        got = loadNii(fn)
+        np.testing.assert_array_equal(got["img"], w)
+        np.testing.assert_allclose(got["hdr"]["dime"]["pixdim"][1:4], r.vsz, rtol=1e-6)
