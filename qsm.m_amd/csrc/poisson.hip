@@ -54,6 +54,9 @@ struct Dim {
 
 dim3 grid3(Dim d) { return dim3((d.nx + BX - 1) / BX, (d.ny + BY - 1) / BY, d.nz); }
 
+// one lane per voxel of one colour: (nx + 1) / 2 lanes per row
+dim3 grid_half(Dim d) { return dim3(((d.nx + 1) / 2 + BX - 1) / BX, (d.ny + BY - 1) / BY, d.nz); }
+
 __device__ __forceinline__ bool vox(Dim d, int &i, int &j, int &k, size_t &l)
 {
     i = blockIdx.x * BX + threadIdx.x;
@@ -179,15 +182,21 @@ __global__ __launch_bounds__(NT) void band_kernel(uint8_t *B, const uint8_t *b, 
 
 // One colour of a red-black Gauss-Seidel sweep (gauss_seidel_mex.c):
 // v = (f + hx (v-x + v+x) + hy (...) + hz (...)) / (2 (hx + hy + hz)) on the
-// voxels of M with (i + j + k) % 2 == colour.
+// voxels of M with (i + j + k) % 2 == colour.  Lanes map onto the
+// colour's voxels only: lane t of a row takes i = 2 t + ((colour + j + k)
+// & 1), so no lane idles on the other colour (grid: grid_half).
 template <typename T>
 __global__ __launch_bounds__(NT) void gs_kernel(T *v, const T *f, const uint8_t *M, Dim d, Lap c,
                                                 int colour)
 {
-    int i, j, k;
-    size_t l;
-    if (!vox(d, i, j, k, l)) return;
-    if (((i + j + k) & 1) != colour || !interior(d, i, j, k) || !M[l]) return;
+    const int t = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (j >= d.ny) return;
+    const int i = 2 * t + ((colour + j + k) & 1);
+    if (i >= d.nx || !interior(d, i, j, k)) return;
+    const size_t l = size_t(i) + size_t(d.nx) * (size_t(j) + size_t(d.ny) * k);
+    if (!M[l]) return;
     const size_t sy = size_t(d.nx), sz = size_t(d.nx) * d.ny;
     const T hh = T(1.0 / (2.0 * (c.hx + c.hy + c.hz)));
     v[l] = hh * (f[l] + ((T(c.hx) * (v[l - 1] + v[l + 1]) + T(c.hy) * (v[l - sy] + v[l + sy])) +
@@ -484,7 +493,7 @@ int dot(const T *a, const T *b, size_t n, Solver<T> &s, double &out, hipStream_t
 template <typename T>
 int smooth(Level<T> &L, const uint8_t *M, int n, bool reverse, hipStream_t st)
 {
-    const dim3 g = grid3(L.d), b(BX, BY);
+    const dim3 g = grid_half(L.d), b(BX, BY);
     const int first = reverse ? ((L.d.nx + 1) & 1) : 0;
     for (int s = 0; s < n; ++s)
         for (int h = 0; h < 2; ++h) {
