@@ -457,7 +457,7 @@ int qsm_vsharp(int dtype, const void *f, const uint8_t *mask, const size_t sz[3]
 
 /* mgopts of mgpcg.m:11-52 (nlevels <= 0: max(1, floor(log2(min(sz))) - 3)). */
 typedef struct qsm_mg_opts {
-    double tol;     /* multigrid tolerance; < 0: fixed cycles (-1) */
+    double tol;     /* multigrid tolerance; <= 0: fixed cycles (-1) */
     int maxit;      /* V-cycles per preconditioner application (1) */
     int mu;         /* 1 V-cycle, 2 W-cycle (1) */
     int npre;       /* pre-smoothing sweeps (1) */
@@ -474,9 +474,11 @@ size_t qsm_mgpcg_work_bytes(int dtype, const size_t sz[3], int nlevels);
 /* Replaces mgpcg.m:1-58 / mgpcg_mex.c: v solves -Lap v = f on the mask
  * (7-point Laplacian of voxel size vsz, v = 0 off the mask and outside the
  * volume) by CG preconditioned with mgopts->maxit multigrid V-cycles,
- * stopping at ||r|| <= tol ||f|| or maxit iterations.  The smoother is
- * red-black Gauss-Seidel (the reference's GAUSS_SEIDEL_RED_BLACK variant)
- * and mgopts->tol is not used (fixed cycles, the reference's default).
+ * stopping at ||r|| <= tol ||f|| (norms over the whole array) or maxit
+ * iterations.  The smoother is red-black Gauss-Seidel (the reference's
+ * GAUSS_SEIDEL_RED_BLACK variant); mgopts->tol > 0 stops the cycles of one
+ * preconditioner application once ||r - A q|| < mgopts->tol ||r|| (mx_mg);
+ * the coarsest level takes a fixed-degree Chebyshev iteration.
  * iters: iterations done; resnorm (may be NULL): maxit doubles, ||r|| per
  * iteration.  Device pointers; synchronises the stream (the CG scalars are
  * read on the host). */
