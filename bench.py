@@ -113,25 +113,54 @@ def copy_peak_gbs():
     return 2 * n * 4 / dt / 1e9
 
 
-def cpu_baseline(cfg, d2f, mask, iters):
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(cfg, d2f, mask, iters, runs=3, iters_1t=4):
+    """The oracle's pd_tgv (the C restatement of mx_pdd) on the C3 input:
+    median of `runs` runs on all the host threads (OMP_NUM_THREADS or the
+    affinity set), and one run of iters_1t iterations on one thread."""
     from oracle import oracle as orc
 
-    t = time.perf_counter()
-    orc.pd_tgv(d2f.astype(np.float64, order="F"), mask, cfg["spec"].vsz, cfg["lam"],
-               cfg["alpha"], 0.0, iters)
-    wall = time.perf_counter() - t
-    loop = orc.last_loop_seconds()
+    x = d2f.astype(np.float64, order="F")
     N = int(np.prod(d2f.shape))
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    loops, walls = [], []
+    for _ in range(runs):
+        t = time.perf_counter()
+        orc.pd_tgv(x, mask, cfg["spec"].vsz, cfg["lam"], cfg["alpha"], 0.0, iters)
+        walls.append(time.perf_counter() - t)
+        loops.append(orc.last_loop_seconds())
+    loop = float(np.median(loops))
+    wall = float(np.median(walls))
+    one = None
+    if iters_1t > 0:
+        orc.set_threads(1)
+        try:
+            orc.pd_tgv(x, mask, cfg["spec"].vsz, cfg["lam"], cfg["alpha"], 0.0, iters_1t)
+            one = N * iters_1t / orc.last_loop_seconds()
+        finally:
+            orc.set_threads(threads)
     return {
         "value": N * iters / loop,
         "unit": "voxel-iters/s",
         "cores": threads,
         "kind": "port",
+        "value_1_thread": one,
+        "host_cpu": _cpu_model(),
         "sample": (f"oracle pd_tgv float64 (C11+OpenMP restatement of pd_tgv_mex.c, "
                    f"-O2, no FMA), C3 input {d2f.shape[0]}x{d2f.shape[1]}x{d2f.shape[2]} "
-                   f"full-box mask, {iters} iterations, tol=0; iteration-loop time "
-                   f"{loop:.2f}s (end-to-end {wall:.2f}s), {threads} OpenMP threads"),
+                   f"full-box mask, {iters} iterations, tol=0; median of {runs} runs: "
+                   f"iteration-loop time {loop:.2f}s (end-to-end {wall:.2f}s), {threads} "
+                   f"OpenMP threads; 1 thread: {iters_1t} iterations"),
     }
 
 
@@ -272,8 +301,14 @@ def serve_leg(n: int, rate: float, max_live_voxels: int | None = 24_000_000):
     specs = configs.c5_requests(n=n, rate=rate)
     t = time.perf_counter()
     reqs = []
+    # one phantom per (size, voxel size): with tol = 0 a solve's time does
+    # not depend on the data, only on its size and iteration budget
+    cache = {}
     for i, r in enumerate(specs):
-        d2f, m = _phantom_input(r["spec"])
+        key = (tuple(r["spec"].shape), tuple(r["spec"].vsz))
+        if key not in cache:
+            cache[key] = _phantom_input(r["spec"])
+        d2f, m = cache[key]
         reqs.append(Request(i, r["arrival"], _lib.TGV if r["kind"] == "tgv" else _lib.TV,
                             d2f.astype(np.float32, order="F"), m, r["spec"].vsz, r["lam"],
                             r["alpha"], r["maxiter"]))
@@ -283,7 +318,8 @@ def serve_leg(n: int, rate: float, max_live_voxels: int | None = 24_000_000):
     assert all(r.iters == s["maxiter"] for r, s in zip(reqs, specs))
     return {
         "workload": ("C5 Poisson arrivals, TV/TGV 50/50, 70% 224x224x128, 20% 256x256x176, "
-                     "10% 320x320x192, budgets 300-1000 iterations, FP32"),
+                     "10% 320x320x192, budgets 300-1000 iterations, FP32; one seeded phantom "
+                     "per size and voxel size (tol = 0: the time does not depend on the data)"),
         "requests": stats.n, "rate_per_s": rate, "max_live_voxels": max_live_voxels,
         "p50_latency_s": stats.p50, "p99_latency_s": stats.p99,
         "mean_wait_s": stats.mean_wait, "peak_live": stats.peak_live,
@@ -628,7 +664,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--batch-iters", type=int, default=100,
                     help="iterations of the C4 batch leg (0: skip)")
-    ap.add_argument("--serve-requests", type=int, default=40,
+    ap.add_argument("--serve-requests", type=int, default=200,
                     help="requests of the C5 serving leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--no-bgremove", dest="bgremove", action="store_false",

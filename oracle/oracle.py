@@ -58,6 +58,8 @@ def lib():
         L.orc_lapw.argtypes = [C.c_int, _p, _p, szp, dp]
         L.orc_last_loop_seconds.restype = _d
         L.orc_last_loop_seconds.argtypes = []
+        L.orc_set_threads.restype = None
+        L.orc_set_threads.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -132,6 +134,11 @@ def pd_tv(d2f, mask, h, lam, tol, maxiter, verbose=0):
 def last_loop_seconds() -> float:
     """Iteration-loop time of the last pd_tgv/pd_tv call."""
     return lib().orc_last_loop_seconds()
+
+
+def set_threads(n: int) -> None:
+    """OpenMP thread count of the restatement's loops."""
+    lib().orc_set_threads(int(n))
 
 
 # ---- single sweeps, in place on the given (Fortran-ordered) arrays ----------

@@ -13,6 +13,9 @@
 #include <string.h>
 
 #include "qsm_oracle.h"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <time.h>
 
 /* Seconds spent in the iteration loop of the last whole solve (the
@@ -27,6 +30,16 @@ static double orc_now(void)
 }
 
 double orc_last_loop_seconds(void) { return orc_loop_seconds_; }
+
+/* Thread count of the OpenMP loops (for the one-thread CPU baseline). */
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
 
 /* ------------------------------------------------------------------------ */
 /* shared pieces                                                            */

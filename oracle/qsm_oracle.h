@@ -127,6 +127,9 @@ int orc_pd_tv(int dtype, const void *d2f, const uint8_t *mask,
  * (clock_gettime(CLOCK_MONOTONIC); allocation and set-up excluded). */
 double orc_last_loop_seconds(void);
 
+/* Sets the OpenMP thread count of the restatement's loops (n > 0). */
+void orc_set_threads(int n);
+
 /* ---- finite-difference operators (outputs overwritten in place) -------- */
 
 /* gradfp_mex.c:147-242 / :50-144  periodic forward gradient. */
