@@ -84,6 +84,22 @@ def time_solver(kind, dtype, cfg, d2f, mask, steps, warmup):
     return {"gpu_s": gpu, "wall_s": wall, "chi": chi, "tiles": tiles}
 
 
+def brain_leg(steps=200, warmup=20):
+    """C3 with the ellipsoidal brain mask (semi-axes 0.40, 0.45, 0.40 of the
+    volume): FP32 TGV on the same 256x256x176 grid, algorithmic bytes
+    N_active * 41 words + N mask bytes per iteration."""
+    from qsm_amd import _lib
+
+    cfg, d2f, mask, _ = make_input("c3_brain")
+    r = time_solver(_lib.TGV, np.float32, cfg, d2f, mask, steps, warmup)
+    N = int(np.prod(d2f.shape))
+    act = int(np.count_nonzero(mask))
+    t = r["gpu_s"] / steps
+    return {"ms_per_iter": t * 1e3, "voxel_iters_per_s": N / t, "n_active": act, "n": N,
+            "active_fraction": act / N, "algorithmic_gbs": (act * 41 * 4 + N) / t / 1e9,
+            "tiles": r["tiles"], "steps": steps}
+
+
 def copy_peak_gbs():
     """Device-to-device bandwidth of the library's plain 16-byte vector copy
     kernel (qsm_copy_dev) over 2 GiB: the achievable-peak reference."""
@@ -669,6 +685,8 @@ def main(argv=None) -> int:
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--no-bgremove", dest="bgremove", action="store_false",
                     help="skip the SHARP / V-SHARP leg")
+    ap.add_argument("--no-brain", dest="brain", action="store_false",
+                    help="skip the C3 brain-mask variant")
     ap.add_argument("--no-nifti", dest="nifti", action="store_false",
                     help="skip the NIfTI IO leg (saveNii / loadNii at C3)")
     ap.add_argument("--no-unwrap", dest="unwrap", action="store_false",
@@ -781,6 +799,11 @@ def main(argv=None) -> int:
         out["bgremove"] = bgremove_leg(cfg, field, mask, cpu=a.cpu_iters > 0)
         log(f"[bench] sharp f32 {out['bgremove']['sharp']['f32']['device_ms']:.2f} ms on device, "
             f"vsharp f32 {out['bgremove']['vsharp']['f32']['device_ms']:.2f} ms")
+    if a.brain:
+        out["c3_brain"] = brain_leg()
+        log(f"[bench] C3 brain mask ({out['c3_brain']['active_fraction']:.0%} active): FP32 TGV "
+            f"{out['c3_brain']['ms_per_iter']:.3f} ms/iter, "
+            f"{out['c3_brain']['algorithmic_gbs']:.0f} GB/s algorithmic")
     if a.unwrap:
         out["unwrap"] = unwrap_leg(cfg, field, mask, cpu=a.cpu_iters > 0)
         log(f"[bench] unwrapLaplacian f32 {out['unwrap']['f32']['device_ms']:.2f} ms on device "
