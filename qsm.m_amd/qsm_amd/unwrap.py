@@ -62,10 +62,11 @@ def mgpcg(f, mask, vsz, tol=None, maxit=None, mgopts=None, return_info=False):
     o = make_mgopts(mgopts)
     v = np.zeros(f.shape, dtype=f.dtype, order="F")
     it = C.c_int(0)
-    res = np.zeros(int(maxit), dtype=np.float64)
+    # the history has maxit entries (numel(f) by default): only when asked
+    res = np.zeros(int(maxit), dtype=np.float64) if return_info else None
     check(lib().qsm_mgpcg(dtype_code(f.dtype), ptr(f), ptr(m), (_sz * 3)(*f.shape), h3(vsz),
                           float(tol), int(maxit), C.byref(o), ptr(v), C.byref(it),
-                          res.ctypes.data_as(_dp)))
+                          res.ctypes.data_as(_dp) if res is not None else None))
     if return_info:
         return v, (it.value, res[: it.value])
     return v
@@ -131,12 +132,15 @@ def mgpcg_dev(f, mask, vsz, tol, maxit, mgopts, v, work=None, stream=None):
     o = make_mgopts(mgopts)
     work = _work(f, shape, o.nlevels, work)
     it = C.c_int(0)
-    res = np.zeros(int(maxit), dtype=np.float64)
+    res = np.zeros(min(int(maxit), 1 << 16), dtype=np.float64)
     sp = stream.cuda_stream if stream is not None else None
+    if int(maxit) > res.size:
+        res = None     # a long history is not kept
     check(lib().qsm_mgpcg_dev(_torch_code(f), f.data_ptr(), mask.data_ptr(), sz3(shape), h3(vsz),
                               float(tol), int(maxit), C.byref(o), v.data_ptr(), work.data_ptr(),
-                              C.byref(it), res.ctypes.data_as(_dp), sp))
-    return it.value, res[: it.value]
+                              C.byref(it), res.ctypes.data_as(_dp) if res is not None else None,
+                              sp))
+    return it.value, (res[: it.value] if res is not None else None)
 
 
 def unwrap_laplacian_dev(u, mask, vsz, uphas, d2u, work=None, stream=None):
