@@ -84,6 +84,37 @@ def time_solver(kind, dtype, cfg, d2f, mask, steps, warmup):
     return {"gpu_s": gpu, "wall_s": wall, "chi": chi, "tiles": tiles}
 
 
+def stream_ceiling_gbs(n=256 * 256 * 176, narr=15):
+    """Bandwidth of the library's 15-arrays-in, 15-out streaming kernel
+    (qsm_stream_dev) at the C3 size: the ceiling of the fused pass's access
+    pattern (its state traffic with none of its arithmetic)."""
+    import ctypes as C
+
+    import torch
+    from qsm_amd import _lib
+    from qsm_amd._lib import check, lib
+
+    _lib.register("qsm_stream_dev", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int,
+                                              C.c_void_p])
+    a = torch.zeros(narr * n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        check(lib().qsm_stream_dev(b.data_ptr(), a.data_ptr(), n, narr, None))
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    reps = 20
+    e0.record()
+    for _ in range(reps):
+        check(lib().qsm_stream_dev(b.data_ptr(), a.data_ptr(), n, narr, None))
+    e1.record()
+    e1.synchronize()
+    dt = e0.elapsed_time(e1) / 1e3 / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * narr * n * 4 / dt / 1e9
+
+
 def brain_leg(steps=200, warmup=20):
     """C3 with the ellipsoidal brain mask (semi-axes 0.40, 0.45, 0.40 of the
     volume): FP32 TGV on the same 256x256x176 grid, algorithmic bytes
@@ -784,6 +815,14 @@ def main(argv=None) -> int:
         out["roofline"]["frac_of_copy_peak"] = ach / out["copy_peak_gbs"]
     except Exception as e:  # measurement aid only
         log(f"[bench] copy peak failed: {e}")
+    try:
+        out["stream15_ceiling_gbs"] = stream_ceiling_gbs()
+        if out["roofline"].get("traffic"):
+            real = out["roofline"]["traffic"] / (out["ms_per_step"] * 1e-3) / 1e9
+            out["roofline"]["traffic_gbs"] = real
+            out["roofline"]["traffic_frac_of_stream15"] = real / out["stream15_ceiling_gbs"]
+    except Exception as e:  # measurement aid only
+        log(f"[bench] stream ceiling failed: {e}")
     if a.dipole:
         out["fd"] = fd_leg()
         log("[bench] fd f32: " + ", ".join(f"{k} {v['gbs']:.0f} GB/s" for k, v in out["fd"]["f32"].items()))

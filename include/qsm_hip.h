@@ -516,6 +516,13 @@ int qsm_unwrap_laplacian(int dtype, const void *phas, const uint8_t *mask, const
  * the HBM traffic counters are calibrated against. */
 int qsm_copy_dev(void *dst, const void *src, size_t bytes, void *stream);
 
+/* Streams narr (1..16) float arrays of n elements (array a at a*n) from in
+ * to out, each read and written once, on stream.  No reference
+ * counterpart: with narr = 15 it is the state traffic of one fused
+ * primal-dual pass without its arithmetic, the bandwidth ceiling of that
+ * access pattern (bench.py reports the fused pass against it). */
+int qsm_stream_dev(void *out, const void *in, size_t n, int narr, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
