@@ -491,6 +491,15 @@ int qsm_mgpcg(int dtype, const void *f, const uint8_t *mask, const size_t sz[3],
               const double vsz[3], double tol, int maxit, const qsm_mg_opts *mgopts,
               void *v, int *iters, double *resnorm);
 
+/* Replaces mgpcg_mex.c:75-122 (mexFunction, mx_pcg): qsm_mgpcg from the
+ * initial guess in v (host; overwritten with the solution): r = f - A v,
+ * the rule still ||r|| <= tol ||f||.  As in the reference, the guess's
+ * values off the mask stay in v and act as boundary values of the stencil
+ * at the mask's edge; maxit = 0 returns v unchanged. */
+int qsm_mgpcg_x0(int dtype, const void *f, const uint8_t *mask, const size_t sz[3],
+                 const double vsz[3], double tol, int maxit, const qsm_mg_opts *mgopts,
+                 void *v, int *iters, double *resnorm);
+
 /* Replaces unwrapLaplacian.m:78-98 (unwrap_) for one echo of a volume
  * already cropped to the mask's box +1 voxel: d2u = lapw(u, vsz), uphas =
  * mgpcg(-mask .* d2u, mask, vsz, 1e-8, ceil(sqrt(numel(mask))), mgopts

@@ -42,7 +42,7 @@ using namespace qsm;
 namespace {
 
 constexpr int BX = 64, BY = 4, NT = BX * BY;
-constexpr int MAX_LEVELS = 16;
+constexpr int MAX_LEVELS = 16;    // at most 16 grid levels (mgpcg_mex.c: MAX_DEPTH)
 constexpr int CHEB_MAX = 512;     // Chebyshev degree cap on the coarsest level
 constexpr double CHEB_EPS = 1e-3; // coarse error reduction it aims for
 constexpr int DOT_BLOCKS = 1024;  // fixed grid of the dot products
@@ -363,6 +363,14 @@ __global__ __launch_bounds__(NT) void xpby_kernel(T *p, const T *q, T beta, size
     if (l < n) p[l] = beta * p[l] + q[l];
 }
 
+// r -= q
+template <typename T>
+__global__ __launch_bounds__(NT) void sub_kernel(T *r, const T *q, size_t n)
+{
+    const size_t l = size_t(blockIdx.x) * NT + threadIdx.x;
+    if (l < n) r[l] = r[l] - q[l];
+}
+
 // x += a p; r -= a q
 template <typename T>
 __global__ __launch_bounds__(NT) void update_kernel(T *x, T *r, const T *p, const T *q, T a,
@@ -587,10 +595,14 @@ int cycle(Solver<T> &s, int l, const MgParams &m, hipStream_t st)
 // Solves -Lap v = f on the mask (sz, unpadded, device), v out (sz).  src:
 // f itself, or with unwrap != 0 the wrapped Laplacian d2u whose right-hand
 // side is -m .* d2u (unwrapLaplacian.m:84).
+//
+// x0 (may be null): the initial guess (sz, device); PCG then starts from
+// r = f - A x0, with the stopping rule still relative to ||f||
+// (mgpcg_mex.c: mx_pcg).
 template <typename T>
 int mgpcg_t(const T *src, bool unwrap, const uint8_t *mask, const size_t sz[3],
             const double vsz[3], double tol, int maxit, const MgParams &m, int nlevels, T *v,
-            void *work, int *iters, double *resnorm, hipStream_t st)
+            void *work, int *iters, double *resnorm, hipStream_t st, const T *x0 = nullptr)
 {
     int rc;
     Solver<T> s;
@@ -611,8 +623,7 @@ int mgpcg_t(const T *src, bool unwrap, const uint8_t *mask, const size_t sz[3],
         band_kernel<<<grid3(L.d), b, 0, st>>>(L.B, s.tmp, L.G, L.d);
         QSM_HIP(hipGetLastError());
     }
-    // PCG (mgpcg_mex.c: mx_pcg), x = 0 so r = f
-    zero_kernel<T><<<g1(n0), NT, 0, st>>>(s.x, n0);
+    // PCG (mgpcg_mex.c: mx_pcg): reltol from ||f||, then r = f - A x0
     zero_kernel<T><<<g1(n0), NT, 0, st>>>(s.p, n0);
     QSM_HIP(hipGetLastError());
     T *r = L0.f, *q = L0.v;
@@ -620,6 +631,17 @@ int mgpcg_t(const T *src, bool unwrap, const uint8_t *mask, const size_t sz[3],
     if ((rc = dot(r, r, n0, s, res, st))) return rc;
     res = std::sqrt(res);
     const double reltol = tol * res;
+    if (x0) {
+        pad_kernel<T><<<grid3(L0.d), b, 0, st>>>(s.x, x0, nullptr, mask, L0.d, T(0));
+        apply_kernel<T><<<grid3(L0.d), b, 0, st>>>(q, s.x, L0.G, L0.d, L0.c);
+        sub_kernel<T><<<g1(n0), NT, 0, st>>>(r, q, n0);
+        QSM_HIP(hipGetLastError());
+        if ((rc = dot(r, r, n0, s, res, st))) return rc;
+        res = std::sqrt(res);
+    } else {
+        zero_kernel<T><<<g1(n0), NT, 0, st>>>(s.x, n0);
+        QSM_HIP(hipGetLastError());
+    }
     double rho = 1.0, prevrho;
     int it = 0;
     for (; it < maxit; ++it) {
@@ -679,7 +701,7 @@ int check_mg(int dtype, const size_t sz[3], const double vsz[3], const qsm_mg_op
     int rc = check_grid(sz, vsz, 1);
     if (rc) return rc;
     L = o->nlevels > 0 ? o->nlevels : default_levels(sz);
-    QSM_ARG(L >= 1 && L < MAX_LEVELS, "mgopts.nlevels %d out of range", L);
+    QSM_ARG(L >= 1 && L <= MAX_LEVELS, "mgopts.nlevels %d out of range 1..%d", L, MAX_LEVELS);
     Dim d[MAX_LEVELS];
     level_dims(sz, L, d);
     QSM_ARG(d[L - 1].nx > 2 && d[L - 1].ny > 2 && d[L - 1].nz > 2,
@@ -709,7 +731,7 @@ size_t qsm_mgpcg_work_bytes(int dtype, const size_t sz[3], int nlevels)
 {
     if (!sz) return 0;
     const int L = nlevels > 0 ? nlevels : default_levels(sz);
-    if (L < 1 || L >= MAX_LEVELS) return 0;
+    if (L < 1 || L > MAX_LEVELS) return 0;
     const double one[3] = {1.0, 1.0, 1.0};
     return dtype == QSM_F32 ? solver_layout<float>(sz, one, L, nullptr, nullptr)
                             : solver_layout<double>(sz, one, L, nullptr, nullptr);
@@ -752,6 +774,41 @@ int qsm_mgpcg(int dtype, const void *f, const uint8_t *mask, const size_t sz[3],
     if ((rc = qsm_mgpcg_dev(dtype, df.p, static_cast<const uint8_t *>(dm.p), sz, vsz, tol, maxit,
                             mgopts, dv.p, dw.p, iters, resnorm, nullptr)))
         return rc;
+    QSM_HIP(hipStreamSynchronize(nullptr));
+    QSM_HIP(hipMemcpy(v, dv.p, N * es, hipMemcpyDeviceToHost));
+    return QSM_OK;
+}
+
+int qsm_mgpcg_x0(int dtype, const void *f, const uint8_t *mask, const size_t sz[3],
+                 const double vsz[3], double tol, int maxit, const qsm_mg_opts *mgopts, void *v,
+                 int *iters, double *resnorm)
+{
+    int L, rc = check_mg(dtype, sz, vsz, mgopts, L);
+    if (rc) return rc;
+    QSM_ARG(f && mask && v, "null pointer");
+    QSM_ARG(maxit >= 0 && std::isfinite(tol), "tol must be finite and maxit >= 0");
+    if ((rc = ensure_device())) return rc;
+    const size_t N = sz[0] * sz[1] * sz[2], es = dtype_size(dtype);
+    DevBuf df, dm, dv, dx, dw;
+    if ((rc = df.alloc(N * es)) || (rc = dm.alloc(N)) || (rc = dv.alloc(N * es))
+        || (rc = dx.alloc(N * es)) || (rc = dw.alloc(qsm_mgpcg_work_bytes(dtype, sz, L))))
+        return rc;
+    QSM_HIP(hipMemcpy(df.p, f, N * es, hipMemcpyHostToDevice));
+    QSM_HIP(hipMemcpy(dm.p, mask, N, hipMemcpyHostToDevice));
+    QSM_HIP(hipMemcpy(dx.p, v, N * es, hipMemcpyHostToDevice));
+    const MgParams m{mgopts->tol, mgopts->maxit, mgopts->mu, mgopts->npre, mgopts->npost,
+                     mgopts->nboundary};
+    const hipStream_t st = nullptr;
+    rc = dtype == QSM_F64
+        ? mgpcg_t<double>(static_cast<const double *>(df.p), false,
+                          static_cast<const uint8_t *>(dm.p), sz, vsz, tol, maxit, m, L,
+                          static_cast<double *>(dv.p), dw.p, iters, resnorm, st,
+                          static_cast<const double *>(dx.p))
+        : mgpcg_t<float>(static_cast<const float *>(df.p), false,
+                         static_cast<const uint8_t *>(dm.p), sz, vsz, tol, maxit, m, L,
+                         static_cast<float *>(dv.p), dw.p, iters, resnorm, st,
+                         static_cast<const float *>(dx.p));
+    if (rc) return rc;
     QSM_HIP(hipStreamSynchronize(nullptr));
     QSM_HIP(hipMemcpy(v, dv.p, N * es, hipMemcpyDeviceToHost));
     return QSM_OK;

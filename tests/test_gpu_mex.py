@@ -98,3 +98,51 @@ def test_fd_mex_in_place(cuda, dt):
     _same(d2u.numpy(), orc.lapw(u, H))
     for o in (*outs, du, d2u):
         o.free()
+
+
+def _ball_mask(shape, r):
+    g = np.meshgrid(*(np.arange(n) - (n - 1) / 2 for n in shape), indexing="ij")
+    return np.asfortranarray((g[0] ** 2 + g[1] ** 2 + g[2] ** 2) <= r * r)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_mgpcg_mex_zero_guess_is_mgpcg(cuda, dt):
+    """mgpcg.m calls mgpcg_mex with a zero guess and mgopts' fields: the
+    result is bitwise the Python mirror's (qsm_mgpcg) and the residual table
+    is printed with verbose."""
+    from qsm_amd import unwrap as uw
+
+    shape = (40, 36, 30)
+    m = _ball_mask(shape, 13)
+    f = np.asfortranarray((np.random.default_rng(0).standard_normal(shape) * m).astype(dt))
+    (v,), text = call("mgpcg_mex", 1, np.zeros(shape, dt), f, m, np.array(H), 1e-8, 100,
+                      -1.0, 2, 1, 2, 2, 2, 2, 1)
+    want = uw.mgpcg(f, m, H, 1e-8, 100, {"maxit": 2, "npre": 2, "npost": 2, "nboundary": 2,
+                                         "nlevels": 2})
+    _same(v, want)
+    lines = text.strip().splitlines()
+    assert lines[0].split() == ["iter", "residual"] and len(lines) > 2
+
+
+def test_mgpcg_mex_initial_guess_is_boundary_data(cuda):
+    """A nonzero guess (mgpcg_mex.c: mx_pcg, r = f - A v): on the mask the
+    result solves the stencil with the guess's off-mask values as boundary
+    data, which it keeps; maxit 0 returns the guess."""
+    from oracle import poisson as po
+
+    shape = (30, 28, 26)
+    m = _ball_mask(shape, 10)
+    rng = np.random.default_rng(1)
+    f = np.asfortranarray(rng.standard_normal(shape) * m)
+    g = np.asfortranarray(rng.standard_normal(shape))
+    (v0,), _ = call("mgpcg_mex", 1, g, f, m, np.array(H), 1e-12, 0, -1.0, 1, 1, 1, 1, 2, 0, 0)
+    _same(v0, g)
+    (v,), _ = call("mgpcg_mex", 1, g, f, m, np.array(H), 1e-12, 200, -1.0, 1, 1, 1, 1, 2, 0, 0)
+    np.testing.assert_array_equal(v[~m], g[~m])
+    # the same problem with the boundary data moved to the right-hand side
+    hx, hy, hz = (1.0 / (h * h) for h in H)
+    b = np.pad(np.where(m, 0.0, g), 1)
+    lift = (hx * (b[2:, 1:-1, 1:-1] + b[:-2, 1:-1, 1:-1]) + hy * (b[1:-1, 2:, 1:-1] + b[1:-1, :-2, 1:-1])
+            + hz * (b[1:-1, 1:-1, 2:] + b[1:-1, 1:-1, :-2]))
+    want = po.solve(np.where(m, f + lift, 0.0), m, H)
+    np.testing.assert_allclose(v[m], want[m], rtol=0, atol=1e-9 * np.abs(want).max())

@@ -2,7 +2,7 @@
 here is rejected before any library call).  Usage strings are the
 reference's (pd_tgv_mex.c:107-112, pd_tv_mex.c:103-108, gradfp_mex.c:16-19,
 gradfp_adj_mex.c:18-21, gradfm_mex.c:18-21, gradfm_adj_mex.c:18-21,
-lapw_mex.c:24-27)."""
+lapw_mex.c:24-27, mgpcg_mex.c:78-82)."""
 import numpy as np
 import pytest
 
@@ -20,6 +20,8 @@ USAGE = {
     "gradfm_mex": ("Usage: gradfm_mex(dx, dy, dz, u, G, h);", 6),
     "gradfm_adj_mex": ("Usage: gradfm_adj_mex(du, x, y, z, G, h);", 6),
     "lapw_mex": ("Usage: lapw_mex(d2u, u, h);", 3),
+    "mgpcg_mex": ("Usage: v = mgpcg_mex(v, f, mask, vsz, tolcg, maxitcg, tolmg, maxitmg, mu, "
+                  "npre, npost, nboundary, nlevels, verbose)", 14),
 }
 
 
@@ -56,3 +58,14 @@ def test_fd_rejects_mixed_classes():
         call("gradfp_mex", 0, z, z, z.astype(np.float32), z, np.ones(3))
     with pytest.raises(MexError, match="single or double"):
         call("lapw_mex", 0, z.astype(np.int32), z.astype(np.int32), np.ones(3))
+
+
+def test_mgpcg_mex_rejects_too_many_levels_and_bad_classes():
+    z = np.zeros((6, 6, 6))
+    args = [z, z, np.ones(z.shape, bool), np.ones(3), 1e-6, 10, -1.0, 1, 1, 1, 1, 2]
+    with pytest.raises(MexError, match="MAX_DEPTH"):
+        call("mgpcg_mex", 1, *args, 17, 0)
+    with pytest.raises(MexError, match="same class"):
+        call("mgpcg_mex", 1, z, z.astype(np.float32), *args[2:], 0, 0)
+    with pytest.raises(MexError, match="mask"):
+        call("mgpcg_mex", 1, z, z, np.ones(z.shape), *args[3:], 0, 0)
