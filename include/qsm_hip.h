@@ -206,10 +206,50 @@ int qsm_gradf_adj(int dtype, void *du, const void *x, const void *y,
 int qsm_gradfm_adj(int dtype, void *du, const void *x, const void *y,
                    const void *z, const uint8_t *G,
                    const size_t sz[3], const double h[3]);
+
+/* Replace src/utils/fd/mex/gradb_mex.c:142-226 (backward differences,
+ * forward on the first face), gradbm_mex.c:106-154 (masked: backward if the
+ * -1 neighbour is in G, else forward, else 0; outside G untouched),
+ * gradb_adj_mex.c:159-251 (-1/h forward differences summed (X + Y) + Z,
+ * backward on the last face; (nx-1, ny-1, nz-1) is never written, as in the
+ * reference), gradbm_adj_mex.c:110-162, gradc_mex.c:309-554 (central
+ * differences, one-sided on the faces; the reference's k = 0 loop that
+ * steps j by 1 rewrites l in [nx-1+nx, nx-1+nx(ny-1)) with dx backward, dy
+ * central, dz forward, and (nx-1, ny-1, nz-1) is never written; both kept)
+ * and gradcm_mex.c:122-186 (masked central differences).  Caller-allocated
+ * outputs, overwritten where the reference writes; the unmasked forms need
+ * every dimension >= 2.  Synchronous. */
+int qsm_gradb(int dtype, void *dx, void *dy, void *dz, const void *u,
+              const size_t sz[3], const double h[3]);
+int qsm_gradbm(int dtype, void *dx, void *dy, void *dz, const void *u, const uint8_t *G,
+               const size_t sz[3], const double h[3]);
+int qsm_gradb_adj(int dtype, void *du, const void *x, const void *y, const void *z,
+                  const size_t sz[3], const double h[3]);
+int qsm_gradbm_adj(int dtype, void *du, const void *x, const void *y, const void *z,
+                   const uint8_t *G, const size_t sz[3], const double h[3]);
+int qsm_gradc(int dtype, void *dx, void *dy, void *dz, const void *u,
+              const size_t sz[3], const double h[3]);
+int qsm_gradcm(int dtype, void *dx, void *dy, void *dz, const void *u, const uint8_t *G,
+               const size_t sz[3], const double h[3]);
+
 /* lapw_mex.c:21-51 -> lapwd/lapwf (:112-187 / :54-109).  Interior voxels
  * are written; the faces of d2u are left as the caller passed them. */
 int qsm_lapw(int dtype, void *d2u, const void *u,
              const size_t sz[3], const double h[3]);
+
+/* Device forms of the backward and central differences above. */
+int qsm_gradb_dev(int dtype, void *dx, void *dy, void *dz, const void *u,
+                  const size_t sz[3], const double h[3], void *stream);
+int qsm_gradbm_dev(int dtype, void *dx, void *dy, void *dz, const void *u, const uint8_t *G,
+                   const size_t sz[3], const double h[3], void *stream);
+int qsm_gradb_adj_dev(int dtype, void *du, const void *x, const void *y, const void *z,
+                      const size_t sz[3], const double h[3], void *stream);
+int qsm_gradbm_adj_dev(int dtype, void *du, const void *x, const void *y, const void *z,
+                       const uint8_t *G, const size_t sz[3], const double h[3], void *stream);
+int qsm_gradc_dev(int dtype, void *dx, void *dy, void *dz, const void *u,
+                  const size_t sz[3], const double h[3], void *stream);
+int qsm_gradcm_dev(int dtype, void *dx, void *dy, void *dz, const void *u, const uint8_t *G,
+                   const size_t sz[3], const double h[3], void *stream);
 
 /* Device-pointer forms of the operators above (asynchronous on stream). */
 int qsm_gradfp_dev(int dtype, void *dx, void *dy, void *dz, const void *u,

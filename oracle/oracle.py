@@ -54,6 +54,10 @@ def lib():
         for n in ("gradfp_adj", "gradf_adj"):
             getattr(L, f"orc_{n}").argtypes = [C.c_int, _p, _p, _p, _p, szp, dp]
         L.orc_gradfm.argtypes = [C.c_int, _p, _p, _p, _p, _p, szp, dp]
+        for n in ("gradb", "gradc", "gradb_adj"):
+            getattr(L, f"orc_{n}").argtypes = [C.c_int, _p, _p, _p, _p, szp, dp]
+        for n in ("gradbm", "gradcm", "gradbm_adj"):
+            getattr(L, f"orc_{n}").argtypes = [C.c_int, _p, _p, _p, _p, _p, szp, dp]
         L.orc_gradfm_adj.argtypes = [C.c_int, _p, _p, _p, _p, _p, szp, dp]
         L.orc_lapw.argtypes = [C.c_int, _p, _p, szp, dp]
         L.orc_last_loop_seconds.restype = _d
@@ -224,6 +228,50 @@ def gradfm_adj(x, y, z, G, h, init=None):
     g = _F(np.asarray(G) != 0, np.uint8)
     du = _z(x) if init is None else _F(init).copy(order="F")
     lib().orc_gradfm_adj(_code(x.dtype), _ptr(du), _ptr(x), _ptr(y), _ptr(z), _ptr(g),
+                         _sz3(x.shape), _h3(h))
+    return du
+
+
+def _vec3(name, u, h, G=None, init=None):
+    u = _F(u)
+    d = [_z(u) if init is None else _F(init[c]).copy(order="F") for c in range(3)]
+    if G is None:
+        getattr(lib(), name)(_code(u.dtype), *map(_ptr, d), _ptr(u), _sz3(u.shape), _h3(h))
+    else:
+        g = _F(np.asarray(G) != 0, np.uint8)
+        getattr(lib(), name)(_code(u.dtype), *map(_ptr, d), _ptr(u), _ptr(g), _sz3(u.shape),
+                             _h3(h))
+    return tuple(d)
+
+
+def gradb(u, h, init=None):
+    return _vec3("orc_gradb", u, h, None, init)
+
+
+def gradbm(u, G, h, init=None):
+    return _vec3("orc_gradbm", u, h, G, init)
+
+
+def gradc(u, h, init=None):
+    return _vec3("orc_gradc", u, h, None, init)
+
+
+def gradcm(u, G, h, init=None):
+    return _vec3("orc_gradcm", u, h, G, init)
+
+
+def gradb_adj(x, y, z, h, init=None):
+    x, y, z = _F(x), _F(y), _F(z)
+    du = _z(x) if init is None else _F(init).copy(order="F")
+    lib().orc_gradb_adj(_code(x.dtype), _ptr(du), _ptr(x), _ptr(y), _ptr(z), _sz3(x.shape), _h3(h))
+    return du
+
+
+def gradbm_adj(x, y, z, G, h, init=None):
+    x, y, z = _F(x), _F(y), _F(z)
+    g = _F(np.asarray(G) != 0, np.uint8)
+    du = _z(x) if init is None else _F(init).copy(order="F")
+    lib().orc_gradbm_adj(_code(x.dtype), _ptr(du), _ptr(x), _ptr(y), _ptr(z), _ptr(g),
                          _sz3(x.shape), _h3(h))
     return du
 

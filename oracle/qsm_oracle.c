@@ -241,6 +241,48 @@ void orc_gradf_adj(int dtype, void *du, const void *x, const void *y,
     else gradf_adj_f(du, x, y, z, sz, h);
 }
 
+void orc_gradb(int dtype, void *dx, void *dy, void *dz, const void *u,
+               const size_t sz[3], const double h[3])
+{
+    if (dtype == 0) gradb_d(dx, dy, dz, u, sz, h);
+    else gradb_f(dx, dy, dz, u, sz, h);
+}
+
+void orc_gradbm(int dtype, void *dx, void *dy, void *dz, const void *u,
+                const uint8_t *G, const size_t sz[3], const double h[3])
+{
+    if (dtype == 0) gradbm_d(dx, dy, dz, u, G, sz, h);
+    else gradbm_f(dx, dy, dz, u, G, sz, h);
+}
+
+void orc_gradb_adj(int dtype, void *du, const void *x, const void *y,
+                   const void *z, const size_t sz[3], const double h[3])
+{
+    if (dtype == 0) gradb_adj_d(du, x, y, z, sz, h);
+    else gradb_adj_f(du, x, y, z, sz, h);
+}
+
+void orc_gradbm_adj(int dtype, void *du, const void *x, const void *y,
+                    const void *z, const uint8_t *G, const size_t sz[3], const double h[3])
+{
+    if (dtype == 0) gradbm_adj_d(du, x, y, z, G, sz, h);
+    else gradbm_adj_f(du, x, y, z, G, sz, h);
+}
+
+void orc_gradc(int dtype, void *dx, void *dy, void *dz, const void *u,
+               const size_t sz[3], const double h[3])
+{
+    if (dtype == 0) gradc_d(dx, dy, dz, u, sz, h);
+    else gradc_f(dx, dy, dz, u, sz, h);
+}
+
+void orc_gradcm(int dtype, void *dx, void *dy, void *dz, const void *u,
+                const uint8_t *G, const size_t sz[3], const double h[3])
+{
+    if (dtype == 0) gradcm_d(dx, dy, dz, u, G, sz, h);
+    else gradcm_f(dx, dy, dz, u, G, sz, h);
+}
+
 void orc_gradfm_adj(int dtype, void *du, const void *x, const void *y,
                     const void *z, const uint8_t *G,
                     const size_t sz[3], const double h[3])

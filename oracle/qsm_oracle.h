@@ -148,6 +148,21 @@ void orc_gradfm(int dtype, void *dx, void *dy, void *dz, const void *u,
 void orc_gradf_adj(int dtype, void *du, const void *x, const void *y,
                    const void *z, const size_t sz[3], const double h[3]);
 /* gradfm_adj_mex.c:109-162 / :53-106  masked variant. */
+/* gradb_mex.c, gradbm_mex.c, gradb_adj_mex.c, gradbm_adj_mex.c, gradc_mex.c,
+ * gradcm_mex.c (src/utils/fd/mex): backward and central differences. */
+void orc_gradb(int dtype, void *dx, void *dy, void *dz, const void *u,
+               const size_t sz[3], const double h[3]);
+void orc_gradbm(int dtype, void *dx, void *dy, void *dz, const void *u,
+                const uint8_t *G, const size_t sz[3], const double h[3]);
+void orc_gradb_adj(int dtype, void *du, const void *x, const void *y,
+                   const void *z, const size_t sz[3], const double h[3]);
+void orc_gradbm_adj(int dtype, void *du, const void *x, const void *y,
+                    const void *z, const uint8_t *G, const size_t sz[3], const double h[3]);
+void orc_gradc(int dtype, void *dx, void *dy, void *dz, const void *u,
+               const size_t sz[3], const double h[3]);
+void orc_gradcm(int dtype, void *dx, void *dy, void *dz, const void *u,
+                const uint8_t *G, const size_t sz[3], const double h[3]);
+
 void orc_gradfm_adj(int dtype, void *du, const void *x, const void *y,
                     const void *z, const uint8_t *G,
                     const size_t sz[3], const double h[3]);

@@ -1,5 +1,6 @@
 // fd.hip -- finite-difference operators of QSM.m (src/inversion/mex/gradfp*,
-// src/utils/fd/mex/grad{f,fm}{,_adj}_mex.c) and the wrapped-phase Laplacian
+// src/utils/fd/mex/grad{f,fm,b,bm}{,_adj}_mex.c, gradc{,m}_mex.c) and the
+// wrapped-phase Laplacian
 // (src/unwrap/mex/lapw_mex.c) for gfx950.
 //
 // One thread per voxel, x fastest, so every wavefront reads and writes 64
@@ -143,6 +144,131 @@ __global__ __launch_bounds__(FX *FY) void gradfm_adj_kernel(T *du, const T *x,
     du[l] = (DX + DY) + DZ;
 }
 
+// gradb_mex.c:142-226: backward differences, forward on the first face.
+template <typename T>
+__global__ __launch_bounds__(FX *FY) void gradb_kernel(T *dx, T *dy, T *dz,
+                                                       const T *u, G3 g,
+                                                       T hx, T hy, T hz)
+{
+    int i, j, k, l;
+    if (!vox(g, i, j, k, l)) return;
+    const int sy = g.nx, sk = g.nx * g.ny;
+    const T c = u[l];
+    dx[l] = (i > 0) ? hx * (c - u[l - 1]) : hx * (u[l + 1] - c);
+    dy[l] = (j > 0) ? hy * (c - u[l - sy]) : hy * (u[l + sy] - c);
+    dz[l] = (k > 0) ? hz * (c - u[l - sk]) : hz * (u[l + sk] - c);
+}
+
+// gradbm_mex.c:121-150: inside G, backward if the -1 neighbour is in G,
+// else forward if the +1 neighbour is, else 0; outside G untouched.
+template <typename T>
+__global__ __launch_bounds__(FX *FY) void gradbm_kernel(T *dx, T *dy, T *dz,
+                                                        const T *u,
+                                                        const uint8_t *G, G3 g,
+                                                        T hx, T hy, T hz)
+{
+    int i, j, k, l;
+    if (!vox(g, i, j, k, l)) return;
+    if (!G[l]) return;
+    const int sy = g.nx, sk = g.nx * g.ny;
+    const T c = u[l];
+    dx[l] = (i > 0 && G[l - 1]) ? hx * (c - u[l - 1])
+          : (i + 1 < g.nx && G[l + 1]) ? hx * (u[l + 1] - c) : T(0);
+    dy[l] = (j > 0 && G[l - sy]) ? hy * (c - u[l - sy])
+          : (j + 1 < g.ny && G[l + sy]) ? hy * (u[l + sy] - c) : T(0);
+    dz[l] = (k > 0 && G[l - sk]) ? hz * (c - u[l - sk])
+          : (k + 1 < g.nz && G[l + sk]) ? hz * (u[l + sk] - c) : T(0);
+}
+
+// gradb_adj_mex.c:159-251: (X+Y)+Z, forward differences scaled by -1/h,
+// backward on the last face; the reference's loops never write (nx-1,
+// ny-1, nz-1), which keeps the caller's value.
+template <typename T>
+__global__ __launch_bounds__(FX *FY) void gradb_adj_kernel(T *du, const T *x,
+                                                           const T *y, const T *z,
+                                                           G3 g, T hx, T hy, T hz)
+{
+    int i, j, k, l;
+    if (!vox(g, i, j, k, l)) return;
+    if (i + 1 == g.nx && j + 1 == g.ny && k + 1 == g.nz) return;
+    const int sy = g.nx, sk = g.nx * g.ny;
+    const T X = (i + 1 < g.nx) ? hx * (x[l + 1] - x[l]) : hx * (x[l] - x[l - 1]);
+    const T Y = (j + 1 < g.ny) ? hy * (y[l + sy] - y[l]) : hy * (y[l] - y[l - sy]);
+    const T Z = (k + 1 < g.nz) ? hz * (z[l + sk] - z[l]) : hz * (z[l] - z[l - sk]);
+    du[l] = (X + Y) + Z;
+}
+
+// gradbm_adj_mex.c:131-157: inside G, forward if the +1 neighbour is in G,
+// else backward if the -1 neighbour is, else 0; du = (dx+dy)+dz.
+template <typename T>
+__global__ __launch_bounds__(FX *FY) void gradbm_adj_kernel(T *du, const T *x,
+                                                            const T *y, const T *z,
+                                                            const uint8_t *G, G3 g,
+                                                            T hx, T hy, T hz)
+{
+    int i, j, k, l;
+    if (!vox(g, i, j, k, l)) return;
+    if (!G[l]) return;
+    const int sy = g.nx, sk = g.nx * g.ny;
+    const T DX = (i + 1 < g.nx && G[l + 1]) ? hx * (x[l + 1] - x[l])
+               : (i > 0 && G[l - 1]) ? hx * (x[l] - x[l - 1]) : T(0);
+    const T DY = (j + 1 < g.ny && G[l + sy]) ? hy * (y[l + sy] - y[l])
+               : (j > 0 && G[l - sy]) ? hy * (y[l] - y[l - sy]) : T(0);
+    const T DZ = (k + 1 < g.nz && G[l + sk]) ? hz * (z[l + sk] - z[l])
+               : (k > 0 && G[l - sk]) ? hz * (z[l] - z[l - sk]) : T(0);
+    du[l] = (DX + DY) + DZ;
+}
+
+// gradc_mex.c:309-554: per axis the central difference (0.5/h), forward on
+// the first face, backward on the last.  As the reference's loops leave it:
+// in plane k = 0 every l in [NX + nx, NX + NY) (NX = nx-1, NY = nx(ny-1))
+// holds what its "i = nx-1" loop writes last (it steps j by 1: dx backward,
+// dy central, dz forward), and (nx-1, ny-1, nz-1) is never written.
+template <typename T>
+__global__ __launch_bounds__(FX *FY) void gradc_kernel(T *dx, T *dy, T *dz, const T *u, G3 g,
+                                                       T hx, T hy, T hz, T hx2, T hy2, T hz2)
+{
+    int i, j, k, l;
+    if (!vox(g, i, j, k, l)) return;
+    if (i + 1 == g.nx && j + 1 == g.ny && k + 1 == g.nz) return;
+    const int sy = g.nx, sk = g.nx * g.ny;
+    if (k == 0 && l >= (g.nx - 1) + g.nx && l < (g.nx - 1) + g.nx * (g.ny - 1)) {
+        dx[l] = hx * (u[l] - u[l - 1]);
+        dy[l] = hy2 * (u[l + sy] - u[l - sy]);
+        dz[l] = hz * (u[l + sk] - u[l]);
+        return;
+    }
+    dx[l] = (i == 0) ? hx * (u[l + 1] - u[l])
+          : (i + 1 == g.nx) ? hx * (u[l] - u[l - 1]) : hx2 * (u[l + 1] - u[l - 1]);
+    dy[l] = (j == 0) ? hy * (u[l + sy] - u[l])
+          : (j + 1 == g.ny) ? hy * (u[l] - u[l - sy]) : hy2 * (u[l + sy] - u[l - sy]);
+    dz[l] = (k == 0) ? hz * (u[l + sk] - u[l])
+          : (k + 1 == g.nz) ? hz * (u[l] - u[l - sk]) : hz2 * (u[l + sk] - u[l - sk]);
+}
+
+// gradcm_mex.c:122-186: inside G, central if both neighbours are in G, else
+// forward if the +1 neighbour is, else backward if the -1 neighbour is,
+// else 0; outside G untouched.
+template <typename T>
+__global__ __launch_bounds__(FX *FY) void gradcm_kernel(T *dx, T *dy, T *dz, const T *u,
+                                                        const uint8_t *G, G3 g, T hx, T hy,
+                                                        T hz, T hx2, T hy2, T hz2)
+{
+    int i, j, k, l;
+    if (!vox(g, i, j, k, l)) return;
+    if (!G[l]) return;
+    const int sy = g.nx, sk = g.nx * g.ny;
+    const bool xm = i > 0 && G[l - 1], xp = i + 1 < g.nx && G[l + 1];
+    const bool ym = j > 0 && G[l - sy], yp = j + 1 < g.ny && G[l + sy];
+    const bool zm = k > 0 && G[l - sk], zp = k + 1 < g.nz && G[l + sk];
+    dx[l] = (xm && xp) ? hx2 * (u[l + 1] - u[l - 1])
+          : xp ? hx * (u[l + 1] - u[l]) : xm ? hx * (u[l] - u[l - 1]) : T(0);
+    dy[l] = (ym && yp) ? hy2 * (u[l + sy] - u[l - sy])
+          : yp ? hy * (u[l + sy] - u[l]) : ym ? hy * (u[l] - u[l - sy]) : T(0);
+    dz[l] = (zm && zp) ? hz2 * (u[l + sk] - u[l - sk])
+          : zp ? hz * (u[l + sk] - u[l]) : zm ? hz * (u[l] - u[l - sk]) : T(0);
+}
+
 __device__ __forceinline__ double fmod_t(double a, double b) { return fmod(a, b); }
 __device__ __forceinline__ float fmod_t(float a, float b) { return fmodf(a, b); }
 
@@ -213,7 +339,19 @@ __global__ __launch_bounds__(FX *FY) void lapw_kernel(T *du, const T *u, G3 g,
 
 dim3 fd_grid(G3 g) { return dim3((g.nx + FX - 1) / FX, (g.ny + FY - 1) / FY, g.nz); }
 
-enum Op { GRADFP, GRADFP_ADJ, GRADF, GRADFM, GRADF_ADJ, GRADFM_ADJ, LAPW };
+enum Op { GRADFP, GRADFP_ADJ, GRADF, GRADFM, GRADF_ADJ, GRADFM_ADJ, LAPW,
+          GRADB, GRADBM, GRADB_ADJ, GRADBM_ADJ, GRADC, GRADCM };
+
+bool is_adj(Op op)
+{
+    return op == GRADFP_ADJ || op == GRADF_ADJ || op == GRADFM_ADJ || op == GRADB_ADJ ||
+           op == GRADBM_ADJ;
+}
+
+bool is_masked(Op op)
+{
+    return op == GRADFM || op == GRADFM_ADJ || op == GRADBM || op == GRADBM_ADJ || op == GRADCM;
+}
 
 // Coefficients as the reference forms them: 1/h (gradients), -1/h (adjoints),
 // 1/h^2 (lapw), in double and cast on the single path.
@@ -222,7 +360,7 @@ void coefs(Op op, const double h[3], T c[3])
 {
     for (int a = 0; a < 3; ++a) {
         if (op == LAPW) c[a] = T(1.0 / (h[a] * h[a]));
-        else if (op == GRADFP_ADJ || op == GRADF_ADJ || op == GRADFM_ADJ) c[a] = T(-1.0 / h[a]);
+        else if (is_adj(op)) c[a] = T(-1.0 / h[a]);
         else c[a] = T(1.0 / h[a]);
     }
 }
@@ -244,11 +382,30 @@ hipError_t launch_op(Op op, T *s, T *a0, T *a1, T *a2, const uint8_t *G,
     case GRADF_ADJ: gradf_adj_kernel<T><<<grid, blk, 0, st>>>(s, a0, a1, a2, g, c[0], c[1], c[2]); break;
     case GRADFM_ADJ: gradfm_adj_kernel<T><<<grid, blk, 0, st>>>(s, a0, a1, a2, G, g, c[0], c[1], c[2]); break;
     case LAPW: lapw_kernel<T><<<grid, blk, 0, st>>>(a0, s, g, c[0], c[1], c[2]); break;
+    case GRADB: gradb_kernel<T><<<grid, blk, 0, st>>>(a0, a1, a2, s, g, c[0], c[1], c[2]); break;
+    case GRADBM: gradbm_kernel<T><<<grid, blk, 0, st>>>(a0, a1, a2, s, G, g, c[0], c[1], c[2]); break;
+    case GRADB_ADJ: gradb_adj_kernel<T><<<grid, blk, 0, st>>>(s, a0, a1, a2, g, c[0], c[1], c[2]); break;
+    case GRADBM_ADJ: gradbm_adj_kernel<T><<<grid, blk, 0, st>>>(s, a0, a1, a2, G, g, c[0], c[1], c[2]); break;
+    case GRADC:
+    case GRADCM: {
+        // 0.5/h formed in double and cast, as gradc_mex.c / gradcm_mex.c do
+        const T c2[3] = {T(0.5 / h[0]), T(0.5 / h[1]), T(0.5 / h[2])};
+        if (op == GRADC)
+            gradc_kernel<T><<<grid, blk, 0, st>>>(a0, a1, a2, s, g, c[0], c[1], c[2], c2[0], c2[1], c2[2]);
+        else
+            gradcm_kernel<T><<<grid, blk, 0, st>>>(a0, a1, a2, s, G, g, c[0], c[1], c[2], c2[0], c2[1], c2[2]);
+        break;
+    }
     }
     return hipGetLastError();
 }
 
-size_t min_dim(Op op) { return (op == GRADF || op == GRADF_ADJ) ? 2 : 1; }
+// the unmasked one-sided forms read the +/-1 neighbour on every face
+size_t min_dim(Op op)
+{
+    return (op == GRADF || op == GRADF_ADJ || op == GRADB || op == GRADB_ADJ || op == GRADC) ? 2
+                                                                                           : 1;
+}
 
 int run_dev(Op op, int dtype, void *s, void *a0, void *a1, void *a2,
             const uint8_t *G, const size_t sz[3], const double h[3], void *stream)
@@ -257,7 +414,7 @@ int run_dev(Op op, int dtype, void *s, void *a0, void *a1, void *a2,
     if (rc) return rc;
     QSM_ARG(dtype == QSM_F64 || dtype == QSM_F32, "bad dtype");
     QSM_ARG(s && a0 && (op == LAPW || (a1 && a2)), "null array pointer");
-    QSM_ARG(!(op == GRADFM || op == GRADFM_ADJ) || G, "null mask pointer");
+    QSM_ARG(!is_masked(op) || G, "null mask pointer");
     const G3 g{int(sz[0]), int(sz[1]), int(sz[2])};
     if (dtype == QSM_F64)
         QSM_HIP(launch_op<double>(op, (double *)s, (double *)a0, (double *)a1, (double *)a2, G, g, h, as_stream(stream)));
@@ -278,11 +435,15 @@ int run_host(Op op, int dtype, void *s, void *a0, void *a1, void *a2,
     QSM_ARG(dtype == QSM_F64 || dtype == QSM_F32, "bad dtype");
     const bool lap = op == LAPW;
     QSM_ARG(s && a0 && (lap || (a1 && a2)), "null array pointer");
-    const bool masked = op == GRADFM || op == GRADFM_ADJ;
+    const bool masked = is_masked(op);
     QSM_ARG(!masked || G, "null mask pointer");
+    // outputs with voxels the operator leaves untouched (besides the masked
+    // forms): the corner gradb_adj / gradc never write
+    const bool keep = masked || op == GRADB_ADJ || op == GRADC;
     if ((rc = ensure_device())) return rc;
     const size_t N = sz[0] * sz[1] * sz[2], bytes = N * dtype_size(dtype);
-    const bool vec_out = op == GRADFP || op == GRADF || op == GRADFM;
+    const bool vec_out = op == GRADFP || op == GRADF || op == GRADFM || op == GRADB ||
+                         op == GRADBM || op == GRADC || op == GRADCM;
     const int nv = lap ? 1 : 3;
     DevBuf ds, dv[3], dg;
     if ((rc = ds.alloc(bytes))) return rc;
@@ -290,10 +451,10 @@ int run_host(Op op, int dtype, void *s, void *a0, void *a1, void *a2,
         if ((rc = dv[v].alloc(bytes))) return rc;
     void *hv[3] = {a0, a1, a2};
     // scalar array: input of gradients; output of adjoints (staged when masked)
-    if (vec_out || masked || lap)
+    if (vec_out || keep || lap)
         QSM_HIP(hipMemcpy(ds.p, s, bytes, hipMemcpyHostToDevice));
     for (int v = 0; v < nv; ++v)
-        if (!vec_out || masked || lap)
+        if (!vec_out || keep || lap)
             QSM_HIP(hipMemcpy(dv[v].p, hv[v], bytes, hipMemcpyHostToDevice));
     if (masked) {
         if ((rc = dg.alloc(N))) return rc;
@@ -355,6 +516,44 @@ int qsm_gradfm_adj(int dtype, void *du, const void *x, const void *y,
                     const_cast<void *>(z), G, sz, h);
 }
 
+int qsm_gradb(int dtype, void *dx, void *dy, void *dz, const void *u,
+              const size_t sz[3], const double h[3])
+{
+    return run_host(GRADB, dtype, const_cast<void *>(u), dx, dy, dz, nullptr, sz, h);
+}
+
+int qsm_gradbm(int dtype, void *dx, void *dy, void *dz, const void *u, const uint8_t *G,
+               const size_t sz[3], const double h[3])
+{
+    return run_host(GRADBM, dtype, const_cast<void *>(u), dx, dy, dz, G, sz, h);
+}
+
+int qsm_gradb_adj(int dtype, void *du, const void *x, const void *y, const void *z,
+                  const size_t sz[3], const double h[3])
+{
+    return run_host(GRADB_ADJ, dtype, du, const_cast<void *>(x), const_cast<void *>(y),
+                    const_cast<void *>(z), nullptr, sz, h);
+}
+
+int qsm_gradbm_adj(int dtype, void *du, const void *x, const void *y, const void *z,
+                   const uint8_t *G, const size_t sz[3], const double h[3])
+{
+    return run_host(GRADBM_ADJ, dtype, du, const_cast<void *>(x), const_cast<void *>(y),
+                    const_cast<void *>(z), G, sz, h);
+}
+
+int qsm_gradc(int dtype, void *dx, void *dy, void *dz, const void *u,
+              const size_t sz[3], const double h[3])
+{
+    return run_host(GRADC, dtype, const_cast<void *>(u), dx, dy, dz, nullptr, sz, h);
+}
+
+int qsm_gradcm(int dtype, void *dx, void *dy, void *dz, const void *u, const uint8_t *G,
+               const size_t sz[3], const double h[3])
+{
+    return run_host(GRADCM, dtype, const_cast<void *>(u), dx, dy, dz, G, sz, h);
+}
+
 int qsm_lapw(int dtype, void *d2u, const void *u, const size_t sz[3],
              const double h[3])
 {
@@ -402,6 +601,44 @@ int qsm_gradfm_adj_dev(int dtype, void *du, const void *x, const void *y,
 {
     return run_dev(GRADFM_ADJ, dtype, du, const_cast<void *>(x), const_cast<void *>(y),
                    const_cast<void *>(z), G, sz, h, stream);
+}
+
+int qsm_gradb_dev(int dtype, void *dx, void *dy, void *dz, const void *u,
+                  const size_t sz[3], const double h[3], void *stream)
+{
+    return run_dev(GRADB, dtype, const_cast<void *>(u), dx, dy, dz, nullptr, sz, h, stream);
+}
+
+int qsm_gradbm_dev(int dtype, void *dx, void *dy, void *dz, const void *u, const uint8_t *G,
+                   const size_t sz[3], const double h[3], void *stream)
+{
+    return run_dev(GRADBM, dtype, const_cast<void *>(u), dx, dy, dz, G, sz, h, stream);
+}
+
+int qsm_gradb_adj_dev(int dtype, void *du, const void *x, const void *y, const void *z,
+                      const size_t sz[3], const double h[3], void *stream)
+{
+    return run_dev(GRADB_ADJ, dtype, du, const_cast<void *>(x), const_cast<void *>(y),
+                   const_cast<void *>(z), nullptr, sz, h, stream);
+}
+
+int qsm_gradbm_adj_dev(int dtype, void *du, const void *x, const void *y, const void *z,
+                       const uint8_t *G, const size_t sz[3], const double h[3], void *stream)
+{
+    return run_dev(GRADBM_ADJ, dtype, du, const_cast<void *>(x), const_cast<void *>(y),
+                   const_cast<void *>(z), G, sz, h, stream);
+}
+
+int qsm_gradc_dev(int dtype, void *dx, void *dy, void *dz, const void *u,
+                  const size_t sz[3], const double h[3], void *stream)
+{
+    return run_dev(GRADC, dtype, const_cast<void *>(u), dx, dy, dz, nullptr, sz, h, stream);
+}
+
+int qsm_gradcm_dev(int dtype, void *dx, void *dy, void *dz, const void *u, const uint8_t *G,
+                   const size_t sz[3], const double h[3], void *stream)
+{
+    return run_dev(GRADCM, dtype, const_cast<void *>(u), dx, dy, dz, G, sz, h, stream);
 }
 
 int qsm_lapw_dev(int dtype, void *d2u, const void *u, const size_t sz[3],

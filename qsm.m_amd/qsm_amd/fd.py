@@ -1,6 +1,6 @@
 """Finite-difference operators on the GPU, mirroring the reference MEX
-(src/inversion/mex/gradfp*_mex.c, src/utils/fd/mex/grad{f,fm}{,_adj}_mex.c,
-src/unwrap/mex/lapw_mex.c): outputs are written IN PLACE into the arrays the
+(src/inversion/mex/gradfp*_mex.c, src/utils/fd/mex/grad{f,fm,b,bm}{,_adj}_mex.c,
+gradc{,m}_mex.c, src/unwrap/mex/lapw_mex.c): outputs are written IN PLACE into the arrays the
 caller passes first, which must be column-major (Fortran-ordered) and of the
 input's dtype, exactly as the MEX files overwrite their caller-allocated
 prhs[0..2].  The wrappers below also offer allocating forms (gradfp(u, h)
@@ -80,6 +80,45 @@ def gradfm_adj_mex(du, x, y, z, G, h):
     return 1.0
 
 
+def gradb_mex(dx, dy, dz, u, h):
+    """gradb_mex(dx, dy, dz, u, h): backward gradient, first face forward."""
+    _vec_op("qsm_gradb", (dx, dy, dz), u, h)
+    return 1.0
+
+
+def gradbm_mex(dx, dy, dz, u, G, h):
+    """gradbm_mex(dx, dy, dz, u, G, h): masked backward gradient."""
+    _vec_op("qsm_gradbm", (dx, dy, dz), u, h, G)
+    return 1.0
+
+
+def gradb_adj_mex(du, x, y, z, h):
+    """gradb_adj_mex(du, x, y, z, h): -1/h forward differences summed; the
+    voxel (nx-1, ny-1, nz-1) keeps its value (the reference never writes
+    it)."""
+    _adj_op("qsm_gradb_adj", du, x, y, z, h)
+    return 1.0
+
+
+def gradbm_adj_mex(du, x, y, z, G, h):
+    """gradbm_adj_mex(du, x, y, z, G, h)."""
+    _adj_op("qsm_gradbm_adj", du, x, y, z, h, G)
+    return 1.0
+
+
+def gradc_mex(dx, dy, dz, u, h):
+    """gradc_mex(dx, dy, dz, u, h): central gradient, one-sided faces, with
+    the reference's plane-0 rewrite and its unwritten last corner."""
+    _vec_op("qsm_gradc", (dx, dy, dz), u, h)
+    return 1.0
+
+
+def gradcm_mex(dx, dy, dz, u, G, h):
+    """gradcm_mex(dx, dy, dz, u, G, h): masked central gradient."""
+    _vec_op("qsm_gradcm", (dx, dy, dz), u, h, G)
+    return 1.0
+
+
 def lapw_mex(d2u, u, h):
     """lapw_mex(d2u, u, h): Laplacian of wrapped phase, interior, in place."""
     u = fortran(u)
@@ -128,6 +167,39 @@ def gradf_adj(x, y, z, mask=None, h=(1.0, 1.0, 1.0)):
     else:
         gradfm_adj_mex(du, x, y, z, mask, h)
     return du
+
+
+def gradb(u, mask=None, h=(1.0, 1.0, 1.0)):
+    """src/utils/fd/gradb.m:30-46: unmasked MEX when the mask is all true."""
+    u = fortran(u)
+    d = [_zeros_like(u) for _ in range(3)]
+    if mask is None or np.all(np.asarray(mask) != 0):
+        gradb_mex(*d, u, h)
+    else:
+        gradbm_mex(*d, u, mask, h)
+    return tuple(d)
+
+
+def gradb_adj(x, y, z, mask=None, h=(1.0, 1.0, 1.0)):
+    """src/utils/fd/gradbAdj.m."""
+    x = fortran(x)
+    du = _zeros_like(x)
+    if mask is None or np.all(np.asarray(mask) != 0):
+        gradb_adj_mex(du, x, y, z, h)
+    else:
+        gradbm_adj_mex(du, x, y, z, mask, h)
+    return du
+
+
+def gradc(u, mask=None, h=(1.0, 1.0, 1.0)):
+    """src/utils/fd/gradc.m:30-46."""
+    u = fortran(u)
+    d = [_zeros_like(u) for _ in range(3)]
+    if mask is None or np.all(np.asarray(mask) != 0):
+        gradc_mex(*d, u, h)
+    else:
+        gradcm_mex(*d, u, mask, h)
+    return tuple(d)
 
 
 def lapw(u, h=(1.0, 1.0, 1.0)):

@@ -110,7 +110,7 @@ def test_mg_opts_defaults_are_mgpcg_m():
     assert (o.npre, o.nlevels, o.tol) == (2, 3, -1.0)
 
 
-@pytest.mark.parametrize("kw,msg", [({"mgopts": {"nlevels": 16}}, "nlevels"),
+@pytest.mark.parametrize("kw,msg", [({"mgopts": {"nlevels": 17}}, "nlevels"),
                                     ({"mgopts": {"mu": 0}}, "mgopts"),
                                     ({"maxit": 0}, "maxit")])
 def test_mgpcg_argument_checks_without_gpu(kw, msg):

@@ -188,3 +188,61 @@ def test_gradfp_adjoint_pair_on_gpu(cuda):
     du = fd.gradfp_adj(*p, H)
     lhs = sum((a * b).sum() for a, b in zip(g, p))
     assert abs(lhs - (u * du).sum()) <= 1e-12 * (abs(lhs) + 1)
+
+
+BC_SHAPES = [(33, 20, 17), (64, 7, 9), (2, 2, 2), (5, 3, 2), (130, 3, 70), (1, 5, 6)]
+
+
+@pytest.mark.parametrize("shape", BC_SHAPES)
+@pytest.mark.parametrize("dt", DTS)
+def test_backward_and_central_differences(cuda, shape, dt):
+    """gradb, gradbm, gradb_adj, gradbm_adj, gradc, gradcm (src/utils/fd/mex)
+    bitwise against the oracle, in place on outputs with random contents:
+    voxels the reference leaves alone (outside G, the corner gradb_adj and
+    gradc never write) keep them; the unmasked forms need two voxels per
+    axis."""
+    rng = np.random.default_rng(6)
+    u = rand_field(rng, shape, dt, 2.0)
+    G = _mask(shape, 7)
+    x, y, z = (rand_field(rng, shape, dt) for _ in range(3))
+    init = [rand_field(rng, shape, dt) for _ in range(3)]
+    for mex, oracle_fn, masked in ((fd.gradbm_mex, orc.gradbm, True), (fd.gradcm_mex, orc.gradcm, True),
+                                   (fd.gradb_mex, orc.gradb, False), (fd.gradc_mex, orc.gradc, False)):
+        outs = [a.copy(order="F") for a in init]
+        args = (u, G, H) if masked else (u, H)
+        if not masked and min(shape) < 2:
+            with pytest.raises(_lib.QsmError, match="needs >= 2"):
+                mex(*outs, *args)
+            continue
+        mex(*outs, *args)
+        for a, b in zip(outs, oracle_fn(*args, init=init)):
+            _same(a, b)
+    for mex, oracle_fn, masked in ((fd.gradbm_adj_mex, orc.gradbm_adj, True),
+                                   (fd.gradb_adj_mex, orc.gradb_adj, False)):
+        du = init[0].copy(order="F")
+        args = (x, y, z, G, H) if masked else (x, y, z, H)
+        if not masked and min(shape) < 2:
+            with pytest.raises(_lib.QsmError, match="needs >= 2"):
+                mex(du, *args)
+            continue
+        mex(du, *args)
+        _same(du, oracle_fn(*args, init=init[0]))
+
+
+@pytest.mark.parametrize("dt", DTS)
+def test_backward_and_central_allocating_forms(cuda, dt):
+    """gradb.m / gradbAdj.m / gradc.m: zeros, then the unmasked MEX for an
+    all-true mask, the masked one otherwise."""
+    shape = (24, 18, 12)
+    rng = np.random.default_rng(8)
+    u = rand_field(rng, shape, dt)
+    G = _mask(shape, 9)
+    for a, b in zip(fd.gradb(u, None, H), orc.gradb(u, H)):
+        _same(a, b)
+    for a, b in zip(fd.gradc(u, np.ones(shape), H), orc.gradc(u, H)):
+        _same(a, b)
+    for a, b in zip(fd.gradc(u, G, H), orc.gradcm(u, G, H)):
+        _same(a, b)
+    x, y, z = (rand_field(rng, shape, dt) for _ in range(3))
+    _same(fd.gradb_adj(x, y, z, None, H), orc.gradb_adj(x, y, z, H))
+    _same(fd.gradb_adj(x, y, z, G, H), orc.gradbm_adj(x, y, z, G, H))

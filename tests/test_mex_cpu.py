@@ -2,7 +2,9 @@
 here is rejected before any library call).  Usage strings are the
 reference's (pd_tgv_mex.c:107-112, pd_tv_mex.c:103-108, gradfp_mex.c:16-19,
 gradfp_adj_mex.c:18-21, gradfm_mex.c:18-21, gradfm_adj_mex.c:18-21,
-lapw_mex.c:24-27, mgpcg_mex.c:78-82)."""
+lapw_mex.c:24-27, mgpcg_mex.c:78-82, gradb_mex.c:16-19, gradbm_mex.c:18-21,
+gradb_adj_mex.c:18-21, gradbm_adj_mex.c:18-21, gradc_mex.c:16-19,
+gradcm_mex.c:18-21)."""
 import numpy as np
 import pytest
 
@@ -20,6 +22,12 @@ USAGE = {
     "gradfm_mex": ("Usage: gradfm_mex(dx, dy, dz, u, G, h);", 6),
     "gradfm_adj_mex": ("Usage: gradfm_adj_mex(du, x, y, z, G, h);", 6),
     "lapw_mex": ("Usage: lapw_mex(d2u, u, h);", 3),
+    "gradb_mex": ("Usage: gradb_mex(dx, dy, dz, u, h);", 5),
+    "gradbm_mex": ("Usage: gradbm_mex(dx, dy, dz, u, G, h);", 6),
+    "gradb_adj_mex": ("Usage: gradb_adj_mex(du, x, y, z, h);", 5),
+    "gradbm_adj_mex": ("Usage: gradbm_adj_mex(du, x, y, z, G, h);", 6),
+    "gradc_mex": ("Usage: gradc_mex(dx, dy, dz, u, h);", 5),
+    "gradcm_mex": ("Usage: gradcm_mex(dx, dy, dz, u, G, h);", 6),
     "mgpcg_mex": ("Usage: v = mgpcg_mex(v, f, mask, vsz, tolcg, maxitcg, tolmg, maxitmg, mu, "
                   "npre, npost, nboundary, nlevels, verbose)", 14),
 }
