@@ -338,7 +338,7 @@ def batch_leg(iters: int, warm: int = 10):
     }
 
 
-def serve_leg(n: int, rate: float, max_live_voxels: int | None = 24_000_000):
+def serve_leg(n: int, rate: float, max_live_voxels: int | None = 60_000_000, chunk: int = 25):
     """C5: Poisson arrivals served by continuous batching; latency from
     arrival to chi on the host."""
     from qsm_amd import _lib, configs
@@ -361,13 +361,14 @@ def serve_leg(n: int, rate: float, max_live_voxels: int | None = 24_000_000):
                             r["alpha"], r["maxiter"]))
     log(f"[bench] C5: {n} requests prepared in {time.perf_counter() - t:.1f}s")
     with Batch(np.float32, tiles=(64, 8), hbm_budget=160 << 30, max_volumes=64) as b:
-        stats = Server(b, chunk=10, keep_chi=False, max_live_voxels=max_live_voxels).run(reqs)
+        stats = Server(b, chunk=chunk, keep_chi=False, max_live_voxels=max_live_voxels).run(reqs)
     assert all(r.iters == s["maxiter"] for r, s in zip(reqs, specs))
     return {
         "workload": ("C5 Poisson arrivals, TV/TGV 50/50, 70% 224x224x128, 20% 256x256x176, "
                      "10% 320x320x192, budgets 300-1000 iterations, FP32; one seeded phantom "
                      "per size and voxel size (tol = 0: the time does not depend on the data)"),
         "requests": stats.n, "rate_per_s": rate, "max_live_voxels": max_live_voxels,
+        "chunk_iterations": chunk,
         "p50_latency_s": stats.p50, "p99_latency_s": stats.p99,
         "mean_wait_s": stats.mean_wait, "peak_live": stats.peak_live,
         "voxel_iters_per_s": stats.voxel_iters_per_s, "wall_s": stats.wall,
