@@ -225,6 +225,7 @@ def fd_leg(shape=(256, 256, 176), reps=20):
     """The finite-difference entry points at the C3 size on device arrays
     (both precisions), against their algorithmic bytes: gradients read 1
     and write 3 arrays, the adjoints read 3 and write 1, lapw reads 1 and
+    writes 1, lap1 (on a full mask) reads 1 array and the 1-byte mask and
     writes 1.  lapw is timed on what it is given in use, a wrapped phase
     (values in [-pi, pi)), and as lapw_jumps on samples 9 standard normals
     wide, whose differences mostly exceed 3 pi and take the fmod path; the
@@ -246,6 +247,7 @@ def fd_leg(shape=(256, 256, 176), reps=20):
             a = [torch.randn(N, dtype=dt, device="cuda") for _ in range(4)]
             ph = torch.remainder(4 * torch.randn(N, dtype=dt, device="cuda"), 2 * np.pi) - np.pi
             jumps = 9 * torch.randn(N, dtype=dt, device="cuda")
+            G = torch.ones(N, dtype=torch.uint8, device="cuda")
         st.synchronize()
         sp = C.c_void_p(st.cuda_stream)
         ops = {
@@ -259,6 +261,10 @@ def fd_leg(shape=(256, 256, 176), reps=20):
                                                       a[2].data_ptr(), a[3].data_ptr(), sz, h, sp), 4),
             "lapw": (lambda: L.qsm_lapw_dev(code, a[1].data_ptr(), ph.data_ptr(), sz, h, sp), 2),
             "lapw_jumps": (lambda: L.qsm_lapw_dev(code, a[1].data_ptr(), jumps.data_ptr(), sz, h, sp), 2),
+            "gradc": (lambda: L.qsm_gradc_dev(code, a[1].data_ptr(), a[2].data_ptr(), a[3].data_ptr(),
+                                              a[0].data_ptr(), sz, h, sp), 4),
+            "lap1": (lambda: L.qsm_lap1_dev(code, a[1].data_ptr(), a[0].data_ptr(), G.data_ptr(),
+                                            sz, h, sp), 2 + 1 / s),
         }
         res = {}
         for op, (fn, words) in ops.items():
@@ -273,7 +279,7 @@ def fd_leg(shape=(256, 256, 176), reps=20):
             t = e0.elapsed_time(e1) / 1e3 / reps
             res[op] = {"ms": t * 1e3, "gbs": words * N * s / t / 1e9}
         out[name] = res
-        del a, ph, jumps
+        del a, ph, jumps, G
     torch.cuda.empty_cache()
     return out
 

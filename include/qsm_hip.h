@@ -232,6 +232,14 @@ int qsm_gradc(int dtype, void *dx, void *dy, void *dz, const void *u,
 int qsm_gradcm(int dtype, void *dx, void *dy, void *dz, const void *u, const uint8_t *G,
                const size_t sz[3], const double h[3]);
 
+/* lap1_mex.c:29-60 -> lapd/lapf (:136-205 / :63-133), the kernel of lap.m:
+ * on G, d2u = sum over axes of 1/h^2 times a second difference chosen by
+ * which neighbours lie in G (central, or one-sided of third, second or
+ * first order); voxels outside G keep the caller's d2u.  Every extent must
+ * be >= 2 (the reference reads outside the array on an axis of extent 1). */
+int qsm_lap1(int dtype, void *d2u, const void *u, const uint8_t *G,
+             const size_t sz[3], const double h[3]);
+
 /* lapw_mex.c:21-51 -> lapwd/lapwf (:112-187 / :54-109).  Interior voxels
  * are written; the faces of d2u are left as the caller passed them. */
 int qsm_lapw(int dtype, void *d2u, const void *u,
@@ -270,6 +278,8 @@ int qsm_gradfm_adj_dev(int dtype, void *du, const void *x, const void *y,
                        const double h[3], void *stream);
 int qsm_lapw_dev(int dtype, void *d2u, const void *u, const size_t sz[3],
                  const double h[3], void *stream);
+int qsm_lap1_dev(int dtype, void *d2u, const void *u, const uint8_t *G,
+                 const size_t sz[3], const double h[3], void *stream);
 
 /* ------------------------------------------------------------------------
  * Batched inversions: many uneven volumes advanced together (no reference

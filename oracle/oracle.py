@@ -56,6 +56,7 @@ def lib():
         L.orc_gradfm.argtypes = [C.c_int, _p, _p, _p, _p, _p, szp, dp]
         for n in ("gradb", "gradc", "gradb_adj"):
             getattr(L, f"orc_{n}").argtypes = [C.c_int, _p, _p, _p, _p, szp, dp]
+        L.orc_lap1.argtypes = [C.c_int, _p, _p, _p, szp, dp]
         for n in ("gradbm", "gradcm", "gradbm_adj"):
             getattr(L, f"orc_{n}").argtypes = [C.c_int, _p, _p, _p, _p, _p, szp, dp]
         L.orc_gradfm_adj.argtypes = [C.c_int, _p, _p, _p, _p, _p, szp, dp]
@@ -274,6 +275,14 @@ def gradbm_adj(x, y, z, G, h, init=None):
     lib().orc_gradbm_adj(_code(x.dtype), _ptr(du), _ptr(x), _ptr(y), _ptr(z), _ptr(g),
                          _sz3(x.shape), _h3(h))
     return du
+
+
+def lap1(u, G, h, init=None):
+    u = _F(u)
+    g = _F(np.asarray(G) != 0, np.uint8)
+    d = _z(u) if init is None else _F(init).copy(order="F")
+    lib().orc_lap1(_code(u.dtype), _ptr(d), _ptr(u), _ptr(g), _sz3(u.shape), _h3(h))
+    return d
 
 
 def lapw(u, h, init=None):

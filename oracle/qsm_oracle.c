@@ -269,6 +269,13 @@ void orc_gradbm_adj(int dtype, void *du, const void *x, const void *y,
     else gradbm_adj_f(du, x, y, z, G, sz, h);
 }
 
+void orc_lap1(int dtype, void *du, const void *u, const uint8_t *G, const size_t sz[3],
+              const double h[3])
+{
+    if (dtype == 0) lap1_d(du, u, G, sz, h);
+    else lap1_f(du, u, G, sz, h);
+}
+
 void orc_gradc(int dtype, void *dx, void *dy, void *dz, const void *u,
                const size_t sz[3], const double h[3])
 {

@@ -158,6 +158,9 @@ void orc_gradb_adj(int dtype, void *du, const void *x, const void *y,
                    const void *z, const size_t sz[3], const double h[3]);
 void orc_gradbm_adj(int dtype, void *du, const void *x, const void *y,
                     const void *z, const uint8_t *G, const size_t sz[3], const double h[3]);
+/* lap1_mex.c (src/utils/fd/mex): the masked Laplacian of lap.m. */
+void orc_lap1(int dtype, void *du, const void *u, const uint8_t *G, const size_t sz[3],
+              const double h[3]);
 void orc_gradc(int dtype, void *dx, void *dy, void *dz, const void *u,
                const size_t sz[3], const double h[3]);
 void orc_gradcm(int dtype, void *dx, void *dy, void *dz, const void *u,

@@ -1,6 +1,6 @@
 // fd.hip -- finite-difference operators of QSM.m (src/inversion/mex/gradfp*,
-// src/utils/fd/mex/grad{f,fm,b,bm}{,_adj}_mex.c, gradc{,m}_mex.c) and the
-// wrapped-phase Laplacian
+// src/utils/fd/mex/grad{f,fm,b,bm}{,_adj}_mex.c, gradc{,m}_mex.c), the masked
+// Laplacian (src/utils/fd/mex/lap1_mex.c) and the wrapped-phase Laplacian
 // (src/unwrap/mex/lapw_mex.c) for gfx950.
 //
 // One thread per voxel, x fastest, so every wavefront reads and writes 64
@@ -337,10 +337,58 @@ __global__ __launch_bounds__(FX *FY) void lapw_kernel(T *du, const T *u, G3 g,
     du[l] = acc;
 }
 
+// lap1_mex.c:208-315: the second difference along one axis at a voxel of
+// G (stride st, index ii of extent n >= 2).  Inside the axis the +/-1
+// neighbours in G pick central / forward / backward / none; on the first
+// and last index forward / backward whatever G holds.  One-sided forms are
+// third order when the next two cells are in G, else second order when the
+// second is, else the first difference (backward's with the reference's
+// sign, u[-1] - u[0]).
+template <typename T>
+__device__ __forceinline__ T lap1_axis(const T *u, const uint8_t *G, int l, int st,
+                                       int ii, int n)
+{
+    int idx;
+    if (ii >= 1 && ii + 2 <= n)
+        idx = 2 * (G[l + st] != 0) + (G[l - st] != 0);
+    else
+        idx = 2 * (ii == 0) + (ii + 1 == n);
+    const T two = T(2), four = T(4), five = T(5);
+    if (idx == 3) return (u[l - st] - two * u[l]) + u[l + st];
+    if (idx == 2) {
+        if (ii + 3 < n && G[l + 2 * st] && G[l + 3 * st])
+            return ((two * u[l] - five * u[l + st]) + four * u[l + 2 * st]) - u[l + 3 * st];
+        if (ii + 2 < n && G[l + 2 * st]) return (u[l] - two * u[l + st]) + u[l + 2 * st];
+        return u[l + st] - u[l];
+    }
+    if (idx == 1) {
+        if (ii >= 3 && G[l - 3 * st] && G[l - 2 * st])
+            return ((-u[l - 3 * st] + four * u[l - 2 * st]) - five * u[l - st]) + two * u[l];
+        if (ii >= 2 && G[l - 2 * st]) return (u[l - 2 * st] - two * u[l - st]) + u[l];
+        return u[l - st] - u[l];
+    }
+    return T(0);
+}
+
+// lap1_mex.c:63-205: on G, d2u = (hx D_x + hy D_y) + hz D_z, each partial sum
+// rounded in T as the reference's "+=" does; outside G untouched.
+template <typename T>
+__global__ __launch_bounds__(FX *FY) void lap1_kernel(T *du, const T *u, const uint8_t *G,
+                                                      G3 g, T hx, T hy, T hz)
+{
+    int i, j, k, l;
+    if (!vox(g, i, j, k, l)) return;
+    if (!G[l]) return;
+    T d = hx * lap1_axis(u, G, l, 1, i, g.nx);
+    d = d + hy * lap1_axis(u, G, l, g.nx, j, g.ny);
+    d = d + hz * lap1_axis(u, G, l, g.nx * g.ny, k, g.nz);
+    du[l] = d;
+}
+
 dim3 fd_grid(G3 g) { return dim3((g.nx + FX - 1) / FX, (g.ny + FY - 1) / FY, g.nz); }
 
 enum Op { GRADFP, GRADFP_ADJ, GRADF, GRADFM, GRADF_ADJ, GRADFM_ADJ, LAPW,
-          GRADB, GRADBM, GRADB_ADJ, GRADBM_ADJ, GRADC, GRADCM };
+          GRADB, GRADBM, GRADB_ADJ, GRADBM_ADJ, GRADC, GRADCM, LAP1 };
 
 bool is_adj(Op op)
 {
@@ -350,16 +398,17 @@ bool is_adj(Op op)
 
 bool is_masked(Op op)
 {
-    return op == GRADFM || op == GRADFM_ADJ || op == GRADBM || op == GRADBM_ADJ || op == GRADCM;
+    return op == GRADFM || op == GRADFM_ADJ || op == GRADBM || op == GRADBM_ADJ ||
+           op == GRADCM || op == LAP1;
 }
 
 // Coefficients as the reference forms them: 1/h (gradients), -1/h (adjoints),
-// 1/h^2 (lapw), in double and cast on the single path.
+// 1/h^2 (lapw, lap1), in double and cast on the single path.
 template <typename T>
 void coefs(Op op, const double h[3], T c[3])
 {
     for (int a = 0; a < 3; ++a) {
-        if (op == LAPW) c[a] = T(1.0 / (h[a] * h[a]));
+        if (op == LAPW || op == LAP1) c[a] = T(1.0 / (h[a] * h[a]));
         else if (is_adj(op)) c[a] = T(-1.0 / h[a]);
         else c[a] = T(1.0 / h[a]);
     }
@@ -382,6 +431,7 @@ hipError_t launch_op(Op op, T *s, T *a0, T *a1, T *a2, const uint8_t *G,
     case GRADF_ADJ: gradf_adj_kernel<T><<<grid, blk, 0, st>>>(s, a0, a1, a2, g, c[0], c[1], c[2]); break;
     case GRADFM_ADJ: gradfm_adj_kernel<T><<<grid, blk, 0, st>>>(s, a0, a1, a2, G, g, c[0], c[1], c[2]); break;
     case LAPW: lapw_kernel<T><<<grid, blk, 0, st>>>(a0, s, g, c[0], c[1], c[2]); break;
+    case LAP1: lap1_kernel<T><<<grid, blk, 0, st>>>(a0, s, G, g, c[0], c[1], c[2]); break;
     case GRADB: gradb_kernel<T><<<grid, blk, 0, st>>>(a0, a1, a2, s, g, c[0], c[1], c[2]); break;
     case GRADBM: gradbm_kernel<T><<<grid, blk, 0, st>>>(a0, a1, a2, s, G, g, c[0], c[1], c[2]); break;
     case GRADB_ADJ: gradb_adj_kernel<T><<<grid, blk, 0, st>>>(s, a0, a1, a2, g, c[0], c[1], c[2]); break;
@@ -400,11 +450,12 @@ hipError_t launch_op(Op op, T *s, T *a0, T *a1, T *a2, const uint8_t *G,
     return hipGetLastError();
 }
 
-// the unmasked one-sided forms read the +/-1 neighbour on every face
+// the unmasked one-sided forms read the +/-1 neighbour on every face, and so
+// do lap1's (lap1_mex.c reads out of bounds on an axis of extent 1)
 size_t min_dim(Op op)
 {
-    return (op == GRADF || op == GRADF_ADJ || op == GRADB || op == GRADB_ADJ || op == GRADC) ? 2
-                                                                                           : 1;
+    return (op == GRADF || op == GRADF_ADJ || op == GRADB || op == GRADB_ADJ || op == GRADC ||
+            op == LAP1) ? 2 : 1;
 }
 
 int run_dev(Op op, int dtype, void *s, void *a0, void *a1, void *a2,
@@ -413,7 +464,7 @@ int run_dev(Op op, int dtype, void *s, void *a0, void *a1, void *a2,
     int rc = check_grid(sz, h, min_dim(op));
     if (rc) return rc;
     QSM_ARG(dtype == QSM_F64 || dtype == QSM_F32, "bad dtype");
-    QSM_ARG(s && a0 && (op == LAPW || (a1 && a2)), "null array pointer");
+    QSM_ARG(s && a0 && (op == LAPW || op == LAP1 || (a1 && a2)), "null array pointer");
     QSM_ARG(!is_masked(op) || G, "null mask pointer");
     const G3 g{int(sz[0]), int(sz[1]), int(sz[2])};
     if (dtype == QSM_F64)
@@ -433,7 +484,7 @@ int run_host(Op op, int dtype, void *s, void *a0, void *a1, void *a2,
     int rc = check_grid(sz, h, min_dim(op));
     if (rc) return rc;
     QSM_ARG(dtype == QSM_F64 || dtype == QSM_F32, "bad dtype");
-    const bool lap = op == LAPW;
+    const bool lap = op == LAPW || op == LAP1;
     QSM_ARG(s && a0 && (lap || (a1 && a2)), "null array pointer");
     const bool masked = is_masked(op);
     QSM_ARG(!masked || G, "null mask pointer");
@@ -554,6 +605,12 @@ int qsm_gradcm(int dtype, void *dx, void *dy, void *dz, const void *u, const uin
     return run_host(GRADCM, dtype, const_cast<void *>(u), dx, dy, dz, G, sz, h);
 }
 
+int qsm_lap1(int dtype, void *d2u, const void *u, const uint8_t *G, const size_t sz[3],
+             const double h[3])
+{
+    return run_host(LAP1, dtype, const_cast<void *>(u), d2u, nullptr, nullptr, G, sz, h);
+}
+
 int qsm_lapw(int dtype, void *d2u, const void *u, const size_t sz[3],
              const double h[3])
 {
@@ -645,6 +702,12 @@ int qsm_lapw_dev(int dtype, void *d2u, const void *u, const size_t sz[3],
                  const double h[3], void *stream)
 {
     return run_dev(LAPW, dtype, const_cast<void *>(u), d2u, nullptr, nullptr, nullptr, sz, h, stream);
+}
+
+int qsm_lap1_dev(int dtype, void *d2u, const void *u, const uint8_t *G, const size_t sz[3],
+                 const double h[3], void *stream)
+{
+    return run_dev(LAP1, dtype, const_cast<void *>(u), d2u, nullptr, nullptr, G, sz, h, stream);
 }
 
 }  // extern "C"

@@ -86,6 +86,7 @@ _SIGS["qsm_gradfm_dev"] = (C.c_int, [C.c_int, _p, _p, _p, _p, _u8p, C.POINTER(_s
 _SIGS["qsm_gradfm_adj"] = (C.c_int, [C.c_int, _p, _p, _p, _p, _u8p, C.POINTER(_sz_t), _dp])
 _SIGS["qsm_gradfm_adj_dev"] = (C.c_int, [C.c_int, _p, _p, _p, _p, _u8p, C.POINTER(_sz_t), _dp, _p])
 _SIGS["qsm_lapw"] = (C.c_int, [C.c_int, _p, _p, C.POINTER(_sz_t), _dp])
+_SIGS["qsm_lap1"] = (C.c_int, [C.c_int, _p, _p, _p, C.POINTER(_sz_t), _dp])
 for _n in ("gradb", "gradc", "gradb_adj"):
     _SIGS[f"qsm_{_n}"] = (C.c_int, [C.c_int, _p, _p, _p, _p, C.POINTER(_sz_t), _dp])
     _SIGS[f"qsm_{_n}_dev"] = (C.c_int, [C.c_int, _p, _p, _p, _p, C.POINTER(_sz_t), _dp, _p])
@@ -94,6 +95,7 @@ for _n in ("gradbm", "gradcm", "gradbm_adj"):
     _SIGS[f"qsm_{_n}_dev"] = (C.c_int, [C.c_int, _p, _p, _p, _p, _u8p, C.POINTER(_sz_t), _dp,
                                         _p])
 _SIGS["qsm_lapw_dev"] = (C.c_int, [C.c_int, _p, _p, C.POINTER(_sz_t), _dp, _p])
+_SIGS["qsm_lap1_dev"] = (C.c_int, [C.c_int, _p, _p, _p, C.POINTER(_sz_t), _dp, _p])
 
 
 def register(name: str, restype, argtypes) -> None:

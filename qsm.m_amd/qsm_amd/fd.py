@@ -1,6 +1,6 @@
 """Finite-difference operators on the GPU, mirroring the reference MEX
 (src/inversion/mex/gradfp*_mex.c, src/utils/fd/mex/grad{f,fm,b,bm}{,_adj}_mex.c,
-gradc{,m}_mex.c, src/unwrap/mex/lapw_mex.c): outputs are written IN PLACE into the arrays the
+gradc{,m}_mex.c, lap1_mex.c, src/unwrap/mex/lapw_mex.c): outputs are written IN PLACE into the arrays the
 caller passes first, which must be column-major (Fortran-ordered) and of the
 input's dtype, exactly as the MEX files overwrite their caller-allocated
 prhs[0..2].  The wrappers below also offer allocating forms (gradfp(u, h)
@@ -119,6 +119,19 @@ def gradcm_mex(dx, dy, dz, u, G, h):
     return 1.0
 
 
+def lap1_mex(d2u, u, G, h):
+    """lap1_mex(d2u, u, G, h): masked Laplacian, written in place on G."""
+    u = fortran(u)
+    if u.dtype not in (np.float32, np.float64):
+        raise TypeError("u must be single or double")
+    _out_ok(d2u, u)
+    g = fortran(np.asarray(G) != 0, np.uint8)
+    if g.shape != u.shape:
+        raise ValueError("G must have the shape of u")
+    check(lib().qsm_lap1(dtype_code(u.dtype), ptr(d2u), ptr(u), ptr(g), sz3(u.shape), h3(h)))
+    return 1.0
+
+
 def lapw_mex(d2u, u, h):
     """lapw_mex(d2u, u, h): Laplacian of wrapped phase, interior, in place."""
     u = fortran(u)
@@ -200,6 +213,15 @@ def gradc(u, mask=None, h=(1.0, 1.0, 1.0)):
     else:
         gradcm_mex(*d, u, mask, h)
     return tuple(d)
+
+
+def lap(u, mask=None, vsz=(1.0, 1.0, 1.0)):
+    """src/utils/fd/lap.m:1-26: zeros like u, then lap1_mex on the mask (all
+    true by default) -- the MEX runs even for a full mask."""
+    u = fortran(u)
+    d = _zeros_like(u)
+    lap1_mex(d, u, np.ones(u.shape, np.uint8) if mask is None else mask, vsz)
+    return d
 
 
 def lapw(u, h=(1.0, 1.0, 1.0)):

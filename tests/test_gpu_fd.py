@@ -1,5 +1,5 @@
 """GPU finite-difference operators (gradfp, gradfp_adj, gradf, gradf_adj,
-gradfm, gradfm_adj, lapw) against the oracle, bitwise, through both the
+gradfm, gradfm_adj, lapw, lap1) against the oracle, bitwise, through both the
 host-pointer entry points (the MEX shape: outputs overwritten in place) and
 the device-pointer *_dev entry points on a torch stream.  Shapes include
 degenerate axes (length 1 and 2), which take the reference's special
@@ -246,3 +246,54 @@ def test_backward_and_central_allocating_forms(cuda, dt):
     x, y, z = (rand_field(rng, shape, dt) for _ in range(3))
     _same(fd.gradb_adj(x, y, z, None, H), orc.gradb_adj(x, y, z, H))
     _same(fd.gradb_adj(x, y, z, G, H), orc.gradbm_adj(x, y, z, G, H))
+
+
+LAP1_SHAPES = [(33, 20, 17), (64, 7, 9), (2, 2, 2), (5, 3, 2), (130, 3, 70), (4, 9, 4)]
+
+
+@pytest.mark.parametrize("shape", LAP1_SHAPES)
+@pytest.mark.parametrize("dt", DTS)
+@pytest.mark.parametrize("frac", [0.5, 0.9, 1.0])
+def test_lap1_host(cuda, shape, dt, frac):
+    """lap1_mex (src/utils/fd/mex/lap1_mex.c) bitwise against the oracle, in
+    place on an output with random contents (voxels outside G keep them);
+    random masks of several densities pick every branch of the per-axis rule
+    (central, one-sided of third / second / first order, none)."""
+    rng = np.random.default_rng(int(frac * 10) + shape[0])
+    u = rand_field(rng, shape, dt, 2.0)
+    G = np.asfortranarray((rng.random(shape) < frac).astype(np.uint8))
+    init = rand_field(rng, shape, dt)
+    d = init.copy(order="F")
+    fd.lap1_mex(d, u, G, H)
+    _same(d, orc.lap1(u, G, H, init=init))
+
+
+def test_lap1_rejects_unit_extent_and_allocating_form(cuda):
+    rng = np.random.default_rng(12)
+    u = rand_field(rng, (1, 5, 6), np.float64)
+    with pytest.raises(_lib.QsmError, match="needs >= 2"):
+        fd.lap(u)
+    shape = (40, 31, 22)
+    for dt in DTS:
+        u = rand_field(rng, shape, dt)
+        G = _mask(shape, 13)
+        _same(fd.lap(u, G, H), orc.lap1(u, G, H))
+        _same(fd.lap(u, None, H), orc.lap1(u, np.ones(shape, bool), H))
+
+
+@pytest.mark.parametrize("dt", DTS)
+def test_lap1_device_entry_point(cuda, dt):
+    import torch
+
+    shape = (70, 45, 33)
+    rng = np.random.default_rng(14)
+    u = rand_field(rng, shape, dt)
+    G = _mask(shape, 15)
+    init = rand_field(rng, shape, dt)
+    st = torch.cuda.Stream()
+    with torch.cuda.stream(st):
+        du, U, Gd = to_dev(init), to_dev(u), to_dev(G)
+    st.synchronize()
+    _dev_call("qsm_lap1_dev", dt, (du,), (U,), shape, Gd, stream=st)
+    st.synchronize()
+    _same(from_dev(du, shape), orc.lap1(u, G, H, init=init))

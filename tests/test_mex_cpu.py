@@ -22,6 +22,7 @@ USAGE = {
     "gradfm_mex": ("Usage: gradfm_mex(dx, dy, dz, u, G, h);", 6),
     "gradfm_adj_mex": ("Usage: gradfm_adj_mex(du, x, y, z, G, h);", 6),
     "lapw_mex": ("Usage: lapw_mex(d2u, u, h);", 3),
+    "lap1_mex": ("Usage: lap_mex(d2u, u, G, h);", 4),
     "gradb_mex": ("Usage: gradb_mex(dx, dy, dz, u, h);", 5),
     "gradbm_mex": ("Usage: gradbm_mex(dx, dy, dz, u, G, h);", 6),
     "gradb_adj_mex": ("Usage: gradb_adj_mex(du, x, y, z, h);", 5),
