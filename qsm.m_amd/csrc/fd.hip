@@ -338,52 +338,130 @@ __global__ __launch_bounds__(FX *FY) void lapw_kernel(T *du, const T *u, G3 g,
 }
 
 // lap1_mex.c:208-315: the second difference along one axis at a voxel of
-// G (stride st, index ii of extent n >= 2).  Inside the axis the +/-1
-// neighbours in G pick central / forward / backward / none; on the first
-// and last index forward / backward whatever G holds.  One-sided forms are
-// third order when the next two cells are in G, else second order when the
-// second is, else the first difference (backward's with the reference's
-// sign, u[-1] - u[0]).
-template <typename T>
-__device__ __forceinline__ T lap1_axis(const T *u, const uint8_t *G, int l, int st,
-                                       int ii, int n)
+// G, index ii of extent n >= 2 along the axis; U(d) / M(d) give u and "in
+// G" at offset d in [-3, 3] along it (read only where ii + d is inside).
+// Inside the axis the +/-1 neighbours in G pick central / forward /
+// backward / none; on the first and last index forward / backward whatever
+// G holds.  One-sided forms are third order when the next two cells are in
+// G, else second order when the second is, else the first difference
+// (backward's with the reference's sign, u[-1] - u[0]).
+template <typename T, typename FU, typename FM>
+__device__ __forceinline__ T lap1_axis(const FU &U, const FM &M, int ii, int n)
 {
     int idx;
     if (ii >= 1 && ii + 2 <= n)
-        idx = 2 * (G[l + st] != 0) + (G[l - st] != 0);
+        idx = 2 * M(1) + M(-1);
     else
         idx = 2 * (ii == 0) + (ii + 1 == n);
     const T two = T(2), four = T(4), five = T(5);
-    if (idx == 3) return (u[l - st] - two * u[l]) + u[l + st];
+    if (idx == 3) return (U(-1) - two * U(0)) + U(1);
     if (idx == 2) {
-        if (ii + 3 < n && G[l + 2 * st] && G[l + 3 * st])
-            return ((two * u[l] - five * u[l + st]) + four * u[l + 2 * st]) - u[l + 3 * st];
-        if (ii + 2 < n && G[l + 2 * st]) return (u[l] - two * u[l + st]) + u[l + 2 * st];
-        return u[l + st] - u[l];
+        if (ii + 3 < n && M(2) && M(3))
+            return ((two * U(0) - five * U(1)) + four * U(2)) - U(3);
+        if (ii + 2 < n && M(2)) return (U(0) - two * U(1)) + U(2);
+        return U(1) - U(0);
     }
     if (idx == 1) {
-        if (ii >= 3 && G[l - 3 * st] && G[l - 2 * st])
-            return ((-u[l - 3 * st] + four * u[l - 2 * st]) - five * u[l - st]) + two * u[l];
-        if (ii >= 2 && G[l - 2 * st]) return (u[l - 2 * st] - two * u[l - st]) + u[l];
-        return u[l - st] - u[l];
+        if (ii >= 3 && M(-3) && M(-2))
+            return ((-U(-3) + four * U(-2)) - five * U(-1)) + two * U(0);
+        if (ii >= 2 && M(-2)) return (U(-2) - two * U(-1)) + U(0);
+        return U(-1) - U(0);
     }
     return T(0);
 }
 
 // lap1_mex.c:63-205: on G, d2u = (hx D_x + hy D_y) + hz D_z, each partial sum
 // rounded in T as the reference's "+=" does; outside G untouched.
+//
+// The +/-3 reach is served without re-reading: a block of LX x LY threads
+// owns a tile of columns and marches up LZ planes; each thread keeps its
+// column's u and mask at planes k-3 .. k+3 in registers (the z terms), and
+// the block stages plane k of the tile plus the axis-aligned halo the x and
+// y terms read (3 cells beyond each edge, no corners) in LDS, two plane
+// slots so one barrier per plane suffices.  Cells outside the grid hold 0
+// and are never read (the rule tests ii against the extent first).
+constexpr int LX = 64, LY = 8, LZ = 32, LH = 3;
+constexpr int LW = LX + 2 * LH, LHT = LY + 2 * LH;
+constexpr int LHALO = 2 * LH * LX + 2 * LH * LY;  // y rows + x columns
+
 template <typename T>
-__global__ __launch_bounds__(FX *FY) void lap1_kernel(T *du, const T *u, const uint8_t *G,
+__global__ __launch_bounds__(LX *LY) void lap1_kernel(T *du, const T *u, const uint8_t *G,
                                                       G3 g, T hx, T hy, T hz)
 {
-    int i, j, k, l;
-    if (!vox(g, i, j, k, l)) return;
-    if (!G[l]) return;
-    T d = hx * lap1_axis(u, G, l, 1, i, g.nx);
-    d = d + hy * lap1_axis(u, G, l, g.nx, j, g.ny);
-    d = d + hz * lap1_axis(u, G, l, g.nx * g.ny, k, g.nz);
-    du[l] = d;
+    __shared__ T su[2][LHT][LW];
+    __shared__ uint8_t sg[2][LHT][LW];
+    const int tx = threadIdx.x, ty = threadIdx.y, t = tx + LX * ty;
+    const int i0 = blockIdx.x * LX, j0 = blockIdx.y * LY;
+    const int kb = blockIdx.z * LZ, ke = min(kb + LZ, g.nz);
+    const int i = i0 + tx, j = j0 + ty;
+    const bool own = i < g.nx && j < g.ny;
+    const size_t sk = size_t(g.nx) * g.ny;
+    const size_t col = own ? size_t(i) + size_t(g.nx) * j : 0;
+    // this thread's halo cell (at most one: LHALO <= LX * LY) and its column
+    int hx_ = -1, hy_ = -1;
+    bool hin = false;
+    size_t hcol = 0;
+    if (t < LHALO) {
+        if (t < 2 * LH * LX) {
+            const int r = t / LX;
+            hx_ = LH + t % LX;
+            hy_ = r < LH ? r : r + LY;
+        } else {
+            const int q = t - 2 * LH * LX, c = q % (2 * LH);
+            hy_ = LH + q / (2 * LH);
+            hx_ = c < LH ? c : LX + c;
+        }
+        const int gi = i0 + hx_ - LH, gj = j0 + hy_ - LH;
+        hin = gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny;
+        hcol = hin ? size_t(gi) + size_t(g.nx) * gj : 0;
+    }
+    T wu[7];
+    uint8_t wg[7];
+#pragma unroll
+    for (int d = 0; d < 7; ++d) {
+        const int kk = kb - 3 + d;
+        const bool in = own && kk >= 0 && kk < g.nz;
+        wu[d] = in ? u[col + sk * kk] : T(0);
+        wg[d] = in ? uint8_t(G[col + sk * kk] != 0) : uint8_t(0);
+    }
+    for (int k = kb; k < ke; ++k) {
+        const int slot = k & 1;
+        // next plane of the own column, consumed at the end of this step
+        const int kn = k + 4;
+        const bool nin = own && kn < g.nz;
+        const T un = nin ? u[col + sk * kn] : T(0);
+        const uint8_t gn = nin ? uint8_t(G[col + sk * kn] != 0) : uint8_t(0);
+        su[slot][ty + LH][tx + LH] = wu[3];
+        sg[slot][ty + LH][tx + LH] = wg[3];
+        if (t < LHALO) {
+            su[slot][hy_][hx_] = hin ? u[hcol + sk * k] : T(0);
+            sg[slot][hy_][hx_] = hin ? uint8_t(G[hcol + sk * k] != 0) : uint8_t(0);
+        }
+        __syncthreads();
+        if (own && wg[3]) {
+            const int cy = ty + LH, cx = tx + LH;
+            auto UX = [&](int d) { return su[slot][cy][cx + d]; };
+            auto MX = [&](int d) { return int(sg[slot][cy][cx + d]); };
+            auto UY = [&](int d) { return su[slot][cy + d][cx]; };
+            auto MY = [&](int d) { return int(sg[slot][cy + d][cx]); };
+            auto UZ = [&](int d) { return wu[3 + d]; };
+            auto MZ = [&](int d) { return int(wg[3 + d]); };
+            T d2 = hx * lap1_axis<T>(UX, MX, i, g.nx);
+            d2 = d2 + hy * lap1_axis<T>(UY, MY, j, g.ny);
+            d2 = d2 + hz * lap1_axis<T>(UZ, MZ, k, g.nz);
+            du[col + sk * k] = d2;
+        }
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+            wu[d] = wu[d + 1];
+            wg[d] = wg[d + 1];
+        }
+        wu[6] = un;
+        wg[6] = gn;
+    }
 }
+
+dim3 lap1_grid(G3 g) { return dim3((g.nx + LX - 1) / LX, (g.ny + LY - 1) / LY, (g.nz + LZ - 1) / LZ); }
 
 dim3 fd_grid(G3 g) { return dim3((g.nx + FX - 1) / FX, (g.ny + FY - 1) / FY, g.nz); }
 
@@ -431,7 +509,9 @@ hipError_t launch_op(Op op, T *s, T *a0, T *a1, T *a2, const uint8_t *G,
     case GRADF_ADJ: gradf_adj_kernel<T><<<grid, blk, 0, st>>>(s, a0, a1, a2, g, c[0], c[1], c[2]); break;
     case GRADFM_ADJ: gradfm_adj_kernel<T><<<grid, blk, 0, st>>>(s, a0, a1, a2, G, g, c[0], c[1], c[2]); break;
     case LAPW: lapw_kernel<T><<<grid, blk, 0, st>>>(a0, s, g, c[0], c[1], c[2]); break;
-    case LAP1: lap1_kernel<T><<<grid, blk, 0, st>>>(a0, s, G, g, c[0], c[1], c[2]); break;
+    case LAP1:
+        lap1_kernel<T><<<lap1_grid(g), dim3(LX, LY), 0, st>>>(a0, s, G, g, c[0], c[1], c[2]);
+        break;
     case GRADB: gradb_kernel<T><<<grid, blk, 0, st>>>(a0, a1, a2, s, g, c[0], c[1], c[2]); break;
     case GRADBM: gradbm_kernel<T><<<grid, blk, 0, st>>>(a0, a1, a2, s, G, g, c[0], c[1], c[2]); break;
     case GRADB_ADJ: gradb_adj_kernel<T><<<grid, blk, 0, st>>>(s, a0, a1, a2, g, c[0], c[1], c[2]); break;
