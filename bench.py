@@ -391,8 +391,9 @@ def dipole_leg(shape=(256, 256, 176), reps=10):
     written once; the half spectrum along x, H = M0/2 + 1 columns, read and
     written by each pass over the rows and planes that hold data) -- the
     FETCH_SIZE/WRITE_SIZE counters agree with it.  complex_equiv_bytes: the
-    survey's figure for two full complex 3D transforms of the padded volume
-    (2 x 3 axis passes x 2*c*M), which the real-input half spectrum and the
+    bytes of two full complex 3D transforms of the padded volume, each axis
+    pass reading and writing it once (2 x 3 x 2*c*M), which the real-input
+    half spectrum and the
     unmaterialised padding avoid; its rate can exceed HBM peak."""
     import torch
     from qsm_amd import dipole
