@@ -139,12 +139,14 @@ size_t al(size_t n) { return (n + 255) / 256 * 256; }
 
 // Work space of one call: the ball image on the transform grid, two ball
 // spectra (V-SHARP keeps the first radius's for its deconvolution), the
-// convolution's work spectrum, two float volumes and two of the class of f
-// on the crop, three byte masks.
+// convolution's work spectrum, the spectra of the mask and of a single
+// echo's f that V-SHARP reuses for every radius, two float volumes and two
+// of the class of f on the crop, three byte masks.
 template <typename T>
 struct Work {
     T *ball;
-    Cx<T> *Sb, *Sb1, *W;
+    Cx<T> *Sb, *Sb1, *W, *Sf;
+    Cx<float> *Sm;
     float *mf, *sf;
     T *g, *c;
     uint8_t *m1, *mp, *mo;
@@ -158,7 +160,8 @@ size_t layout(const size_t szc[3], const size_t M[3], Work<T> *w, char *base)
     size_t off = 0;
     auto take = [&](size_t b) { char *p = base ? base + off : nullptr; off += al(b); return p; };
     char *pb = take(Mt * sizeof(T)), *ps = take(spec), *ps1 = take(spec), *pw = take(spec);
-    char *pmf = take(n * sizeof(float)), *psf = take(n * sizeof(float));
+    char *psf = take(spec), *psm = take(real_conv_work_bytes(M, sizeof(float)));
+    char *pmf = take(n * sizeof(float)), *pvf = take(n * sizeof(float));
     char *pg = take(n * sizeof(T)), *pc = take(n * sizeof(T));
     char *pm1 = take(n), *pmp = take(n), *pmo = take(n);
     if (w) {
@@ -166,8 +169,10 @@ size_t layout(const size_t szc[3], const size_t M[3], Work<T> *w, char *base)
         w->Sb = reinterpret_cast<Cx<T> *>(ps);
         w->Sb1 = reinterpret_cast<Cx<T> *>(ps1);
         w->W = reinterpret_cast<Cx<T> *>(pw);
+        w->Sf = reinterpret_cast<Cx<T> *>(psf);
+        w->Sm = reinterpret_cast<Cx<float> *>(psm);
         w->mf = reinterpret_cast<float *>(pmf);
-        w->sf = reinterpret_cast<float *>(psf);
+        w->sf = reinterpret_cast<float *>(pvf);
         w->g = reinterpret_cast<T *>(pg);
         w->c = reinterpret_cast<T *>(pc);
         w->m1 = reinterpret_cast<uint8_t *>(pm1);
@@ -199,16 +204,18 @@ int ball_spectrum(Cx<T> *Sb, T *ball, const size_t M[3], const double vsz[3], do
     return real_conv<T, false>(ball, static_cast<T *>(nullptr), M, M, zero, Sb, KeepOp<T>{}, st);
 }
 
-// The eroded mask of one radius: real(ifft3(H .* fft3(single(mask)))) > 0.999.
+// The eroded mask of one radius: real(ifft3(H .* fft3(single(mask)))) > 0.999;
+// from_spectrum: fft3(single(mask)) is already in w.Sm.
 template <typename T>
 int erode(uint8_t *mout, const Cx<T> *Sb, const size_t szc[3], const size_t M[3],
-          const size_t off[3], Work<T> &w, hipStream_t st)
+          const size_t off[3], Work<T> &w, hipStream_t st, bool from_spectrum = false)
 {
     const size_t n = szc[0] * szc[1] * szc[2];
     SmvOp<T, float, 0> op{Sb, int(M[0] / 2 + 1), int(M[1]),
                           0.5 / (double(M[0]) * double(M[1]) * double(M[2])), T(0)};
-    int rc = real_conv<float, true>(w.mf, w.sf, szc, M, off, reinterpret_cast<Cx<float> *>(w.W),
-                                    op, st);
+    Cx<float> *W = reinterpret_cast<Cx<float> *>(w.W);
+    int rc = from_spectrum ? real_conv_from<float>(w.Sm, w.sf, szc, M, off, W, op, st)
+                           : real_conv<float, true>(w.mf, w.sf, szc, M, off, W, op, st);
     if (rc) return rc;
     threshold_kernel<<<grid1(n), NT, 0, st>>>(mout, w.sf, n);
     QSM_HIP(hipGetLastError());
@@ -275,13 +282,24 @@ int vsharp_t(const T *fc, const uint8_t *mc, const size_t szc[3], size_t nech,
     zero_kernel<uint8_t><<<grid1(n), NT, 0, st>>>(w.mp, n);
     zero_kernel<T><<<grid1(n * nech), NT, 0, st>>>(flc, n * nech);
     QSM_HIP(hipGetLastError());
+    // The forward transforms of the mask and (one echo) of f do not depend
+    // on the radius: computed once, each radius runs only op + inverse.
+    if ((rc = real_conv<float, false>(w.mf, static_cast<float *>(nullptr), szc, M, off, w.Sm,
+                                      KeepOp<float>{}, st)))
+        return rc;
+    const bool one = nech == 1;
+    if (one && (rc = real_conv<T, false>(fc, static_cast<T *>(nullptr), szc, M, off, w.Sf,
+                                         KeepOp<T>{}, st)))
+        return rc;
     for (size_t q = 0; q < nrs; ++q) {
         Cx<T> *Sb = q == 0 ? w.Sb1 : w.Sb;  // the first radius's spectrum is kept for ih
         if ((rc = ball_spectrum<T>(Sb, w.ball, M, vsz, rs[q], st))) return rc;
-        if ((rc = erode<T>(w.m1, Sb, szc, M, off, w, st))) return rc;
+        if ((rc = erode<T>(w.m1, Sb, szc, M, off, w, st, true))) return rc;
         const SmvOp<T, T, 1> hp{Sb, H, M1, scale, T(thr)};
         for (size_t e = 0; e < nech; ++e) {
-            if ((rc = real_conv<T, true>(fc + e * n, w.c, szc, M, off, w.W, hp, st))) return rc;
+            rc = one ? real_conv_from<T>(w.Sf, w.c, szc, M, off, w.W, hp, st)
+                     : real_conv<T, true>(fc + e * n, w.c, szc, M, off, w.W, hp, st);
+            if (rc) return rc;
             accumulate_kernel<T><<<grid1(n), NT, 0, st>>>(flc + e * n, w.c, w.m1, w.mp, n);
             QSM_HIP(hipGetLastError());
         }

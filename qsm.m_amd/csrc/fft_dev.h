@@ -602,6 +602,45 @@ __global__ __launch_bounds__(FFT_THREADS) void z_pass(
           });
 }
 
+// z pass from a stored spectrum (all M2 planes, as real_conv<T, false>
+// leaves it): op on every sample of src, the inverse z transform, and the
+// planes [zlo, zhi) stored into data for the inverse y and x passes.  The
+// op sees the values the fused z pass would hold after its forward
+// transform, so results are bitwise those of real_conv<T, true>.
+template <typename T, typename Op>
+__global__ __launch_bounds__(FFT_THREADS) void z_inv_pass(
+    const Cx<T> *src, Cx<T> *data, Vol v, Pen pn, const Plan *pl, const Cx<T> *tw, int zlo,
+    int zhi, Op op)
+{
+    extern __shared__ unsigned char smem[];
+    Cx<T> *A = reinterpret_cast<Cx<T> *>(smem);
+    const int P = 1 << pn.logP, PP = pn.PP;
+    Cx<T> *B = A + size_t(PP) * pl->M;
+    const int M = pl->M;
+    const int nxb = (v.M0 + P - 1) / P;
+    const int lb = xcd_block();  // neighbouring x groups share L2 lines
+    const int x0 = (lb % nxb) * P;
+    const int j = lb / nxb;
+    const int np = min(P, v.M0 - x0);
+    const size_t plane = size_t(v.M0) * v.M1;
+    const size_t base = size_t(x0) + size_t(j) * v.M0;
+    const int logP = pn.logP;
+    fill(A, P * M, [&](int e, int &d) {
+        const int i = e >> logP, p = e & (P - 1);
+        d = i * PP + p;
+        return p < np ? op(x0 + p, j, i, src[base + p + size_t(i) * plane])
+                      : Cx<T>{T(0), T(0)};
+    });
+    __syncthreads();
+    Cx<T> *Q = stockham(A, B, pn, pl, tw, T(1));
+    drain(Q, P * M, [&](int e) { return (e >> logP) * PP + (e & (P - 1)); },
+          [&](int e, Cx<T> x) {
+              const int i = e >> logP, p = e & (P - 1);
+              if (p < np && i >= zlo && i < zhi)
+                  data[base + p + size_t(i) * plane] = x;
+          });
+}
+
 // x pass, inverse: rows inside the output region, two per complex
 // transform.  From the half spectra A, B (k = 0..H-1) of rows 2p and 2p+1,
 // pencil p is Z(k) = A(k) + i B(k), with A(k) = conj(A(M - k)) (likewise B)
@@ -719,6 +758,28 @@ int make_axis(AxisPlan &ap, size_t M, const char *what);
 // ROUND = false the passes stop after op: work then holds op(2 FFT3(pad(in)))
 // on every plane, and out is not touched.  skip (device flag, may be NULL):
 // the passes do nothing when it is set at their turn on the stream.
+// Inverse y pass on the output planes, then the inverse x pass on the
+// output rows (two per transform, cropped into out).
+template <typename T>
+int inverse_yx(Cx<T> *work, T *out, const RealGeom &g, const Vol &v, const AxisPlan ap[3],
+               hipStream_t st, const int *skip)
+{
+    const size_t cs = sizeof(Cx<T>);
+    const int H = v.M0, M1 = v.M1, M0 = int(ap[0].pl.M);
+    const int ny = g.ny, nz = g.nz;
+    const Pen pny = pencils(M1, cs);
+    const int Py = 1 << pny.logP;
+    cy_pass<T><<<((H + Py - 1) / Py) * nz, FFT_THREADS, lds_bytes(pny, M1, cs), st>>>(
+        work, v, g.o2, pny, ap[1].dpl, tw_of<T>(ap[1]), T(1), 0, M1, g.o1, g.o1 + ny, skip);
+    QSM_HIP(hipGetLastError());
+    const Pen pn = pencils(M0, cs);
+    const int P2 = 2 << pn.logP;
+    rx_inv<T><<<(ny * nz + P2 - 1) / P2, FFT_THREADS, lds_bytes(pn, M0, cs), st>>>(
+        work, out, g, pn, ap[0].dpl, tw_of<T>(ap[0]), skip);
+    QSM_HIP(hipGetLastError());
+    return QSM_OK;
+}
+
 template <typename T, bool ROUND, typename Op>
 int real_conv(const T *in, T *out, const size_t sz[3], const size_t M[3], const size_t off[3],
               Cx<T> *work, Op op, hipStream_t st, const int *skip = nullptr)
@@ -760,19 +821,35 @@ int real_conv(const T *in, T *out, const size_t sz[3], const size_t M[3], const 
     }
     if (!ROUND)
         return QSM_OK;
-    {   // y inverse on output planes
-        cy_pass<T><<<((H + Py - 1) / Py) * nz, FFT_THREADS, lds_bytes(pny, M1, cs), st>>>(
-            work, v, g.o2, pny, ap[1].dpl, tw_of<T>(ap[1]), T(1), 0, M1, g.o1, g.o1 + ny, skip);
-        QSM_HIP(hipGetLastError());
-    }
-    {   // x inverse on output rows, two per transform, cropped
-        const Pen pn = pencils(M0, cs);
-        const int P2 = 2 << pn.logP;
-        rx_inv<T><<<(ny * nz + P2 - 1) / P2, FFT_THREADS, lds_bytes(pn, M0, cs), st>>>(
-            work, out, g, pn, ap[0].dpl, tw_of<T>(ap[0]), skip);
-        QSM_HIP(hipGetLastError());
-    }
-    return QSM_OK;
+    return inverse_yx<T>(work, out, g, v, ap, st, skip);
+}
+
+// The convolution's second half from a stored spectrum S (real_conv<T,
+// false>'s output for the same geometry): op, inverse z into work, then the
+// inverse y and x passes into out.
+template <typename T, typename Op>
+int real_conv_from(const Cx<T> *S, T *out, const size_t sz[3], const size_t M[3],
+                   const size_t off[3], Cx<T> *work, Op op, hipStream_t st)
+{
+    int rc;
+    AxisPlan ap[3];
+    for (int a = 0; a < 3; ++a)
+        if ((rc = make_axis<T>(ap[a], M[a], "transform length"))) return rc;
+    if ((rc = allow_lds<&cy_pass<T>>()) || (rc = allow_lds<&z_inv_pass<T, Op>>())
+        || (rc = allow_lds<&rx_inv<T>>()))
+        return rc;
+    const int M0 = int(M[0]), M1 = int(M[1]), M2 = int(M[2]);
+    const int H = M0 / 2 + 1;
+    const Vol v{H, M1, M2};
+    const RealGeom g{int(sz[0]), int(sz[1]), int(sz[2]), int(off[0]), int(off[1]), int(off[2]),
+                     H, M1};
+    const size_t cs = sizeof(Cx<T>);
+    const Pen pn = pencils(M2, cs);
+    const int P = 1 << pn.logP;
+    z_inv_pass<T, Op><<<((H + P - 1) / P) * M1, FFT_THREADS, lds_bytes(pn, M2, cs), st>>>(
+        S, work, v, pn, ap[2].dpl, tw_of<T>(ap[2]), g.o2, g.o2 + g.nz, op);
+    QSM_HIP(hipGetLastError());
+    return inverse_yx<T>(work, out, g, v, ap, st, nullptr);
 }
 
 // In-place complex 3D transform of data (M[0] x M[1] x M[2], x fastest):
