@@ -17,6 +17,12 @@ fail at least one check:
   -tau*K^T for isotropic voxels on masked domains (q paired with weights
   1,2,2,1,2,1), which fixes every sign and every forward/backward index;
 * projection invariants (|p| <= alpha1, weighted |q| <= alpha0);
+* the TV dual's projection onto the lambda-ball as the Euclidean projection
+  of a known vector (chi_ = 0 leaves p at its input), with lambda < 1 so
+  that a threshold on |p| or on lambda instead of lambda^2 fails;
+* convergence norms on data whose ratios are exact (1, 1/4, 3/4, 0, NaN for
+  0/0), the flag's two strict thresholds on both sides, and the single
+  path's double accumulation and mixed float/double difference term;
 * whole loop: zero data stays zero, tol = 0 never stops, and the iterates
   converge to the constraint Delta(eta) - W(chi) + f = 0 on the interior
   mask (the saddle point of the primal-dual problem).
@@ -379,6 +385,166 @@ def test_tv_dual_primal_adjoint():
     lhs = (knu / sigma * nu).sum() + (kp / sigma * p).sum()
     rhs = (-chi / tau * cb).sum() + (-eta * (1 + tau) / tau * eb).sum()
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+# ---------------------------------------------------------------------------
+# TV dual: projection onto the lambda-ball (pd_tv_mex.c:395-402, 498-506)
+# ---------------------------------------------------------------------------
+
+def _tv_dual_run(p0, lam, dt, cb=None):
+    sh = p0.shape[:3]
+    nu = zeros(sh, dt=dt)
+    p = F(p0, dt)
+    cb = zeros(sh, dt=dt) if cb is None else F(cb, dt)
+    mi = F(interior(sh), np.uint8)
+    tau, sigma = orc.step_sizes(HANI)
+    orc.tv_dual(nu, p, cb, zeros(sh, dt=dt), zeros(sh, dt=dt), mi, HANI, lam, sigma)
+    return nu, p
+
+
+@pytest.mark.parametrize("dt,rt", [(np.float64, 1e-15), (np.float32, 3e-7)])
+def test_tv_dual_projection_onto_lambda_ball(dt, rt):
+    """With chi_ = eta_ = f = 0 the dual step leaves p at its input, so the
+    result must be the Euclidean projection of p0 onto {|p| <= lam}:
+    p0 where |p0| <= lam, lam * p0 / |p0| elsewhere.  lam < 1 separates the
+    right threshold (|p|^2 > lam^2) from |p|^2 > lam or |p| > lam^2, and the
+    scaled copy fixes the factor lam / sqrt(|p|^2) (not lam / |p|^2)."""
+    lam = 0.05
+    rng = np.random.default_rng(21)
+    sh = SHAPE
+    d = rng.standard_normal(sh + (3,))
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    # magnitudes across [0.3 lam, 3 lam], away from the rounding band at lam
+    r = lam * rng.uniform(0.3, 3.0, sh)
+    r[np.abs(r / lam - 1) < 1e-4] = 0.5 * lam
+    p0 = (d * r[..., None]).astype(dt)
+    _, p = _tv_dual_run(p0, lam, dt)
+    it = interior(sh)
+    p0d = p0.astype(np.float64)
+    n0 = np.linalg.norm(p0d, axis=-1)
+    inside = it & (n0 < lam)
+    outside = it & (n0 > lam)
+    assert inside.sum() > 20 and outside.sum() > 20
+    # untouched below the threshold, bit for bit
+    np.testing.assert_array_equal(p[inside], p0[inside])
+    # projected onto the sphere of radius lam along p0 above it
+    want = p0d[outside] * (lam / n0[outside])[:, None]
+    np.testing.assert_allclose(p[outside].astype(np.float64), want, rtol=4 * rt, atol=0)
+    pn = np.linalg.norm(p.astype(np.float64), axis=-1)
+    assert pn[it].max() <= lam * (1 + 4 * rt)
+    np.testing.assert_allclose(pn[outside], lam, rtol=4 * rt)
+    # voxels outside the dual mask are not visited
+    np.testing.assert_array_equal(p[~it], p0[~it])
+
+
+@pytest.mark.parametrize("dt,rt", [(np.float64, 1e-13), (np.float32, 1e-5)])
+def test_tv_dual_projection_after_gradient_step(dt, rt):
+    """A finite lam on a random chi_: every |p| <= lam, projected vectors are
+    the unprojected update scaled by lam / |update|, the rest and nu are
+    those of the run without projection (lam = 1e300)."""
+    lam = 0.05
+    rng = np.random.default_rng(22)
+    cb = 0.3 * rng.standard_normal(SHAPE)
+    p0 = 0.01 * rng.standard_normal(SHAPE + (3,))
+    nu_f, p_free = _tv_dual_run(p0, 1e300, dt, cb)
+    nu, p = _tv_dual_run(p0, lam, dt, cb)
+    np.testing.assert_array_equal(nu, nu_f)
+    it = interior(SHAPE)
+    pf = np.linalg.norm(p_free.astype(np.float64), axis=-1)
+    big = it & (pf > lam * (1 + 1e-4))
+    small = it & (pf < lam * (1 - 1e-4))
+    assert big.sum() > 20 and small.sum() > 20
+    np.testing.assert_array_equal(p[small], p_free[small])
+    np.testing.assert_allclose(p[big].astype(np.float64),
+                               p_free[big].astype(np.float64) * (lam / pf[big])[:, None],
+                               rtol=rt)
+    assert np.linalg.norm(p.astype(np.float64), axis=-1)[it].max() <= lam * (1 + rt)
+
+
+# ---------------------------------------------------------------------------
+# convergence norms and flag (pd_tgv_mex.c:980-1045, pd_tv_mex.c:730-795)
+# ---------------------------------------------------------------------------
+
+def _conv(chi, chi0, eta, eta0, tol, dt=np.float64):
+    a = [np.ascontiguousarray(x, dtype=dt) for x in (chi, chi0, eta, eta0)]
+    return orc.convergence(*a, tol)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_convergence_closed_forms(dt):
+    """n1 = ||chi - chi0|| / ||chi||, n2 = ||eta - eta0|| / ||eta||, on
+    values where the ratios are exact: chi0 = 0 gives 1, chi0 = 3 chi / 4
+    gives 1/4 (Sterbenz: the difference is exact), eta0 = eta gives 0.  A
+    swapped numerator / denominator gives 4 instead of 1/4; swapped n1 / n2
+    is caught by the distinct values."""
+    rng = np.random.default_rng(23)
+    # short significands (11 bits) so that 3 chi / 4 is exact in either class
+    chi = rng.integers(512, 2048, 1000) / 1024.0 * np.where(rng.random(1000) < 0.5, -1, 1)
+    eta = rng.integers(512, 2048, 1000) / 1024.0
+    chi = chi.astype(dt)
+    eta = eta.astype(dt)
+    n1, n2, flag = _conv(chi, np.zeros_like(chi), eta, eta, 0.0, dt)
+    assert n1 == 1.0 and n2 == 0.0 and flag == 0
+    n1, n2, _ = _conv(chi, chi * dt(0.75), eta, np.zeros_like(eta), 0.0, dt)
+    assert n1 == 0.25 and n2 == 1.0
+    # 0/0: both NaN, never a stop (pd_tgv_mex.c:1006-1010)
+    z = np.zeros(10, dt)
+    n1, n2, flag = _conv(z, z, z, z, 1.0, dt)
+    assert np.isnan(n1) and np.isnan(n2) and flag == 0
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_convergence_flag_thresholds(dt):
+    """flag = (n1 < tol) and (n2 < 2 tol), strict, checked on both sides of
+    each threshold with n1 = 1/4 and n2 = 3/4."""
+    rng = np.random.default_rng(24)
+    # short significands: 3 chi / 4, chi / 4 and 3 eta / 4 are exact in either class
+    chi = (rng.integers(512, 2048, 500) / 1024.0).astype(dt)
+    eta = (4 * rng.integers(1, 1 << 12, 500)).astype(dt)
+    c0 = chi * dt(0.75)
+    e0 = eta * dt(0.25)
+    n1, n2, _ = _conv(chi, c0, eta, e0, 0.0, dt)
+    assert n1 == 0.25 and n2 == 0.75
+    # n1 < tol and n2 < 2 tol
+    assert _conv(chi, c0, eta, e0, 0.40, dt)[2] == 1     # 0.25 < 0.40, 0.75 < 0.80
+    assert _conv(chi, c0, eta, e0, 0.376, dt)[2] == 1    # 0.75 < 0.752
+    assert _conv(chi, c0, eta, e0, 0.374, dt)[2] == 0    # 0.75 >= 0.748
+    # the n1 threshold alone (eta unchanged: n2 = 0)
+    assert _conv(chi, c0, eta, eta, 0.2501, dt)[2] == 1
+    assert _conv(chi, c0, eta, eta, 0.2499, dt)[2] == 0
+    assert _conv(chi, c0, eta, eta, 0.25, dt)[2] == 0    # strict: n1 == tol is no stop
+    # the n2 threshold is strict too: n2 = 3/4 at 2 tol = 3/4
+    assert _conv(chi, chi, eta, e0, 0.375, dt)[2] == 0
+
+
+def test_convergence_random_against_direct_sums():
+    """Random data: the norms equal the square roots of the plain sums (any
+    summation order agrees to rounding)."""
+    rng = np.random.default_rng(25)
+    chi, chi0, eta, eta0 = rng.standard_normal((4, 4096))
+    n1, n2, _ = _conv(chi, chi0, eta, eta0, 0.0)
+    assert n1 == pytest.approx(np.sqrt(((chi - chi0) ** 2).sum() / (chi ** 2).sum()), rel=1e-13)
+    assert n2 == pytest.approx(np.sqrt(((eta - eta0) ** 2).sum() / (eta ** 2).sum()), rel=1e-13)
+
+
+def test_convergence_single_path_accumulates_in_double():
+    """The single path sums in double and forms each difference term as
+    float(chi - chi0) * (double(chi) - chi0) (pd_tgv_mex.c:1031-1036).
+
+    * d1 = 1 + 2^-26 must not round to 1 (a float accumulator would):
+      chi = (1, 2^-13), chi0 = (1, 0) gives n1 = sqrt(2^-26 / (1 + 2^-26)).
+    * chi = 1, chi0 = 2^-30: float(1 - 2^-30) = 1 while the double
+      difference is exact, so n1^2 = 1 - 2^-30 (all-double would give
+      (1 - 2^-30)^2, all-float 1)."""
+    f32 = np.float32
+    one = np.ones(1, f32)
+    n1, _, _ = _conv(np.array([1.0, 2.0 ** -13], f32), np.array([1.0, 0.0], f32),
+                     np.array([1.0, 1.0], f32), np.array([1.0, 1.0], f32), 0.0, f32)
+    assert n1 == np.sqrt(2.0 ** -26 / (1.0 + 2.0 ** -26))
+    assert n1 != 2.0 ** -13
+    n1, _, _ = _conv(one, np.array([2.0 ** -30], f32), one, one, 0.0, f32)
+    assert n1 == np.sqrt(1.0 - 2.0 ** -30)
+    assert n1 != 1.0 and n1 != 1.0 - 2.0 ** -30
 
 
 # ---------------------------------------------------------------------------
