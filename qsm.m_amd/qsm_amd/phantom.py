@@ -122,13 +122,19 @@ def dipole_field_gpu(chi, vsz, bdir) -> np.ndarray:
     return dipole_conv(np.asfortranarray(chi), vsz, bdir, [s // 2 for s in chi.shape])
 
 
-def make(spec: PhantomSpec, field: str = "host"):
-    """Returns dict(chi, field, mask) for the spec (field with noise).
-    field="gpu" synthesises the field with the GPU dipole convolution."""
+def _host_part(spec: PhantomSpec):
+    """The random draws of a spec, in the order make() takes them: mask,
+    ellipsoids, then the unit-variance noise (rng.normal(0, s) is
+    s * rng.standard_normal() bit for bit)."""
     rng = np.random.default_rng(spec.seed)
     mask = make_mask(spec.shape, spec.mask, spec.mask_param)
     chi = ellipsoids(spec.shape, mask, spec.n_ellipsoids, spec.semi_axes,
                      spec.chi_range, rng)
+    z = rng.standard_normal(size=spec.shape)
+    return chi, mask, z
+
+
+def _finish(spec, chi, mask, z, field):
     if field == "gpu":
         fld = dipole_field_gpu(chi, spec.vsz, spec.bdir)
     elif field == "host":
@@ -136,5 +142,52 @@ def make(spec: PhantomSpec, field: str = "host"):
     else:
         raise ValueError(f"field must be 'host' or 'gpu', not {field!r}")
     sd = float(np.std(fld[mask])) if mask.any() else 0.0
-    fld = fld + rng.normal(0.0, spec.noise * sd, size=spec.shape)
+    fld = fld + (spec.noise * sd) * z
     return {"chi": chi, "field": np.asfortranarray(fld), "mask": mask}
+
+
+def make(spec: PhantomSpec, field: str = "host"):
+    """Returns dict(chi, field, mask) for the spec (field with noise).
+    field="gpu" synthesises the field with the GPU dipole convolution."""
+    if field not in ("gpu", "host"):
+        raise ValueError(f"field must be 'host' or 'gpu', not {field!r}")
+    return _finish(spec, *_host_part(spec), field)
+
+
+def _to_input(spec, ph):
+    from . import fd
+
+    d2f = -fd.lapw(ph["field"], spec.vsz)
+    return np.asfortranarray(d2f), ph["mask"].astype(np.uint8, order="F")
+
+
+def solver_input(spec: PhantomSpec):
+    """The primal-dual solver's input for a spec, as sstgv.m:79-87 forms it:
+    (d2f = -lapw(field) in float64, mask as uint8), both column-major.  The
+    field is synthesised with the GPU dipole convolution and lapw runs on
+    the GPU."""
+    return _to_input(spec, make(spec, field="gpu"))
+
+
+def solver_inputs(specs, workers: int = 8, dtype=np.float64):
+    """solver_input for many specs, equal to calling it on each (d2f cast
+    to dtype): the random draws run in a thread pool, at most 2*workers
+    specs ahead of the consumer, and the GPU steps (dipole convolution,
+    lapw) one at a time in the calling thread, in the order of specs."""
+    import collections
+    import concurrent.futures as cf
+
+    specs = list(specs)
+    out = []
+    workers = max(1, workers)
+    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+        q = collections.deque()
+        nxt = 0
+        while len(out) < len(specs):
+            while nxt < len(specs) and len(q) < 2 * workers:
+                q.append(ex.submit(_host_part, specs[nxt]))
+                nxt += 1
+            s = specs[len(out)]
+            d2f, m = _to_input(s, _finish(s, *q.popleft().result(), "gpu"))
+            out.append((np.asfortranarray(d2f.astype(dtype, copy=False)), m))
+    return out
