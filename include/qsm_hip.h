@@ -468,9 +468,17 @@ int qsm_rts(int dtype, const void *f, const uint8_t *mask, const size_t sz[3],
  * Background field removal (src/bgremove/sharp.m, vsharp.m)
  * ------------------------------------------------------------------------ */
 
-/* Bytes of device work space qsm_sharp_dev / qsm_vsharp_dev need for a
- * cropped volume szc, voxel size vsz and the largest radius rmax. */
+/* Bytes of device work space qsm_sharp_dev needs for a cropped volume szc,
+ * voxel size vsz and radius rmax (the padding ceil(rmax/min(vsz)) of
+ * sharp.m:53).  0 for arguments no call accepts: a bad dtype, an empty
+ * crop, a non-finite or non-positive vsz or rmax. */
 size_t qsm_sharp_work_bytes(int dtype, const size_t szc[3], const double vsz[3], double rmax);
+
+/* The same for qsm_vsharp_dev with the largest radius rmax (vsharp.m:56)
+ * and nechoes echoes: V-SHARP also keeps the spectrum of the mask and, for
+ * one echo, that of f, reused for every radius.  0 on bad arguments. */
+size_t qsm_vsharp_work_bytes(int dtype, const size_t szc[3], const double vsz[3], double rmax,
+                             size_t nechoes);
 
 /* Replaces sharp.m:47-84 after its crop: f and mask already cropped to the
  * mask's non-empty slices (cropIndices.m), szc that crop, nechoes echoes

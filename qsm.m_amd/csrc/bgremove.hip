@@ -141,7 +141,9 @@ size_t al(size_t n) { return (n + 255) / 256 * 256; }
 // spectra (V-SHARP keeps the first radius's for its deconvolution), the
 // convolution's work spectrum, the spectra of the mask and of a single
 // echo's f that V-SHARP reuses for every radius, two float volumes and two
-// of the class of f on the crop, three byte masks.
+// of the class of f on the crop, three byte masks.  The two reused spectra
+// are laid out only for the calls that use them (SPEC_* bits): SHARP needs
+// neither, multi-echo V-SHARP only the mask's.
 template <typename T>
 struct Work {
     T *ball;
@@ -152,15 +154,21 @@ struct Work {
     uint8_t *m1, *mp, *mo;
 };
 
+enum { SPEC_F = 1, SPEC_M = 2 };
+
+// Spectra a call keeps: V-SHARP the mask's, and f's for a single echo.
+inline int vsharp_spectra(size_t nech) { return SPEC_M | (nech == 1 ? SPEC_F : 0); }
+
 template <typename T>
-size_t layout(const size_t szc[3], const size_t M[3], Work<T> *w, char *base)
+size_t layout(const size_t szc[3], const size_t M[3], int spectra, Work<T> *w, char *base)
 {
     const size_t n = szc[0] * szc[1] * szc[2], Mt = M[0] * M[1] * M[2];
     const size_t spec = al(real_conv_work_bytes(M, sizeof(T)));
     size_t off = 0;
     auto take = [&](size_t b) { char *p = base ? base + off : nullptr; off += al(b); return p; };
     char *pb = take(Mt * sizeof(T)), *ps = take(spec), *ps1 = take(spec), *pw = take(spec);
-    char *psf = take(spec), *psm = take(real_conv_work_bytes(M, sizeof(float)));
+    char *psf = (spectra & SPEC_F) ? take(spec) : nullptr;
+    char *psm = (spectra & SPEC_M) ? take(real_conv_work_bytes(M, sizeof(float))) : nullptr;
     char *pmf = take(n * sizeof(float)), *pvf = take(n * sizeof(float));
     char *pg = take(n * sizeof(T)), *pc = take(n * sizeof(T));
     char *pm1 = take(n), *pmp = take(n), *pmo = take(n);
@@ -233,7 +241,7 @@ int sharp_t(const T *fc, const uint8_t *mc, const size_t szc[3], size_t nech,
     size_t M[3], off[3];
     plan_for(szc, vsz, r, M, off);
     Work<T> w;
-    layout<T>(szc, M, &w, static_cast<char *>(work));
+    layout<T>(szc, M, 0, &w, static_cast<char *>(work));
     const size_t n = szc[0] * szc[1] * szc[2];
     const double scale = 0.5 / (double(M[0]) * double(M[1]) * double(M[2]));
     const int H = int(M[0] / 2 + 1), M1 = int(M[1]);
@@ -269,7 +277,7 @@ int vsharp_t(const T *fc, const uint8_t *mc, const size_t szc[3], size_t nech,
     size_t M[3], off[3];
     plan_for(szc, vsz, rs.front(), M, off);
     Work<T> w;
-    layout<T>(szc, M, &w, static_cast<char *>(work));
+    layout<T>(szc, M, vsharp_spectra(nech), &w, static_cast<char *>(work));
     const size_t n = szc[0] * szc[1] * szc[2];
     const double scale = 0.5 / (double(M[0]) * double(M[1]) * double(M[2]));
     const int H = int(M[0] / 2 + 1), M1 = int(M[1]);
@@ -422,17 +430,40 @@ int host_form(int dtype, const void *f, const uint8_t *mask, const size_t sz[3],
     return QSM_OK;
 }
 
+// Sizes the work space only for arguments a call accepts: a known class,
+// non-empty crop, finite positive voxel size and radius (the padding is
+// ceil(rmax / min(vsz)) voxels, sharp.m:53).
+bool work_args_ok(int dtype, const size_t szc[3], const double vsz[3], double rmax)
+{
+    if (!szc || !vsz || (dtype != QSM_F32 && dtype != QSM_F64)) return false;
+    for (int a = 0; a < 3; ++a)
+        if (szc[a] == 0 || !std::isfinite(vsz[a]) || !(vsz[a] > 0.0)) return false;
+    const double p = std::ceil(rmax / std::min(vsz[0], std::min(vsz[1], vsz[2])));
+    return std::isfinite(rmax) && rmax > 0.0 && p < 1e6;
+}
+
 }  // namespace
 
 extern "C" {
 
 size_t qsm_sharp_work_bytes(int dtype, const size_t szc[3], const double vsz[3], double rmax)
 {
-    if (!szc || !vsz) return 0;
+    if (!work_args_ok(dtype, szc, vsz, rmax)) return 0;
     size_t M[3], off[3];
     plan_for(szc, vsz, rmax, M, off);
-    return dtype == QSM_F32 ? layout<float>(szc, M, nullptr, nullptr)
-                            : layout<double>(szc, M, nullptr, nullptr);
+    return dtype == QSM_F32 ? layout<float>(szc, M, 0, nullptr, nullptr)
+                            : layout<double>(szc, M, 0, nullptr, nullptr);
+}
+
+size_t qsm_vsharp_work_bytes(int dtype, const size_t szc[3], const double vsz[3], double rmax,
+                             size_t nechoes)
+{
+    if (!work_args_ok(dtype, szc, vsz, rmax) || nechoes < 1) return 0;
+    size_t M[3], off[3];
+    plan_for(szc, vsz, rmax, M, off);
+    const int sp = vsharp_spectra(nechoes);
+    return dtype == QSM_F32 ? layout<float>(szc, M, sp, nullptr, nullptr)
+                            : layout<double>(szc, M, sp, nullptr, nullptr);
 }
 
 int qsm_sharp_dev(int dtype, const void *fc, const uint8_t *mc, const size_t szc[3],
@@ -495,7 +526,9 @@ int qsm_vsharp(int dtype, const void *f, const uint8_t *mask, const size_t sz[3]
     if ((rc = sorted_radii(rs, nrs, r))) return rc;
     return host_form(
         dtype, f, mask, sz, nechoes, fl, mask_out,
-        [&](const size_t *szc) { return qsm_sharp_work_bytes(dtype, szc, vsz, r.front()); },
+        [&](const size_t *szc) {
+            return qsm_vsharp_work_bytes(dtype, szc, vsz, r.front(), nechoes);
+        },
         [&](const void *df, const uint8_t *dm, const size_t *szc, void *dfl, uint8_t *dmo,
             void *dw) {
             return qsm_vsharp_dev(dtype, df, dm, szc, nechoes, vsz, rs, nrs, thr, dfl, dmo, dw,

@@ -21,6 +21,7 @@ _sz = C.c_size_t
 _p = C.c_void_p
 _dp = C.POINTER(C.c_double)
 _lib.register("qsm_sharp_work_bytes", _sz, [C.c_int, C.POINTER(_sz), _dp, C.c_double])
+_lib.register("qsm_vsharp_work_bytes", _sz, [C.c_int, C.POINTER(_sz), _dp, C.c_double, _sz])
 _lib.register("qsm_sharp", C.c_int, [C.c_int, _p, _p, C.POINTER(_sz), _sz, _dp, C.c_double,
                                      C.c_double, _p, _p])
 _lib.register("qsm_vsharp", C.c_int, [C.c_int, _p, _p, C.POINTER(_sz), _sz, _dp, _dp, _sz,
@@ -76,18 +77,38 @@ def vsharp(f, mask, vsz, rs=None, thr=0.05):
     return fl, mo.astype(bool)
 
 
+def _code(dtype):
+    return _lib.F32 if str(dtype).endswith("float32") else _lib.F64
+
+
+def _checked(nb, what, rmax, vsz):
+    if nb == 0:
+        raise ValueError(f"{what}: radius {rmax!r} and voxel size {tuple(vsz)!r} must be "
+                         f"finite and > 0, the crop non-empty")
+    return nb
+
+
 def work_bytes(dtype, szc, vsz, rmax) -> int:
-    code = _lib.F32 if str(dtype).endswith("float32") else _lib.F64
-    return int(lib().qsm_sharp_work_bytes(code, (_sz * 3)(*szc), h3(vsz), float(rmax)))
+    """Work space of sharp_dev (qsm_sharp_work_bytes)."""
+    return _checked(int(lib().qsm_sharp_work_bytes(_code(dtype), (_sz * 3)(*szc), h3(vsz),
+                                                   float(rmax))), "sharp", rmax, vsz)
 
 
-def _dev_args(f, mask, work, rmax, vsz, stream):
+def vsharp_work_bytes(dtype, szc, vsz, rmax, nechoes=1) -> int:
+    """Work space of vsharp_dev (qsm_vsharp_work_bytes)."""
+    return _checked(int(lib().qsm_vsharp_work_bytes(_code(dtype), (_sz * 3)(*szc), h3(vsz),
+                                                    float(rmax), int(nechoes))), "vsharp",
+                    rmax, vsz)
+
+
+def _dev_args(f, mask, work, rmax, vsz, stream, v=False):
     import torch
 
     shape = tuple(int(s) for s in f.shape[:3])
     ne = 1 if f.dim() == 3 else int(f.shape[3])
     code = _lib.F64 if f.dtype == torch.float64 else _lib.F32
-    nb = work_bytes(f.dtype, shape, vsz, rmax)
+    nb = (vsharp_work_bytes(f.dtype, shape, vsz, rmax, ne) if v
+          else work_bytes(f.dtype, shape, vsz, rmax))
     if work is None:
         work = torch.empty(nb, dtype=torch.uint8, device=f.device)
     elif work.numel() * work.element_size() < nb:
@@ -109,7 +130,9 @@ def sharp_dev(fc, mc, vsz, r, thr, flc, mask_out, work=None, stream=None):
 def vsharp_dev(fc, mc, vsz, rs, thr, flc, mask_out, work=None, stream=None):
     """Device form of vsharp on an already cropped volume: qsm_vsharp_dev."""
     rs = np.ascontiguousarray(np.asarray(rs, dtype=np.float64).reshape(-1))
-    shape, ne, code, work, sp = _dev_args(fc, mc, work, float(rs.max()), vsz, stream)
+    if rs.size == 0 or not np.all(np.isfinite(rs)):
+        raise ValueError("vsharp: radii must be finite")
+    shape, ne, code, work, sp = _dev_args(fc, mc, work, float(rs.max()), vsz, stream, v=True)
     check(lib().qsm_vsharp_dev(code, fc.data_ptr(), mc.data_ptr(), (_sz * 3)(*shape), ne,
                                h3(vsz), rs.ctypes.data_as(_dp), rs.size, float(thr),
                                flc.data_ptr(), mask_out.data_ptr(), work.data_ptr(), sp))

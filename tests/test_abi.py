@@ -135,3 +135,47 @@ def test_mgpcg_work_bytes_cover_the_hierarchy():
     w = _lib.lib().qsm_mgpcg_work_bytes(_lib.F64, (C.c_size_t * 3)(32, 28, 24), 3)
     lo = sum(3 * 8 * v + 2 * v for v in vox) + 2 * 8 * vox[0] + 2 * 8 * vox[-1] + vox[0]
     assert lo <= w <= lo + 64 * 1024
+
+
+def test_sharp_work_bytes_only_hold_what_each_call_uses():
+    """SHARP holds no reused spectra; V-SHARP holds the mask's (single
+    precision) and, for one echo, f's as well."""
+    from qsm_amd import bgremove as bg
+
+    szc, vsz, r = (60, 70, 50), (1.0, 1.0, 1.0), 9.0
+    for dt in ("float32", "float64"):
+        s = bg.work_bytes(dt, szc, vsz, r)
+        v1 = bg.vsharp_work_bytes(dt, szc, vsz, r, 1)
+        v2 = bg.vsharp_work_bytes(dt, szc, vsz, r, 2)
+        assert s < v2 < v1
+        # f's spectrum (class of f) is at least the single-precision mask's
+        assert v1 - v2 >= v2 - s
+
+
+@pytest.mark.parametrize("rmax,vsz", [(float("nan"), (1, 1, 1)), (-3.0, (1, 1, 1)), (0.0, (1, 1, 1)),
+                                      (float("inf"), (1, 1, 1)), (9.0, (1, 0, 1)),
+                                      (9.0, (1, float("nan"), 1)), (9.0, (1, -1, 1))])
+def test_sharp_work_bytes_refuse_bad_radius_or_voxel_size(rmax, vsz):
+    """Bad radii or voxel sizes give 0 bytes from the library (never an
+    undefined padding) and an error from the Python wrappers."""
+    from qsm_amd import bgremove as bg
+
+    sz = (C.c_size_t * 3)(20, 20, 20)
+    h = (C.c_double * 3)(*[float(x) for x in vsz])
+    assert _lib.lib().qsm_sharp_work_bytes(_lib.F64, sz, h, rmax) == 0
+    assert _lib.lib().qsm_vsharp_work_bytes(_lib.F32, sz, h, rmax, 1) == 0
+    with pytest.raises(ValueError):
+        bg.work_bytes("float64", (20, 20, 20), vsz, rmax)
+    with pytest.raises(ValueError):
+        bg.vsharp_work_bytes("float32", (20, 20, 20), vsz, rmax, 2)
+
+
+def test_sharp_work_bytes_refuse_empty_crop_and_no_echoes():
+    from qsm_amd import bgremove  # noqa: F401  (registers the signatures)
+
+    sz0 = (C.c_size_t * 3)(20, 0, 20)
+    sz = (C.c_size_t * 3)(20, 20, 20)
+    h = (C.c_double * 3)(1.0, 1.0, 1.0)
+    assert _lib.lib().qsm_sharp_work_bytes(_lib.F64, sz0, h, 9.0) == 0
+    assert _lib.lib().qsm_vsharp_work_bytes(_lib.F64, sz, h, 9.0, 0) == 0
+    assert _lib.lib().qsm_sharp_work_bytes(7, sz, h, 9.0) == 0
