@@ -181,9 +181,10 @@ def cpu_baseline(cfg, d2f, mask, iters, runs=3, iters_1t=4):
     N = int(np.prod(d2f.shape))
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     loops, walls = [], []
+    chi = None
     for _ in range(runs):
         t = time.perf_counter()
-        orc.pd_tgv(x, mask, cfg["spec"].vsz, cfg["lam"], cfg["alpha"], 0.0, iters)
+        chi = orc.pd_tgv(x, mask, cfg["spec"].vsz, cfg["lam"], cfg["alpha"], 0.0, iters)[0]
         walls.append(time.perf_counter() - t)
         loops.append(orc.last_loop_seconds())
     loop = float(np.median(loops))
@@ -192,8 +193,11 @@ def cpu_baseline(cfg, d2f, mask, iters, runs=3, iters_1t=4):
     if iters_1t > 0:
         orc.set_threads(1)
         try:
-            orc.pd_tgv(x, mask, cfg["spec"].vsz, cfg["lam"], cfg["alpha"], 0.0, iters_1t)
-            one = N * iters_1t / orc.last_loop_seconds()
+            ones = []
+            for _ in range(runs):
+                orc.pd_tgv(x, mask, cfg["spec"].vsz, cfg["lam"], cfg["alpha"], 0.0, iters_1t)
+                ones.append(orc.last_loop_seconds())
+            one = N * iters_1t / float(np.median(ones))
         finally:
             orc.set_threads(threads)
     return {
@@ -207,18 +211,52 @@ def cpu_baseline(cfg, d2f, mask, iters, runs=3, iters_1t=4):
                    f"-O2, no FMA), C3 input {d2f.shape[0]}x{d2f.shape[1]}x{d2f.shape[2]} "
                    f"full-box mask, {iters} iterations, tol=0; median of {runs} runs: "
                    f"iteration-loop time {loop:.2f}s (end-to-end {wall:.2f}s), {threads} "
-                   f"OpenMP threads; 1 thread: {iters_1t} iterations"),
+                   f"OpenMP threads; 1 thread: median of {runs} runs of {iters_1t} "
+                   f"iterations"),
+        "_chi": chi,
+        "_iters": iters,
     }
 
 
-def _phantom_input(spec):
-    """-lapw(field) of a seeded phantom, field synthesised with the GPU
-    dipole convolution; returns (d2f float64, mask uint8)."""
-    from qsm_amd import fd, phantom
+def fp64_agreement(cfg, d2f, mask, cpu_chi, iters):
+    """The FP64 drop-in against the CPU path on the same input: chi of the
+    GPU solver after the same iterations as the CPU baseline's run
+    (pd_tgv_mex.c:130 returns that chi)."""
+    from qsm_amd import _lib, pd
 
-    ph = phantom.make(spec, field="gpu")
-    d2f = -fd.lapw(ph["field"], spec.vsz)
-    return np.asfortranarray(d2f), ph["mask"].astype(np.uint8, order="F")
+    with pd.Solver(_lib.TGV, np.float64, d2f.shape, cfg["spec"].vsz, cfg["lam"],
+                   cfg["alpha"]) as s:
+        s.load(d2f.astype(np.float64, order="F"), mask)
+        s.iterate(iters)
+        g = s.chi()
+    n = float(np.linalg.norm(cpu_chi))
+    return {"iterations": iters, "rel_l2": float(np.linalg.norm(g - cpu_chi) / n) if n else 0.0,
+            "max_abs": float(np.max(np.abs(g - cpu_chi))),
+            "bitwise_equal": bool(np.array_equal(g, cpu_chi)),
+            "cpu": "oracle pd_tgv float64 (the CPU baseline's last run)"}
+
+
+def accuracy_leg(cfg, d2f, mask, iters=1000):
+    """The precision variant's accuracy at the config's own iteration count,
+    independent of --steps: FP32 chi against FP64 chi (the FP64 GPU chi is
+    bitwise the oracle's) after `iters` iterations, TGV and TV."""
+    from qsm_amd import _lib, pd
+
+    out = {"iterations": iters}
+    for name, kind, alpha in (("tgv", _lib.TGV, cfg["alpha"]), ("tv", _lib.TV, 0.0)):
+        chis = {}
+        for dt in (np.float64, np.float32):
+            with pd.Solver(kind, dt, d2f.shape, cfg["spec"].vsz, cfg["lam"], alpha) as s:
+                s.autotune()
+                s.load(d2f.astype(dt, order="F"), mask)
+                s.iterate(iters)
+                chis[dt] = s.chi().astype(np.float64)
+        c64, c32 = chis[np.float64], chis[np.float32]
+        n = float(np.linalg.norm(c64))
+        out[name] = {"rel_l2": float(np.linalg.norm(c32 - c64) / n) if n else 0.0,
+                     "max_abs": float(np.max(np.abs(c32 - c64))),
+                     "max_abs_chi": float(np.max(np.abs(c64)))}
+    return out
 
 
 def fd_leg(shape=(256, 256, 176), reps=20):
@@ -284,20 +322,11 @@ def fd_leg(shape=(256, 256, 176), reps=20):
     return out
 
 
-def batch_leg(iters: int, warm: int = 10):
-    """C4: 48 uneven TV/TGV volumes advanced together (FP32, captured
-    chunks of 10 iterations); compared with the same volumes solved one
-    after another by the single-volume solver."""
-    import torch
-    from qsm_amd import _lib, configs, pd
+def _timed_batch(vols, kinds, inputs, iters, warm):
+    """Advances the volumes together (FP32, captured chunks of 10); returns
+    (seconds for `iters` iterations after `warm`, resident bytes)."""
     from qsm_amd.batch import Batch
 
-    vols = configs.c4_volumes()
-    t = time.perf_counter()
-    inputs = [_phantom_input(v["spec"]) for v in vols]
-    log(f"[bench] C4: {len(vols)} phantoms in {time.perf_counter() - t:.1f}s")
-    kinds = [_lib.TGV if v["kind"] == "tgv" else _lib.TV for v in vols]
-    vox = sum(int(np.prod(v["spec"].shape)) for v in vols)
     with Batch(np.float32, tiles=(64, 8), hbm_budget=160 << 30, max_volumes=64) as b:
         for v, k, (d2f, m) in zip(vols, kinds, inputs):
             b.add(k, d2f, m, v["spec"].vsz, v["lam"], v["alpha"], maxiter=warm + iters)
@@ -310,8 +339,26 @@ def batch_leg(iters: int, warm: int = 10):
         b.sync()
         dt = time.perf_counter() - t0
         done = [b.state(s)[2] for s in range(len(vols))]
-    assert all(d == warm + 10 * (iters // 10) for d in done), done
+    assert all(d == warm + iters for d in done), done
+    return dt, resident
+
+
+def batch_leg(iters: int, warm: int = 10):
+    """C4: 48 uneven TV/TGV volumes advanced together (FP32, captured
+    chunks of 10 iterations); compared with the same volumes solved one
+    after another by the single-volume solver, for the whole mix and for
+    each size class batched on its own."""
+    import torch
+    from qsm_amd import _lib, configs, pd, phantom
+
+    vols = configs.c4_volumes()
+    t = time.perf_counter()
+    inputs = phantom.solver_inputs([v["spec"] for v in vols], dtype=np.float32)
+    log(f"[bench] C4: {len(vols)} phantoms in {time.perf_counter() - t:.1f}s")
+    kinds = [_lib.TGV if v["kind"] == "tgv" else _lib.TV for v in vols]
     run = 10 * (iters // 10)
+    vox = sum(int(np.prod(v["spec"].shape)) for v in vols)
+    dt, resident = _timed_batch(vols, kinds, inputs, run, warm)
     # the same volumes one after another (per size class and kind, tuned tiles)
     st = torch.cuda.Stream()
     per = {}
@@ -333,6 +380,23 @@ def batch_leg(iters: int, warm: int = 10):
             e1.synchronize()
             per[key] = e0.elapsed_time(e1) / 1e3 / run
     seq = sum(per[(v["spec"].shape, k)] for v, k in zip(vols, kinds)) * run
+    classes = {}
+    for shape in configs.C4_SIZES:
+        idx = [i for i, v in enumerate(vols) if v["spec"].shape == shape]
+        if not idx:
+            continue
+        cv = [vols[i] for i in idx]
+        ck = [kinds[i] for i in idx]
+        ci = [inputs[i] for i in idx]
+        cdt, _ = _timed_batch(cv, ck, ci, run, warm)
+        cseq = sum(per[(v["spec"].shape, k)] for v, k in zip(cv, ck)) * run
+        cvox = len(idx) * int(np.prod(shape))
+        classes["x".join(map(str, shape))] = {
+            "volumes": len(idx), "tgv": int(sum(k == _lib.TGV for k in ck)),
+            "batch_voxel_iters_per_s": cvox * run / cdt,
+            "sequential_voxel_iters_per_s": cvox * run / cseq,
+            "speedup_vs_sequential": cseq / cdt,
+        }
     return {
         "workload": ("C4 48 TV/TGV inversions, sizes uniform over 96^3, 128^3, 160x160x96, "
                      "224x224x128, 256x256x176, own phantom/mask/B0 each, FP32"),
@@ -341,44 +405,73 @@ def batch_leg(iters: int, warm: int = 10):
         "sequential_voxel_iters_per_s": vox * run / seq,
         "speedup_vs_sequential": seq / dt,
         "resident_gib": resident / 2 ** 30,
+        "per_size_class": classes,
     }
 
 
-def serve_leg(n: int, rate: float, max_live_voxels: int | None = 60_000_000, chunk: int = 25):
-    """C5: Poisson arrivals served by continuous batching; latency from
-    arrival to chi on the host."""
-    from qsm_amd import _lib, configs
+def _serve_once(specs, inputs, rate, max_live_voxels, chunk):
+    from qsm_amd import _lib
     from qsm_amd.batch import Batch
     from qsm_amd.serve import Request, Server
 
-    specs = configs.c5_requests(n=n, rate=rate)
-    t = time.perf_counter()
-    reqs = []
-    # one phantom per (size, voxel size): with tol = 0 a solve's time does
-    # not depend on the data, only on its size and iteration budget
-    cache = {}
-    for i, r in enumerate(specs):
-        key = (tuple(r["spec"].shape), tuple(r["spec"].vsz))
-        if key not in cache:
-            cache[key] = _phantom_input(r["spec"])
-        d2f, m = cache[key]
-        reqs.append(Request(i, r["arrival"], _lib.TGV if r["kind"] == "tgv" else _lib.TV,
-                            d2f.astype(np.float32, order="F"), m, r["spec"].vsz, r["lam"],
-                            r["alpha"], r["maxiter"]))
-    log(f"[bench] C5: {n} requests prepared in {time.perf_counter() - t:.1f}s")
+    reqs = [Request(i, r["arrival"], _lib.TGV if r["kind"] == "tgv" else _lib.TV, d2f, m,
+                    r["spec"].vsz, r["lam"], r["alpha"], r["maxiter"])
+            for i, (r, (d2f, m)) in enumerate(zip(specs, inputs))]
     with Batch(np.float32, tiles=(64, 8), hbm_budget=160 << 30, max_volumes=64) as b:
         stats = Server(b, chunk=chunk, keep_chi=False, max_live_voxels=max_live_voxels).run(reqs)
     assert all(r.iters == s["maxiter"] for r, s in zip(reqs, specs))
+    span = float(max(r.arrival for r in reqs))
+    work = sum(int(np.prod(r.d2f.shape)) * r.maxiter for r in reqs)
     return {
-        "workload": ("C5 Poisson arrivals, TV/TGV 50/50, 70% 224x224x128, 20% 256x256x176, "
-                     "10% 320x320x192, budgets 300-1000 iterations, FP32; one seeded phantom "
-                     "per size and voxel size (tol = 0: the time does not depend on the data)"),
-        "requests": stats.n, "rate_per_s": rate, "max_live_voxels": max_live_voxels,
-        "chunk_iterations": chunk,
+        "requests": stats.n, "rate_per_s": rate, "arrival_span_s": span,
         "p50_latency_s": stats.p50, "p99_latency_s": stats.p99,
         "mean_wait_s": stats.mean_wait, "peak_live": stats.peak_live,
+        "offered_voxel_iters_per_s": work / span if span > 0 else None,
         "voxel_iters_per_s": stats.voxel_iters_per_s, "wall_s": stats.wall,
-        "last_arrival_s": float(max(r.arrival for r in reqs)),
+    }
+
+
+def serve_leg(n: int, rate: float, sustained_rate: float | None = None,
+              max_live_voxels: int | None = 60_000_000, chunk: int = 25):
+    """C5: Poisson arrivals served by continuous batching, each request with
+    its own seeded phantom; latency from arrival to chi on the host.  Two
+    rates over the same requests: `rate` (20/s, more than one GPU serves:
+    the latency is mostly queueing) and a sustainable one that offers 0.8 of
+    the throughput measured at `rate` (or `sustained_rate` if given)."""
+    from qsm_amd import configs, phantom
+
+    specs = configs.c5_requests(n=n, rate=rate)
+    t = time.perf_counter()
+    inputs = phantom.solver_inputs([r["spec"] for r in specs], dtype=np.float32)
+    log(f"[bench] C5: {n} requests, own phantom each, prepared in "
+        f"{time.perf_counter() - t:.1f}s")
+    over = _serve_once(specs, inputs, rate, max_live_voxels, chunk)
+    log(f"[bench] C5 at {rate}/s: p50 {over['p50_latency_s']:.2f}s p99 "
+        f"{over['p99_latency_s']:.2f}s, served {over['voxel_iters_per_s']:.3e} of offered "
+        f"{over['offered_voxel_iters_per_s']:.3e} voxel-iters/s")
+    mean_work = sum(int(np.prod(r["spec"].shape)) * r["maxiter"] for r in specs) / n
+    if sustained_rate is None:
+        sustained_rate = 0.8 * over["voxel_iters_per_s"] / mean_work
+    # the same requests (sizes, kinds, budgets and phantoms do not depend on
+    # the rate: arrivals are scaled exponential draws)
+    specs2 = configs.c5_requests(n=n, rate=sustained_rate)
+    assert all(a["spec"] == b["spec"] and a["maxiter"] == b["maxiter"]
+               for a, b in zip(specs, specs2))
+    sus = _serve_once(specs2, inputs, sustained_rate, max_live_voxels, chunk)
+    log(f"[bench] C5 at {sustained_rate:.2f}/s: p50 {sus['p50_latency_s']:.2f}s p99 "
+        f"{sus['p99_latency_s']:.2f}s")
+    return {
+        "workload": ("C5 Poisson arrivals, TV/TGV 50/50, 70% 224x224x128, 20% 256x256x176, "
+                     "10% 320x320x192, budgets 300-1000 iterations, FP32, own seeded phantom "
+                     "per request"),
+        "max_live_voxels": max_live_voxels, "chunk_iterations": chunk,
+        "mean_voxel_iters_per_request": mean_work,
+        "overload": over,
+        "sustained": dict(sus, utilisation=(sus["offered_voxel_iters_per_s"]
+                                            / over["voxel_iters_per_s"])),
+        # the overload figures at the top level, as before
+        "p50_latency_s": over["p50_latency_s"], "p99_latency_s": over["p99_latency_s"],
+        "voxel_iters_per_s": over["voxel_iters_per_s"],
     }
 
 
@@ -717,11 +810,16 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-iters", type=int, default=20,
                     help="iterations of the CPU baseline sample (0: skip)")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--accuracy-iters", type=int, default=1000,
+                    help="iterations of the FP32-vs-FP64 chi accuracy leg (0: skip)")
     ap.add_argument("--batch-iters", type=int, default=100,
                     help="iterations of the C4 batch leg (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=200,
                     help="requests of the C5 serving leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-rate-sustained", type=float, default=None,
+                    help="rate of the second serving leg (default: 0.8 of what the GPU "
+                         "served at --serve-rate)")
     ap.add_argument("--no-bgremove", dest="bgremove", action="store_false",
                     help="skip the SHARP / V-SHARP leg")
     ap.add_argument("--no-brain", dest="brain", action="store_false",
@@ -864,12 +962,20 @@ def main(argv=None) -> int:
         out["batch"] = batch_leg(a.batch_iters)
         log(f"[bench] C4 batch: {out['batch']['voxel_iters_per_s']:.3e} voxel-iters/s")
     if a.serve_requests > 0:
-        out["serving"] = serve_leg(a.serve_requests, a.serve_rate)
-        log(f"[bench] C5 serving: p50 {out['serving']['p50_latency_s']:.3f}s "
-            f"p99 {out['serving']['p99_latency_s']:.3f}s")
+        out["serving"] = serve_leg(a.serve_requests, a.serve_rate, a.serve_rate_sustained)
+    if a.accuracy_iters > 0:
+        out["accuracy"] = accuracy_leg(cfg, d2f, mask, a.accuracy_iters)
+        log(f"[bench] FP32 vs FP64 chi after {a.accuracy_iters} iterations: TGV rel L2 "
+            f"{out['accuracy']['tgv']['rel_l2']:.2e}, TV {out['accuracy']['tv']['rel_l2']:.2e}")
     if a.cpu_iters > 0:
         log(f"[bench] CPU baseline: {a.cpu_iters} iterations of the oracle ...")
-        out["cpu_baseline"] = cpu_baseline(cfg, d2f, mask, a.cpu_iters)
+        cb = cpu_baseline(cfg, d2f, mask, a.cpu_iters)
+        cpu_chi, cpu_it = cb.pop("_chi"), cb.pop("_iters")
+        out["cpu_baseline"] = cb
+        out["fp64_gpu_vs_cpu"] = fp64_agreement(cfg, d2f, mask, cpu_chi, cpu_it)
+        log(f"[bench] FP64 GPU vs CPU chi after {cpu_it} iterations: rel L2 "
+            f"{out['fp64_gpu_vs_cpu']['rel_l2']:.2e}, bitwise "
+            f"{out['fp64_gpu_vs_cpu']['bitwise_equal']}")
     print(json.dumps(out), flush=True)
     return 0
 
