@@ -122,16 +122,24 @@ int qsm_pd_get_chi(qsm_pd *s, void *chi, int on_device);
 /* Frees the handle and its device memory. */
 void qsm_pd_destroy(qsm_pd *s);
 
-/* Selects the sweep schedule of a handle: 0 = fused single pass per
- * iteration (default), 1 = the reference's two sweeps per iteration (dual
- * kernel then primal kernel).  Both give identical results. */
+/* Selects the sweep schedule of a handle: 0 = fused passes that advance
+ * two iterations each, with one-iteration passes where a convergence check
+ * or the call's iteration count needs them (default); 1 = the reference's
+ * two sweeps per iteration (dual kernel then primal kernel); 2 = fused
+ * one-iteration passes only.  All give identical results. */
 int qsm_pd_set_schedule(qsm_pd *s, int schedule);
 
-/* Tile shape of the fused pass: tx x ty voxels per block (supported:
- * 64x4, 64x8, 64x16, 32x8, 32x16) marching kz planes.  Results do not
- * depend on it; speed does (see the autotuner).  Any time between calls. */
+/* Tile shape of the one-iteration fused pass: tx x ty voxels per block
+ * (supported: 64x4, 64x8, 64x16, 32x8, 32x16, 32x32) marching kz planes.
+ * Results do not depend on it; speed does (see the autotuner).  Any time
+ * between calls. */
 int qsm_pd_set_tiles(qsm_pd *s, int tx, int ty, int kz);
 int qsm_pd_get_tiles(qsm_pd *s, int *tx, int *ty, int *kz);
+
+/* z-chunk depth (planes per block) of the two-iteration fused pass, whose
+ * tiles are fixed per precision (32x16 voxels FP32, 16x16 FP64); default:
+ * at most one block per CU.  Results do not depend on it. */
+int qsm_pd_set_kz2(qsm_pd *s, int kz);
 
 /* ------------------------------------------------------------------------
  * Primal-dual single sweeps on device arrays (per-operator access; the

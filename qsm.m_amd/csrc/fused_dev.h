@@ -199,7 +199,7 @@ __device__ __forceinline__ void pcol(const Ctx<T> &c, const PrimalCoef<T> &pc,
     if (q.l0 >= 0) {
         const int l = q.l0 + c.sk * kp;
         const uint32_t vb = uint32_t(l) * S;
-        if (kp + 1 < c.nz) {
+        if (kp + 1 >= 0 && kp + 1 < c.nz) {
             mn = bld_u8(c.mk, uint32_t(l + c.sk));
             // nu is zero wherever the dual is inactive (never written
             // there), so the load need not wait for the mask: one
@@ -282,7 +282,9 @@ __device__ __forceinline__ void col_init(const Ctx<T> &c, int kp0, bool wq, ColQ
     constexpr uint32_t S = sizeof(T);
     q.mc = 0;
     q.nzm = q.nzc = q.p3m = q.q3m = q.q5m = q.q6m = T(0);
-    if (q.l0 < 0)
+    // planes outside the volume hold no active voxel (a march may start
+    // below plane 0; pcol then loads plane 0 as its next plane)
+    if (q.l0 < 0 || kp0 < 0 || kp0 >= c.nz)
         return;
     const int l = q.l0 + c.sk * kp0;
     const uint32_t vb = uint32_t(l) * S;
@@ -318,10 +320,12 @@ __device__ __forceinline__ void ring_pos(int r, int &rx, int &ry)
     else { rx = TX; ry = r - 2 * TX - TY; }
 }
 
-// Sums s[0..3] over the block in a fixed order and writes them from thread 0.
+// Sums s[0..3] over the block in a fixed order and writes them from thread 0
+// (NT, the block size, a multiple of the 64-lane wave).
 template <int NT>
 __device__ __forceinline__ void block_sum4_f(double s[4], double *dst)
 {
+    static_assert(NT % 64 == 0, "whole waves only");
     __shared__ double red[NT / 64][4];
     for (int off = 32; off > 0; off >>= 1)
         for (int q = 0; q < 4; ++q)

@@ -39,14 +39,39 @@ Layout layout(int kind, int dtype, const size_t sz[3], int npart, int hist_cap)
 }
 
 // Partial-sum slots that cover every supported fused shape (at kz = 1, the
-// most blocks a shape can have) and the two-sweep primal.
+// most blocks a shape can have), the two-iteration pass and the two-sweep
+// primal.
 int max_partials(Geom g)
 {
     static const int shapes[][2] = {{64, 4}, {64, 8}, {64, 16}, {32, 8}, {32, 16}, {32, 32}};
     int n = std::max(primal_sweep_blocks(g), 1);
     for (const auto &sh : shapes)
         n = std::max(n, fused_blocks(g, FusedShape{sh[0], sh[1], 1}));
+    n = std::max(n, fused2_blocks(g, QSM_F32, 1));
+    n = std::max(n, fused2_blocks(g, QSM_F64, 1));
     return n;
+}
+
+int cu_count()
+{
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) != hipSuccess
+        || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess
+        || ncu < 1)
+        ncu = 256;
+    return ncu;
+}
+
+// z-chunk depth of the two-iteration pass: as for the one-iteration pass,
+// at most one block per CU (each block is resident for its whole march).
+int default_kz2(int dtype, Geom g)
+{
+    int tx, ty;
+    fused2_tile(dtype, tx, ty);
+    const int inner = std::max(1, g.nz - 2);
+    const int nxy = ((g.nx + tx - 1) / tx) * ((g.ny + ty - 1) / ty);
+    const int nch = std::max(1, cu_count() / nxy);
+    return std::max(1, (inner + nch - 1) / nch);
 }
 
 // Default fused tile shape (the autotuner's table may pick others): 64x16
@@ -59,11 +84,7 @@ int max_partials(Geom g)
 FusedShape default_shape(int kind, int dtype, Geom g)
 {
     FusedShape sh{64, (dtype == QSM_F64 && kind == QSM_TGV) ? 8 : 16, 16};
-    int dev = 0, ncu = 256;
-    if (hipGetDevice(&dev) != hipSuccess
-        || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess
-        || ncu < 1)
-        ncu = 256;
+    const int ncu = cu_count();
     const int inner = std::max(1, g.nz - 2);
     const int nxy = ((g.nx + sh.tx - 1) / sh.tx) * ((g.ny + sh.ty - 1) / sh.ty);
     const int nch = std::max(1, ncu / nxy);  // at most one block per CU
@@ -74,6 +95,9 @@ FusedShape default_shape(int kind, int dtype, Geom g)
 }  // namespace
 
 struct qsm_pd {
+    // schedule 0: two-iteration passes, one-iteration passes where a
+    // convergence check or the call's count needs them; 1: the reference's
+    // two sweeps; 2: one-iteration passes only
     int kind, dtype, schedule;
     size_t sz[3], N;
     double h[3], lam, alpha, tau, sigma;
@@ -85,6 +109,8 @@ struct qsm_pd {
     int dual_ahead;   // fused schedule: dual of iteration n+1 applied
     int loaded;
     FusedShape shape;
+    int kz2;          // z-chunk depth of the two-iteration pass
+    uint32_t cur;     // fused schedules: bank holding the latest iterates
 
     void *bank(int b) const { return mem + size_t(b) * L.bank_bytes; }
     void *f() const { return mem + L.f_off; }
@@ -140,27 +166,53 @@ int iterate_impl(qsm_pd *s, uint32_t niter, double tol)
             "fused tile shape needs %d partial-sum slots, %d allocated",
             fused_blocks(g, s->shape), s->npart);
 
+    QSM_ARG(s->schedule != 0 || fused2_blocks(g, s->dtype, s->kz2) <= s->npart,
+            "two-iteration pass needs %d partial-sum slots, %d allocated",
+            fused2_blocks(g, s->dtype, s->kz2), s->npart);
+
     for (uint32_t t = 0; t < niter; ++t) {
         const uint32_t n1 = s->n + 1;
         const bool check = (n1 % 10) == 0;  // pd_tgv_mex.c:298,309
         double *part = check ? s->partials() : nullptr;
         int nblk = 0;
-        if (s->schedule == 0) {
-            if (!s->dual_ahead) {
-                // Dual of iteration 1 from the zero primal iterates, which
-                // bank 1 holds until the first fused pass writes it.
-                const hipError_t e = tgv
-                    ? launch_dual_sweep<T, true>(b0, bars_of(b1), f, s->mask(), 2, g, dc, stop, s->stream)
-                    : launch_dual_sweep<T, false>(b0, bars_of(b1), f, s->mask(), 2, g, dc, stop, s->stream);
-                QSM_HIP(e);
-                s->dual_ahead = 1;
-            }
-            const T *in = static_cast<const T *>(s->bank((n1 & 1) ? 0 : 1));
-            T *out = static_cast<T *>(s->bank((n1 & 1) ? 1 : 0));
+        if (s->schedule != 1 && !s->dual_ahead) {
+            // Dual of iteration 1 from the zero primal iterates, which bank 1
+            // holds until the first fused pass writes it.
+            const hipError_t e = tgv
+                ? launch_dual_sweep<T, true>(b0, bars_of(b1), f, s->mask(), 2, g, dc, stop, s->stream)
+                : launch_dual_sweep<T, false>(b0, bars_of(b1), f, s->mask(), 2, g, dc, stop, s->stream);
+            QSM_HIP(e);
+            s->dual_ahead = 1;
+        }
+        if (s->schedule == 0 && t + 1 < niter && !check) {
+            // iterations n1 and n1+1 in one pass; a check falls at most on
+            // n1+1 (consecutive iterations), whose chi the pass stores
+            const uint32_t n2 = n1 + 1;
+            const bool check2 = (n2 % 10) == 0;
+            double *part2 = check2 ? s->partials() : nullptr;
+            const T *in = static_cast<const T *>(s->bank(s->cur));
+            T *out = static_cast<T *>(s->bank(s->cur ^ 1u));
+            const hipError_t e = tgv
+                ? launch_fused2<T, true>(in, out, s->N, f, s->mask(), g, s->kz2, dc, pc, stop, part2, s->stream)
+                : launch_fused2<T, false>(in, out, s->N, f, s->mask(), g, s->kz2, dc, pc, stop, part2, s->stream);
+            QSM_HIP(e);
+            s->cur ^= 1u;
+            if (check2)
+                QSM_HIP(launch_conv_finalize(s->partials(), fused2_blocks(g, s->dtype, s->kz2),
+                                             s->ctl(), s->hist(), s->hist_cap, tol, n2, s->cur,
+                                             s->stream));
+            s->n = n2;
+            ++t;
+            continue;
+        }
+        if (s->schedule != 1) {
+            const T *in = static_cast<const T *>(s->bank(s->cur));
+            T *out = static_cast<T *>(s->bank(s->cur ^ 1u));
             const hipError_t e = tgv
                 ? launch_fused<T, true>(in, out, s->N, f, s->mask(), g, s->shape, dc, pc, stop, part, s->stream)
                 : launch_fused<T, false>(in, out, s->N, f, s->mask(), g, s->shape, dc, pc, stop, part, s->stream);
             QSM_HIP(e);
+            s->cur ^= 1u;
             nblk = fused_blocks(g, s->shape);
         } else {
             hipError_t e = tgv
@@ -177,7 +229,7 @@ int iterate_impl(qsm_pd *s, uint32_t niter, double tol)
             // An empty interior gives nblk == 0: all sums zero and 0/0 norms,
             // as the reference computes them.
             QSM_HIP(launch_conv_finalize(s->partials(), nblk, s->ctl(), s->hist(),
-                                         s->hist_cap, tol, n1, s->stream));
+                                         s->hist_cap, tol, n1, s->cur, s->stream));
         }
         s->n = n1;
     }
@@ -190,9 +242,9 @@ int get_chi_impl(qsm_pd *s, void *chi, int on_device)
     const Bank<T> b0 = make_bank<T>(s->bank(0), s->kind, s->N);
     const Bank<T> b1 = make_bank<T>(s->bank(1), s->kind, s->N);
     if (on_device) {
-        if (s->schedule == 0) {
+        if (s->schedule != 1) {
             QSM_HIP(launch_select_copy<T>(static_cast<T *>(chi), b0.chi, b1.chi,
-                                          s->ctl(), s->n, s->N, s->stream));
+                                          s->ctl(), s->cur, s->N, s->stream));
         } else {
             QSM_HIP(hipMemcpyAsync(chi, b0.chi, s->N * sizeof(T),
                                    hipMemcpyDeviceToDevice, s->stream));
@@ -200,12 +252,12 @@ int get_chi_impl(qsm_pd *s, void *chi, int on_device)
         return QSM_OK;
     }
     const T *src = b0.chi;
-    if (s->schedule == 0) {
+    if (s->schedule != 1) {
         Ctl c;
         QSM_HIP(hipMemcpyAsync(&c, s->ctl(), sizeof c, hipMemcpyDeviceToHost, s->stream));
         QSM_HIP(hipStreamSynchronize(s->stream));
-        const uint32_t it = c.stop ? c.stop_iter : s->n;
-        src = (it & 1u) ? b1.chi : b0.chi;
+        const uint32_t b = c.stop ? c.stop_bank : s->cur;
+        src = (b & 1u) ? b1.chi : b0.chi;
     }
     QSM_HIP(hipMemcpyAsync(chi, src, s->N * sizeof(T), hipMemcpyDeviceToHost,
                            s->stream));
@@ -236,6 +288,8 @@ int create_impl(qsm_pd **out, int kind, int dtype, const size_t sz[3],
     s->alpha = alpha;
     s->stream = as_stream(stream);
     s->shape = default_shape(kind, dtype, Geom{int(sz[0]), int(sz[1]), int(sz[2])});
+    s->kz2 = default_kz2(dtype, Geom{int(sz[0]), int(sz[1]), int(sz[2])});
+    s->cur = 0;
     // Banks above 4 GiB exceed the fused pass's 32-bit offsets: use the
     // two-sweep schedule (same results) for such volumes.
     s->schedule = fused_size_supported(kind, dtype, s->N) ? 0 : 1;
@@ -366,6 +420,7 @@ int qsm_pd_load(qsm_pd *s, const void *d2f, const uint8_t *mask, int on_device)
         QSM_HIP(hipStreamSynchronize(s->stream));  // host buffers may go away
     s->n = 0;
     s->dual_ahead = 0;
+    s->cur = 0;
     s->loaded = 1;
     return QSM_OK;
 }
@@ -427,9 +482,9 @@ void qsm_pd_destroy(qsm_pd *s)
 int qsm_pd_set_schedule(qsm_pd *s, int schedule)
 {
     QSM_ARG(s, "null handle");
-    QSM_ARG(schedule == 0 || schedule == 1, "schedule must be 0 or 1");
+    QSM_ARG(schedule >= 0 && schedule <= 2, "schedule must be 0, 1 or 2");
     QSM_ARG(schedule == 1 || fused_size_supported(s->kind, s->dtype, s->N),
-            "volume too large for the fused schedule (bank >= 4 GiB)");
+            "volume too large for the fused schedules (bank >= 4 GiB)");
     QSM_ARG(s->n == 0 && !s->dual_ahead, "schedule can only change before the first iteration");
     s->schedule = schedule;
     return QSM_OK;
@@ -445,6 +500,17 @@ int qsm_pd_set_tiles(qsm_pd *s, int tx, int ty, int kz)
             "tile shape needs %d partial-sum slots, %d allocated",
             fused_blocks(s->geom(), sh), s->npart);
     s->shape = sh;
+    return QSM_OK;
+}
+
+int qsm_pd_set_kz2(qsm_pd *s, int kz)
+{
+    QSM_ARG(s, "null handle");
+    QSM_ARG(kz >= 1, "kz must be >= 1");
+    QSM_ARG(fused2_blocks(s->geom(), s->dtype, kz) <= s->npart,
+            "chunk depth needs %d partial-sum slots, %d allocated",
+            fused2_blocks(s->geom(), s->dtype, kz), s->npart);
+    s->kz2 = kz;
     return QSM_OK;
 }
 
@@ -651,7 +717,7 @@ int qsm_convergence_dev(int dtype, const void *chi, const void *chi0,
                                         static_cast<const float *>(chi0),
                                         static_cast<const float *>(eta),
                                         static_cast<const float *>(eta0), N, part, nblk, st));
-    QSM_HIP(launch_conv_finalize(part, nblk, ctl, nullptr, 0, tol, 0, st));
+    QSM_HIP(launch_conv_finalize(part, nblk, ctl, nullptr, 0, tol, 0, 0, st));
     Ctl c;
     QSM_HIP(hipMemcpyAsync(&c, ctl, sizeof c, hipMemcpyDeviceToHost, st));
     QSM_HIP(hipStreamSynchronize(st));

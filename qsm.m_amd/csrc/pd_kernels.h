@@ -27,7 +27,7 @@ struct Ctl {
     int stop;            // set by the tolerance test; later kernels no-op
     uint32_t stop_iter;  // iteration the test stopped at
     uint32_t checks;     // convergence checks done so far
-    uint32_t pad;
+    uint32_t stop_bank;  // bank holding chi of stop_iter (fused schedules)
     double nr[2];        // last n1, n2
 };
 
@@ -79,13 +79,26 @@ hipError_t launch_fused(const T *in, T *out, size_t N, const T *f,
 
 int fused_blocks(Geom g, FusedShape sh);
 bool fused_shape_supported(int tx, int ty);
+
+// Two-iteration pass (fused2.hip): reads bank `in` (primal n-1, dual n)
+// and writes bank `out` (primal n+1, dual n+2); partials (if non-null)
+// receive the convergence sums of primal n+1.  Tiles of fused2_tile(dtype)
+// marching kz planes.
+template <typename T, bool TGV>
+hipError_t launch_fused2(const T *in, T *out, size_t N, const T *f,
+                         const uint8_t *mk, Geom g, int kz,
+                         const DualCoef<T> &dc, const PrimalCoef<T> &pc,
+                         const int *stop, double *partials, hipStream_t st);
+void fused2_tile(int dtype, int &tx, int &ty);
+int fused2_blocks(Geom g, int dtype, int kz);
 // The fused pass addresses a bank with 32-bit byte offsets (< 4 GiB).
 bool fused_size_supported(int kind, int dtype, size_t N);
 
-// Reduces nblk partial sums (4 doubles each) and applies the tolerance test.
+// Reduces nblk partial sums (4 doubles each) and applies the tolerance test
+// for iteration iter, whose chi is in bank `bank` (recorded on a stop).
 hipError_t launch_conv_finalize(const double *partials, int nblk, Ctl *ctl,
                                 double *hist, int hist_cap, double tol,
-                                uint32_t iter, hipStream_t st);
+                                uint32_t iter, uint32_t bank, hipStream_t st);
 
 // Generic 4-sum over N voxels of (chi, chi0, eta, eta0) into partials.
 template <typename T>
@@ -102,11 +115,11 @@ hipError_t launch_mask_init(uint8_t *out, const uint8_t *m, Geom g, int raw,
 // Resets a Ctl: no stop, no checks, last norms 1.0 (pd_tgv_mex.c:287-288).
 hipError_t launch_ctl_reset(Ctl *ctl, int n, hipStream_t st);
 
-// dst = chi of the bank that holds the result: bank (it & 1) where it is the
-// stop iteration when ctl stopped the loop, else n_enq (fused schedule).
+// dst = chi of the bank that holds the result: ctl->stop_bank when the
+// tolerance test stopped the loop, else cur_bank (fused schedules).
 template <typename T>
 hipError_t launch_select_copy(T *dst, const T *chi0, const T *chi1,
-                              const Ctl *ctl, uint32_t n_enq, size_t N,
+                              const Ctl *ctl, uint32_t cur_bank, size_t N,
                               hipStream_t st);
 
 // f = sg * d2f (pd_tgv_mex.c:280-283 / :186-189).

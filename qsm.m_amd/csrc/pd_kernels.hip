@@ -197,7 +197,7 @@ constexpr int FIN_T = 256;
 
 __global__ __launch_bounds__(FIN_T) void conv_finalize_kernel(
     const double *partials, int nblk, Ctl *ctl, double *hist, int hist_cap,
-    double tol, uint32_t iter)
+    double tol, uint32_t iter, uint32_t bank)
 {
     if (ctl->stop)
         return;
@@ -223,16 +223,17 @@ __global__ __launch_bounds__(FIN_T) void conv_finalize_kernel(
         if ((n1 < tol) && (n2 < 2.0 * tol)) {
             ctl->stop = 1;
             ctl->stop_iter = iter;
+            ctl->stop_bank = bank;
         }
     }
 }
 
 hipError_t launch_conv_finalize(const double *partials, int nblk, Ctl *ctl,
                                 double *hist, int hist_cap, double tol,
-                                uint32_t iter, hipStream_t st)
+                                uint32_t iter, uint32_t bank, hipStream_t st)
 {
     conv_finalize_kernel<<<1, FIN_T, 0, st>>>(partials, nblk, ctl, hist,
-                                              hist_cap, tol, iter);
+                                              hist_cap, tol, iter, bank);
     return hipGetLastError();
 }
 
@@ -301,7 +302,7 @@ __global__ void ctl_reset_kernel(Ctl *ctl, int n)
     ctl[v].stop = 0;
     ctl[v].stop_iter = 0;
     ctl[v].checks = 0;
-    ctl[v].pad = 0;
+    ctl[v].stop_bank = 0;
     ctl[v].nr[0] = 1.0;
     ctl[v].nr[1] = 1.0;
 }
@@ -314,10 +315,10 @@ hipError_t launch_ctl_reset(Ctl *ctl, int n, hipStream_t st)
 
 template <typename T>
 __global__ void select_copy_kernel(T *dst, const T *chi0, const T *chi1,
-                                   const Ctl *ctl, uint32_t n_enq, size_t N)
+                                   const Ctl *ctl, uint32_t cur_bank, size_t N)
 {
-    const uint32_t it = ctl->stop ? ctl->stop_iter : n_enq;
-    const T *src = (it & 1u) ? chi1 : chi0;
+    const uint32_t b = ctl->stop ? ctl->stop_bank : cur_bank;
+    const T *src = (b & 1u) ? chi1 : chi0;
     const size_t stride = size_t(gridDim.x) * blockDim.x;
     for (size_t l = size_t(blockIdx.x) * blockDim.x + threadIdx.x; l < N; l += stride)
         dst[l] = src[l];
@@ -325,10 +326,10 @@ __global__ void select_copy_kernel(T *dst, const T *chi0, const T *chi1,
 
 template <typename T>
 hipError_t launch_select_copy(T *dst, const T *chi0, const T *chi1,
-                              const Ctl *ctl, uint32_t n_enq, size_t N,
+                              const Ctl *ctl, uint32_t cur_bank, size_t N,
                               hipStream_t st)
 {
-    select_copy_kernel<T><<<1024, 256, 0, st>>>(dst, chi0, chi1, ctl, n_enq, N);
+    select_copy_kernel<T><<<1024, 256, 0, st>>>(dst, chi0, chi1, ctl, cur_bank, N);
     return hipGetLastError();
 }
 

@@ -116,11 +116,19 @@ class Solver:
         return self
 
     def set_schedule(self, schedule: int):
+        """0: two-iteration passes (default), 1: the reference's two sweeps,
+        2: one-iteration passes only; identical results."""
         check(lib().qsm_pd_set_schedule(self._h, int(schedule)))
         return self
 
     def set_tiles(self, tx: int, ty: int, kz: int):
+        """Tile shape of the one-iteration fused pass."""
         check(lib().qsm_pd_set_tiles(self._h, int(tx), int(ty), int(kz)))
+        return self
+
+    def set_kz2(self, kz: int):
+        """z-chunk depth of the two-iteration fused pass."""
+        check(lib().qsm_pd_set_kz2(self._h, int(kz)))
         return self
 
     def autotune(self):

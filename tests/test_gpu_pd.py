@@ -172,7 +172,7 @@ def _phantom_input(shape, h, seed, dt):
 
 @pytest.mark.parametrize("dt", DTYPES)
 @pytest.mark.parametrize("kind", ["tgv", "tv"])
-@pytest.mark.parametrize("schedule", [0, 1])
+@pytest.mark.parametrize("schedule", [0, 1, 2])
 def test_whole_solve_fixed_iterations(cuda, dt, kind, schedule):
     shape, h = (26, 23, 21), (0.9, 0.9, 1.2)
     d2f = _phantom_input(shape, h, 11, dt)
@@ -216,12 +216,13 @@ def test_fused_vs_two_sweep_c1_size(cuda):
     m = brainish_mask(shape, seed=7, frac=0.9)
     for kind in (_lib.TGV, _lib.TV):
         outs = []
-        for sched in (0, 1):
+        for sched in (0, 1, 2):
             with pd.Solver(kind, np.float64, shape, h, 1e-3, 2.0) as s:
                 s.set_schedule(sched)
                 s.load(d2f, m).iterate(60, 0.0)
                 outs.append(s.chi())
         np.testing.assert_array_equal(outs[0], outs[1])
+        np.testing.assert_array_equal(outs[2], outs[1])
 
 
 @pytest.mark.parametrize("dt", DTYPES)
@@ -247,3 +248,62 @@ def test_fused_tile_shapes(cuda, dt, kind):
                 got = s.chi()
             np.testing.assert_array_equal(got, ref, err_msg=f"tiles {tx}x{ty} kz={kz}")
 
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+@pytest.mark.parametrize("kind", ["tgv", "tv"])
+def test_two_iteration_pass_chunks_and_partial_tiles(cuda, dt, kind):
+    """The two-iteration pass with partial tiles in x, y and z (70x45x33, a
+    holed mask) and z-chunks of 1, 2, 5 and 31 planes (chunk edges inside
+    the four-stage pipeline) matches the oracle bit for bit."""
+    shape, h = (70, 45, 33), (0.9, 1.0, 1.1)
+    d2f = _phantom_input(shape, h, 15, dt)
+    m = brainish_mask(shape, seed=9, frac=0.95)
+    iters = 24
+    k = _lib.TGV if kind == "tgv" else _lib.TV
+    if kind == "tgv":
+        ref, _, _ = orc.pd_tgv(d2f, m, h, 2e-3, 2.0, 0.0, iters)
+    else:
+        ref, _, _ = orc.pd_tv(d2f, m, h, 2e-3, 0.0, iters)
+    for kz in (1, 2, 5, 31):
+        with pd.Solver(k, dt, shape, h, 2e-3, 2.0) as s:
+            s.set_kz2(kz)
+            s.load(d2f, m).iterate(iters, 0.0)
+            got = s.chi()
+        np.testing.assert_array_equal(got, ref, err_msg=f"kz2={kz}")
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+def test_two_iteration_pass_uneven_calls_and_stop(cuda, dt):
+    """Iterations enqueued in uneven calls (3, 7, 13, ...: a check then falls
+    on the first iteration of a would-be pair, which runs alone) and a
+    tolerance stop: same stopping iteration and chi as the oracle."""
+    shape, h = (30, 28, 26), (1.0, 1.0, 1.0)
+    d2f = _phantom_input(shape, h, 16, dt)
+    m = brainish_mask(shape, seed=10, frac=0.9)
+    ref, it, _ = orc.pd_tgv(d2f, m, h, 1e-3, 2.0, 0.0, 57)
+    with pd.Solver(_lib.TGV, dt, shape, h, 1e-3, 2.0) as s:
+        s.load(d2f, m)
+        for n in (3, 7, 13, 1, 2, 31):
+            s.iterate(n, 0.0)
+        got = s.chi()
+        assert s.status()[0] == 57
+    np.testing.assert_array_equal(got, ref)
+    ref, it, nr = orc.pd_tgv(d2f, m, h, 1e-3, 2.0, 5e-3, 2000)
+    assert 10 <= it < 2000
+    with pd.Solver(_lib.TGV, dt, shape, h, 1e-3, 2.0) as s:
+        s.load(d2f, m)
+        for n in (9, 4, 1987):
+            s.iterate(n, 5e-3)
+        got = s.chi()
+        it_g, stopped, nr_g = s.status()
+    assert stopped and it_g == it
+    np.testing.assert_array_equal(got, ref)
+    import torch
+
+    with pd.Solver(_lib.TGV, dt, shape, h, 1e-3, 2.0) as s:
+        s.load(d2f, m).iterate(2000, 5e-3)
+        dev = torch.zeros(int(np.prod(shape)), device="cuda",
+                          dtype=torch.float64 if dt == np.float64 else torch.float32)
+        s.chi(out=dev)  # device copy of the bank the stop recorded
+        np.testing.assert_array_equal(from_dev(dev, shape), ref)
