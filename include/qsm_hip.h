@@ -142,6 +142,33 @@ int qsm_pd_get_tiles(qsm_pd *s, int *tx, int *ty, int *kz);
 int qsm_pd_set_kz2(qsm_pd *s, int kz);
 
 /* ------------------------------------------------------------------------
+ * Allocation-free solve
+ * ------------------------------------------------------------------------ */
+
+/* Bytes of the library's static device work space (per GPU, held from the
+ * library's first use of that GPU until the process ends). */
+#define QSM_PD_STATIC_BYTES (64u << 20)
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The solve of qsm_pd_tgv (kind QSM_TGV, alpha0 =
+ * alpha*lam, alpha1 = lam) or qsm_pd_tv (kind QSM_TV, alpha ignored) with
+ * bitwise the same chi, but on DEVICE pointers d2f, mask and chi of the
+ * current device, asynchronous on stream (NULL: the null stream), without
+ * verbose output, and with all solver state -- both banks, f, the packed
+ * mask, control block, partial sums -- in the static work space instead of
+ * an allocation: no hipMalloc, hipFree or memset / memcpy node is issued,
+ * only kernels.  Volumes whose state exceeds QSM_PD_STATIC_BYTES are
+ * refused with QSM_EARG (cubes up to about 81^3 voxels for FP32 TGV,
+ * 64^3 for FP64 TGV, 108^3 for FP32 TV and 86^3 for FP64 TV).  Every
+ * call on one GPU shares its work space and leaves its contents there:
+ * calls on one stream run one after another, calls that overlap on one
+ * GPU (other streams, other threads) race and must not be issued. */
+int qsm_pd_solve_static(int kind, int dtype, const void *d2f, const uint8_t *mask,
+                        const size_t sz[3], const double h[3], double lam, double alpha,
+                        double tol, uint32_t maxiter, void *chi, void *stream);
+
+/* ------------------------------------------------------------------------
  * Primal-dual single sweeps on device arrays (per-operator access; the
  * layouts are the reference's, asynchronous on stream)
  * ------------------------------------------------------------------------ */

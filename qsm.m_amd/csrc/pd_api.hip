@@ -4,12 +4,17 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
 #include "pd_kernels.h"
 
 using namespace qsm;
+
+// The allocation-free solve's device work space (qsm_pd_solve_static): one
+// copy per GPU, its address looked up per device with hipGetSymbolAddress.
+static __device__ __align__(256) char scratch[QSM_PD_STATIC_BYTES];
 
 namespace {
 
@@ -366,9 +371,113 @@ int pd_solve(int kind, int dtype, const void *d2f, const uint8_t *mask,
     return rc;
 }
 
+// ---- the allocation-free solve ------------------------------------------
+//
+// Every piece of one solve's device state (the two banks, f, the packed
+// mask, the control block, the partial sums and one norm-history slot) is
+// laid out in `scratch`, a device array of QSM_PD_STATIC_BYTES that each
+// GPU the library runs on holds until the process ends.  Its address is
+// looked up once per device (hipGetSymbolAddress with that device
+// current) and kept in scratch_ptr; host code uses only that address.
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+void *scratch_ptr[MAX_DEVICES] = {};
+
+int scratch_of_current_device(void **out)
+{
+    int dev = 0;
+    QSM_HIP(hipGetDevice(&dev));
+    QSM_ARG(dev >= 0 && dev < MAX_DEVICES, "device %d beyond the %d the library tracks",
+            dev, MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void *p = nullptr;
+        QSM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        QSM_ARG(p != nullptr, "no device address for the static work space");
+        scratch_ptr[dev] = p;
+    }
+    *out = scratch_ptr[dev];
+    return QSM_OK;
+}
+
+template <typename T>
+int solve_static_impl(qsm_pd *s, const T *d2f, const uint8_t *mask, double tol,
+                      uint32_t maxiter, T *chi)
+{
+    const hipStream_t st = s->stream;
+    // qsm_pd_load's work with kernels only (no memset or memcpy nodes when
+    // a caller captures the call): zero banks, packed mask straight from
+    // the caller's mask, f = sigma * d2f, control block reset
+    QSM_HIP(launch_zero_bytes(s->bank(0), 2 * s->L.bank_bytes, st));
+    QSM_HIP(launch_mask_init(s->mask(), mask, s->geom(), 0, st));
+    QSM_HIP(launch_scale<T>(static_cast<T *>(s->f()), d2f, T(s->sigma), s->N, st));
+    QSM_HIP(launch_ctl_reset(s->ctl(), 1, st));
+    s->n = 0;
+    s->dual_ahead = 0;
+    s->cur = 0;
+    s->loaded = 1;
+    int rc = iterate_impl<T>(s, maxiter, tol);
+    if (rc) return rc;
+    const Bank<T> b0 = make_bank<T>(s->bank(0), s->kind, s->N);
+    const Bank<T> b1 = make_bank<T>(s->bank(1), s->kind, s->N);
+    QSM_HIP(launch_select_copy<T>(chi, b0.chi, b1.chi, s->ctl(), s->cur, s->N, st));
+    return QSM_OK;
+}
+
+int pd_solve_static(int kind, int dtype, const void *d2f, const uint8_t *mask,
+                    const size_t sz[3], const double h[3], double lam, double alpha,
+                    double tol, uint32_t maxiter, void *chi, void *stream)
+{
+    QSM_ARG(d2f && mask && chi, "null input or output pointer");
+    QSM_ARG(kind == QSM_TV || kind == QSM_TGV, "kind must be QSM_TV or QSM_TGV");
+    QSM_ARG(dtype == QSM_F64 || dtype == QSM_F32, "dtype must be QSM_F64 or QSM_F32");
+    QSM_ARG(maxiter > 0, "maxiter must be > 0");
+    int rc = check_grid(sz, h, 1);
+    if (rc) return rc;
+    if ((rc = ensure_device())) return rc;
+
+    qsm_pd s{};
+    s.kind = kind;
+    s.dtype = dtype;
+    for (int a = 0; a < 3; ++a) { s.sz[a] = sz[a]; s.h[a] = h[a]; }
+    s.N = sz[0] * sz[1] * sz[2];
+    s.lam = lam;
+    s.alpha = alpha;
+    s.stream = as_stream(stream);
+    s.shape = default_shape(kind, dtype, s.geom());
+    s.kz2 = default_kz2(dtype, s.geom());
+    s.cur = 0;
+    s.schedule = 0;
+    qsm_step_sizes(h, &s.tau, &s.sigma);
+    s.npart = max_partials(s.geom());
+    s.hist_cap = 1;
+    s.L = layout(kind, dtype, sz, s.npart, s.hist_cap);
+    QSM_ARG(s.L.total <= size_t(QSM_PD_STATIC_BYTES),
+            "volume %zux%zux%zu needs %zu bytes of solver state, more than the %zu-byte "
+            "static work space", sz[0], sz[1], sz[2], s.L.total,
+            size_t(QSM_PD_STATIC_BYTES));
+    void *base = nullptr;
+    if ((rc = scratch_of_current_device(&base))) return rc;
+    s.mem = static_cast<char *>(base);
+    return dtype == QSM_F64
+        ? solve_static_impl<double>(&s, static_cast<const double *>(d2f), mask, tol, maxiter,
+                                    static_cast<double *>(chi))
+        : solve_static_impl<float>(&s, static_cast<const float *>(d2f), mask, tol, maxiter,
+                                   static_cast<float *>(chi));
+}
+
 }  // namespace
 
 extern "C" {
+
+int qsm_pd_solve_static(int kind, int dtype, const void *d2f, const uint8_t *mask,
+                        const size_t sz[3], const double h[3], double lam, double alpha,
+                        double tol, uint32_t maxiter, void *chi, void *stream)
+{
+    return pd_solve_static(kind, dtype, d2f, mask, sz, h, lam, alpha, tol, maxiter, chi,
+                           stream);
+}
 
 void qsm_step_sizes(const double h[3], double *tau, double *sigma)
 {

@@ -122,6 +122,9 @@ hipError_t launch_select_copy(T *dst, const T *chi0, const T *chi1,
                               const Ctl *ctl, uint32_t cur_bank, size_t N,
                               hipStream_t st);
 
+// Zeroes `bytes` bytes at p (p 16-byte aligned) with a kernel.
+hipError_t launch_zero_bytes(void *p, size_t bytes, hipStream_t st);
+
 // f = sg * d2f (pd_tgv_mex.c:280-283 / :186-189).
 template <typename T>
 hipError_t launch_scale(T *f, const T *src, T sg, size_t N, hipStream_t st);

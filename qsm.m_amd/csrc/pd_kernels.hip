@@ -333,6 +333,29 @@ hipError_t launch_select_copy(T *dst, const T *chi0, const T *chi1,
     return hipGetLastError();
 }
 
+// Clears n16 16-byte words and then `tail` trailing bytes (a fill kernel,
+// so that code a caller captures into a graph records no memset node).
+__global__ void zero_bytes_kernel(uint4 *p, size_t n16, uint8_t *tail, int ntail)
+{
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (size_t l = g; l < n16; l += stride)
+        p[l] = make_uint4(0u, 0u, 0u, 0u);
+    if (g < size_t(ntail))
+        tail[g] = 0;
+}
+
+hipError_t launch_zero_bytes(void *p, size_t bytes, hipStream_t st)
+{
+    if (bytes == 0)
+        return hipSuccess;
+    const size_t n16 = bytes / 16;
+    uint8_t *tail = static_cast<uint8_t *>(p) + n16 * 16;
+    zero_bytes_kernel<<<2048, 256, 0, st>>>(static_cast<uint4 *>(p), n16, tail,
+                                            int(bytes - n16 * 16));
+    return hipGetLastError();
+}
+
 template <typename T>
 __global__ void scale_kernel(T *f, const T *src, T sg, size_t N)
 {

@@ -74,6 +74,8 @@ _SIGS = {
     "qsm_convergence_dev": (C.c_int, [C.c_int, _p, _p, _p, _p, _sz_t, C.c_double,
                                       _dp, C.POINTER(C.c_int), _p]),
     "qsm_step_sizes": (None, [_dp, _dp, _dp]),
+    "qsm_pd_solve_static": (C.c_int, [C.c_int, C.c_int, _p, _u8p, C.POINTER(_sz_t), _dp,
+                                      C.c_double, C.c_double, C.c_double, C.c_uint32, _p, _p]),
     "qsm_copy_dev": (C.c_int, [_p, _p, _sz_t, _p]),
 }
 for _n in ("gradfp", "gradf"):

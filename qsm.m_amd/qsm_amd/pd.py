@@ -70,6 +70,28 @@ def pd_tv_mex(d2f, mask, vsz, lam, tol, maxiter, verbose=0):
     return chi
 
 
+def solve_static(kind: int, d2f, mask, shape, vsz, lam, alpha, tol, maxiter, chi, stream=None):
+    """qsm_pd_solve_static: the solve of pd_tgv_mex (kind TGV) or pd_tv_mex
+    (kind TV) on torch device tensors holding column-major volumes of
+    `shape` (d2f and chi of one float dtype, mask uint8), with the solver
+    state in the library's static work space (no device allocation).
+    Asynchronous on `stream` (a torch.cuda.Stream, or None: the null
+    stream); returns chi."""
+    import torch
+
+    if d2f.dtype not in (torch.float32, torch.float64) or chi.dtype != d2f.dtype:
+        raise ValueError("d2f and chi must share one float dtype")
+    n = int(np.prod(shape))
+    if d2f.numel() != n or chi.numel() != n or mask.numel() != n or mask.dtype != torch.uint8:
+        raise ValueError("d2f, mask (uint8) and chi must hold prod(shape) elements")
+    code = _lib.F32 if d2f.dtype == torch.float32 else _lib.F64
+    sp = stream.cuda_stream if stream is not None else None
+    check(lib().qsm_pd_solve_static(int(kind), code, d2f.data_ptr(), mask.data_ptr(),
+                                    sz3(shape), h3(vsz), float(lam), float(alpha), float(tol),
+                                    int(maxiter), chi.data_ptr(), sp))
+    return chi
+
+
 def step_sizes(vsz):
     """(tau, sigma) of pd_tgv_mex.c:272-278."""
     t, s = C.c_double(), C.c_double()
