@@ -57,6 +57,41 @@ struct Old {
 template <int BX, int BY>
 constexpr int threads2() { return (BX * BY + 63) / 64 * 64; }
 
+// Thread -> column of the region.  The owned TX x TY tile comes first, row
+// by row, so its stores fill whole waves and whole cache lines (TX = 32
+// floats or 16 doubles: 128 bytes); then the rings from the inside out
+// (depth 2, 1, 0; each a rectangle's perimeter: top row, bottom row, left
+// column, right column), so that the waves of the outer rings have no work
+// in the later stages and skip them as a whole.  dep = -1: padding thread.
+template <int BX, int BY>
+__device__ __forceinline__ void region_cell(int t, int &tx, int &ty, int &dep)
+{
+    constexpr int TX = BX - 6, TY = BY - 6;
+    if (t < TX * TY) {
+        tx = 3 + t % TX;
+        ty = 3 + t / TX;
+        dep = 3;
+        return;
+    }
+    int r = t - TX * TY;
+    for (int d = 2; d >= 0; --d) {
+        const int w = BX - 2 * d, h = BY - 2 * d;
+        const int per = 2 * w + 2 * (h - 2);
+        if (r < per) {
+            dep = d;
+            if (r < w) { tx = d + r; ty = d; }
+            else if (r < 2 * w) { tx = d + r - w; ty = BY - 1 - d; }
+            else if (r < 2 * w + h - 2) { tx = d; ty = d + 1 + (r - 2 * w); }
+            else { tx = BX - 1 - d; ty = d + 1 + (r - 2 * w - (h - 2)); }
+            return;
+        }
+        r -= per;
+    }
+    tx = 0;
+    ty = 0;
+    dep = -1;
+}
+
 template <typename T, bool TGV, int BX, int BY, bool CHECK>
 __global__ __launch_bounds__((BX * BY + 63) / 64 * 64) void fused2_kernel(
     const T *in_base, T *out_base, uint32_t N, const T *f, const uint8_t *mk, Geom g,
@@ -91,11 +126,13 @@ __global__ __launch_bounds__((BX * BY + 63) / 64 * 64) void fused2_kernel(
     c.nz = g.nz;
 
     const int t = threadIdx.x;
-    const int tx = t % BX, ty = t / BX;
+    int tx, ty, dep;
+    region_cell<BX, BY>(t, tx, ty, dep);
+    // LDS position of the thread's column (padding threads: a private slot
+    // past the region)
+    const int lo = dep >= 0 ? ty * BX + tx : t;
     const int x = bx * TX - 3 + tx, y = by * TY - 3 + ty;
-    const bool col = t < BX * BY;  // a column of the region (else padding)
-    const int dep = col ? min(min(tx, BX - 1 - tx), min(ty, BY - 1 - ty)) : -1;
-    const bool valid = col && x >= 0 && x < g.nx && y >= 0 && y < g.ny;
+    const bool valid = dep >= 0 && x >= 0 && x < g.nx && y >= 0 && y < g.ny;
     const bool own = valid && dep >= 3;
     const int k0 = 1 + bz * kz;
     const int k1 = min(k0 + kz, g.nz - 1);  // owned planes [k0, k1)
@@ -127,41 +164,41 @@ __global__ __launch_bounds__((BX * BY + 63) / 64 * 64) void fused2_kernel(
         bool pact;
         POut<T> o;
         pcol<T, TGV>(c, pc, p, true, q, dv, pact, o);
-        L.e1[s][0][t] = o.cb;
-        L.e1[s][1][t] = o.eb;
+        L.e1[s][0][lo] = o.cb;
+        L.e1[s][1][lo] = o.eb;
         if constexpr (TGV) {
-            L.e1[s][2][t] = o.b1;
-            L.e1[s][3][t] = o.b2;
-            L.e1[s][4][t] = o.b3;
+            L.e1[s][2][lo] = o.b1;
+            L.e1[s][3][lo] = o.b2;
+            L.e1[s][4][lo] = o.b3;
         }
 
         // ---- stage 2: dual n+1 at plane p-1 (depth >= 1) -----------------
         const uint8_t m2 = uint8_t(mq);
         DualVox<T> d2 = dv1;
         if (p - 1 >= k0 - 2 && dep >= 1 && (m2 & 2)) {
-            const Star<T> cs{L.e1[ps][0][t], L.e1[ps][0][t - 1], L.e1[ps][0][t + 1],
-                             L.e1[ps][0][t - BX], L.e1[ps][0][t + BX], e1m_c, o.cb};
-            const Star<T> es{L.e1[ps][1][t], L.e1[ps][1][t - 1], L.e1[ps][1][t + 1],
-                             L.e1[ps][1][t - BX], L.e1[ps][1][t + BX], e1m_e, o.eb};
+            const Star<T> cs{L.e1[ps][0][lo], L.e1[ps][0][lo - 1], L.e1[ps][0][lo + 1],
+                             L.e1[ps][0][lo - BX], L.e1[ps][0][lo + BX], e1m_c, o.cb};
+            const Star<T> es{L.e1[ps][1][lo], L.e1[ps][1][lo - 1], L.e1[ps][1][lo + 1],
+                             L.e1[ps][1][lo - BX], L.e1[ps][1][lo + BX], e1m_e, o.eb};
             d2.nu = dual_nu(dc, d2.nu, es, cs, fa);
             if constexpr (TGV) {
                 tgv_dual_pq(dc, d2, cs,
-                            L.e1[ps][2][t], L.e1[ps][2][t + 1], L.e1[ps][2][t + BX], o.b1,
-                            L.e1[ps][3][t], L.e1[ps][3][t + 1], L.e1[ps][3][t + BX], o.b2,
-                            L.e1[ps][4][t], L.e1[ps][4][t + 1], L.e1[ps][4][t + BX], o.b3);
+                            L.e1[ps][2][lo], L.e1[ps][2][lo + 1], L.e1[ps][2][lo + BX], o.b1,
+                            L.e1[ps][3][lo], L.e1[ps][3][lo + 1], L.e1[ps][3][lo + BX], o.b2,
+                            L.e1[ps][4][lo], L.e1[ps][4][lo + 1], L.e1[ps][4][lo + BX], o.b3);
             } else {
                 tv_dual_p(dc, d2, cs);
             }
         }
-        L.d[s][0][t] = d2.nu;
-        L.d[s][1][t] = d2.p1;
-        L.d[s][2][t] = d2.p2;
+        L.d[s][0][lo] = d2.nu;
+        L.d[s][1][lo] = d2.p1;
+        L.d[s][2][lo] = d2.p2;
         if constexpr (TGV) {
-            L.d[s][3][t] = d2.q1;
-            L.d[s][4][t] = d2.q2;
-            L.d[s][5][t] = d2.q3;
-            L.d[s][6][t] = d2.q4;
-            L.d[s][7][t] = d2.q5;
+            L.d[s][3][lo] = d2.q1;
+            L.d[s][4][lo] = d2.q2;
+            L.d[s][5][lo] = d2.q3;
+            L.d[s][6][lo] = d2.q4;
+            L.d[s][7][lo] = d2.q5;
         }
 
         // ---- stage 3: primal n+1 at plane p-2 (depth >= 2) ---------------
@@ -170,17 +207,17 @@ __global__ __launch_bounds__((BX * BY + 63) / 64 * 64) void fused2_kernel(
         T c3 = T(0), e3 = T(0), b31 = T(0), b32 = T(0), b33 = T(0);
         if (k3 >= k0 - 1 && dep >= 2 && (m3 & 1)) {
             PrimalIn<T> pi;
-            pi.nu = Star<T>{da.nu, L.d[ps][0][t - 1], L.d[ps][0][t + 1], L.d[ps][0][t - BX],
-                            L.d[ps][0][t + BX], db.nu, d2.nu};
-            pi.p1c = da.p1; pi.p1xm = L.d[ps][1][t - 1];
-            pi.p2c = da.p2; pi.p2ym = L.d[ps][2][t - BX];
+            pi.nu = Star<T>{da.nu, L.d[ps][0][lo - 1], L.d[ps][0][lo + 1], L.d[ps][0][lo - BX],
+                            L.d[ps][0][lo + BX], db.nu, d2.nu};
+            pi.p1c = da.p1; pi.p1xm = L.d[ps][1][lo - 1];
+            pi.p2c = da.p2; pi.p2ym = L.d[ps][2][lo - BX];
             pi.p3c = da.p3; pi.p3zm = db.p3;
             if constexpr (TGV) {
-                pi.q1c = da.q1; pi.q1xm = L.d[ps][3][t - 1];
-                pi.q2c = da.q2; pi.q2xm = L.d[ps][4][t - 1]; pi.q2ym = L.d[ps][4][t - BX];
-                pi.q3c = da.q3; pi.q3xm = L.d[ps][5][t - 1]; pi.q3zm = db.q3;
-                pi.q4c = da.q4; pi.q4ym = L.d[ps][6][t - BX];
-                pi.q5c = da.q5; pi.q5ym = L.d[ps][7][t - BX]; pi.q5zm = db.q5;
+                pi.q1c = da.q1; pi.q1xm = L.d[ps][3][lo - 1];
+                pi.q2c = da.q2; pi.q2xm = L.d[ps][4][lo - 1]; pi.q2ym = L.d[ps][4][lo - BX];
+                pi.q3c = da.q3; pi.q3xm = L.d[ps][5][lo - 1]; pi.q3zm = db.q3;
+                pi.q4c = da.q4; pi.q4ym = L.d[ps][6][lo - BX];
+                pi.q5c = da.q5; pi.q5ym = L.d[ps][7][lo - BX]; pi.q5zm = db.q5;
                 pi.q6c = da.q6; pi.q6zm = db.q6;
             }
             T en, cn;
@@ -207,12 +244,12 @@ __global__ __launch_bounds__((BX * BY + 63) / 64 * 64) void fused2_kernel(
                     conv_terms(cn, ob.chi, en, ob.eta, acc);
             }
         }
-        L.e3[s][0][t] = c3;
-        L.e3[s][1][t] = e3;
+        L.e3[s][0][lo] = c3;
+        L.e3[s][1][lo] = e3;
         if constexpr (TGV) {
-            L.e3[s][2][t] = b31;
-            L.e3[s][3][t] = b32;
-            L.e3[s][4][t] = b33;
+            L.e3[s][2][lo] = b31;
+            L.e3[s][3][lo] = b32;
+            L.e3[s][4][lo] = b33;
         }
 
         // ---- stage 4: dual n+2 at plane p-3 (the tile) --------------------
@@ -220,17 +257,17 @@ __global__ __launch_bounds__((BX * BY + 63) / 64 * 64) void fused2_kernel(
         const uint8_t m4 = uint8_t(mq >> 16);
         if (k4 >= k0 && own && (m4 & 2)) {
             DualVox<T> d4 = db;
-            const Star<T> cs{L.e3[ps][0][t], L.e3[ps][0][t - 1], L.e3[ps][0][t + 1],
-                             L.e3[ps][0][t - BX], L.e3[ps][0][t + BX], e3m_c, c3};
-            const Star<T> es{L.e3[ps][1][t], L.e3[ps][1][t - 1], L.e3[ps][1][t + 1],
-                             L.e3[ps][1][t - BX], L.e3[ps][1][t + BX], e3m_e, e3};
+            const Star<T> cs{L.e3[ps][0][lo], L.e3[ps][0][lo - 1], L.e3[ps][0][lo + 1],
+                             L.e3[ps][0][lo - BX], L.e3[ps][0][lo + BX], e3m_c, c3};
+            const Star<T> es{L.e3[ps][1][lo], L.e3[ps][1][lo - 1], L.e3[ps][1][lo + 1],
+                             L.e3[ps][1][lo - BX], L.e3[ps][1][lo + BX], e3m_e, e3};
             d4.nu = dual_nu(dc, d4.nu, es, cs, fc);
             const uint32_t vb = uint32_t(q.l0 + c.sk * k4) * S;
             if constexpr (TGV) {
                 tgv_dual_pq(dc, d4, cs,
-                            L.e3[ps][2][t], L.e3[ps][2][t + 1], L.e3[ps][2][t + BX], b31,
-                            L.e3[ps][3][t], L.e3[ps][3][t + 1], L.e3[ps][3][t + BX], b32,
-                            L.e3[ps][4][t], L.e3[ps][4][t + 1], L.e3[ps][4][t + BX], b33);
+                            L.e3[ps][2][lo], L.e3[ps][2][lo + 1], L.e3[ps][2][lo + BX], b31,
+                            L.e3[ps][3][lo], L.e3[ps][3][lo + 1], L.e3[ps][3][lo + BX], b32,
+                            L.e3[ps][4][lo], L.e3[ps][4][lo + 1], L.e3[ps][4][lo + BX], b33);
                 bst(d4.q1, c.out, vb, c.so(I::Q1));
                 bst(d4.q2, c.out, vb, c.so(I::Q2));
                 bst(d4.q3, c.out, vb, c.so(I::Q3));
@@ -247,10 +284,10 @@ __global__ __launch_bounds__((BX * BY + 63) / 64 * 64) void fused2_kernel(
         }
 
         // ---- advance the column's queues -----------------------------------
-        e1m_c = L.e1[ps][0][t];
-        e1m_e = L.e1[ps][1][t];
-        e3m_c = L.e3[ps][0][t];
-        e3m_e = L.e3[ps][1][t];
+        e1m_c = L.e1[ps][0][lo];
+        e1m_e = L.e1[ps][1][lo];
+        e3m_c = L.e3[ps][0][lo];
+        e3m_e = L.e3[ps][1][lo];
         dv1 = dv;
         ob = oa;
         oa = Old<T>{o.chi, o.eta, o.w1, o.w2, o.w3};

@@ -594,7 +594,11 @@ int qsm_pd_set_schedule(qsm_pd *s, int schedule)
     QSM_ARG(schedule >= 0 && schedule <= 2, "schedule must be 0, 1 or 2");
     QSM_ARG(schedule == 1 || fused_size_supported(s->kind, s->dtype, s->N),
             "volume too large for the fused schedules (bank >= 4 GiB)");
-    QSM_ARG(s->n == 0 && !s->dual_ahead, "schedule can only change before the first iteration");
+    // the fused schedules share the ping-pong banks and may alternate; the
+    // in-place two-sweep schedule only starts from a fresh load
+    QSM_ARG((s->n == 0 && !s->dual_ahead) || (schedule != 1 && s->schedule != 1),
+            "the two-sweep schedule can only be switched to or from before the first "
+            "iteration");
     s->schedule = schedule;
     return QSM_OK;
 }

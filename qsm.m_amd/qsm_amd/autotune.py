@@ -1,12 +1,14 @@
-"""Autotuner for the fused primal-dual pass: picks the tile shape (tx x ty)
-and z-chunk depth kz per size class (kind, precision, volume shape) by
-timing candidates on the device with HIP events, and keeps the winners in
-a JSON table.  Results never depend on the choice (every shape is bitwise
-identical), only speed does.
+"""Autotuner for the fused primal-dual passes: per size class (kind,
+precision, volume shape) it picks the one-iteration pass's tile shape
+(tx x ty) and z-chunk depth kz, the two-iteration pass's chunk depth kz2,
+and the faster of the two schedules, by timing candidates on the device
+with HIP events, and keeps the winners in a JSON table.  Results never
+depend on the choice (every choice is bitwise identical), only speed does.
 
     from qsm_amd import autotune
     autotune.tune(TGV, np.float32, (256, 256, 176))      # measure, store
     autotune.best(TGV, np.float32, (256, 256, 176))      # (tx, ty, kz)
+    autotune.best2(TGV, np.float32, (256, 256, 176))     # (kz2, schedule)
 
 The packaged table `tuning_mi355x.json` holds the MI355X measurements made
 with `python -m qsm_amd.autotune`; tune() adds to a user table (path in
@@ -57,9 +59,47 @@ def kz_candidates(nz: int, n_tiles_xy: int, ncu: int = 256):
     return sorted(k for k in c if 1 <= k <= inner)
 
 
-def tune(kind, dtype, shape, iters=40, repeats=3, shapes=SHAPES, store=True, log=None):
-    """Times every (tx, ty, kz) candidate on the current device and returns
-    the fastest as dict(tx, ty, kz, ms)."""
+# owned tile of the two-iteration pass per precision (the library's choice)
+TILE2 = {"float32": (32, 16), "float64": (16, 16)}
+
+
+def kz2_candidates(shape, dtype, ncu: int = 256):
+    """Chunk depths of the two-iteration pass worth timing: one to four
+    blocks per CU, and the whole column."""
+    tx, ty = TILE2[np.dtype(dtype).name]
+    nxy = -(-shape[0] // tx) * -(-shape[1] // ty)
+    inner = max(1, shape[2] - 2)
+    c = {inner}
+    for blocks in (ncu, 2 * ncu, 3 * ncu, 4 * ncu):
+        nch = max(1, blocks // max(1, nxy))
+        c.add(-(-inner // nch))
+    return sorted(k for k in c if 1 <= k <= inner)
+
+
+def _timed(s, st, iters, repeats):
+    import torch
+
+    s.iterate(4)
+    st.synchronize()
+    ts = []
+    for _ in range(repeats):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        s.iterate(iters)
+        e1.record(st)
+        e1.synchronize()
+        ts.append(e0.elapsed_time(e1) / iters)
+    return float(np.median(ts))
+
+
+def tune(kind, dtype, shape, iters=40, repeats=3, shapes=SHAPES, store=True, log=None,
+         one=True):
+    """Times every (tx, ty, kz) candidate of the one-iteration pass (unless
+    one=False: then the stored entry's are kept) and every kz2 of the
+    two-iteration pass on the current device; returns the table entry
+    dict(tx, ty, kz, ms, kz2, ms2, schedule), schedule 0 when the
+    two-iteration pass is faster, else 2."""
     import torch
 
     from .pd import Solver
@@ -69,38 +109,56 @@ def tune(kind, dtype, shape, iters=40, repeats=3, shapes=SHAPES, store=True, log
     d2f = np.asfortranarray((rng.standard_normal(shape) * 1e-2).astype(dtype))
     m = np.ones(shape, np.uint8, order="F")
     st = torch.cuda.Stream()
-    best = None
+    k = key(kind, dtype, shape)
+    best = dict(table().get(k, {})) if not one else None
     with Solver(kind, dtype, shape, (1, 1, 1), 1e-3, 2.0, stream=st) as s:
         s.load(d2f, m)
-        for tx, ty in shapes:
-            nxy = -(-shape[0] // tx) * -(-shape[1] // ty)
-            for kz in kz_candidates(shape[2], nxy):
-                try:
-                    s.set_tiles(tx, ty, kz)
-                except _lib.QsmError:
-                    continue
-                s.iterate(4)
-                st.synchronize()
-                ts = []
-                for _ in range(repeats):
-                    e0 = torch.cuda.Event(enable_timing=True)
-                    e1 = torch.cuda.Event(enable_timing=True)
-                    e0.record(st)
-                    s.iterate(iters)
-                    e1.record(st)
-                    e1.synchronize()
-                    ts.append(e0.elapsed_time(e1) / iters)
-                ms = float(np.median(ts))
-                if log:
-                    log(f"{key(kind, dtype, shape)} {tx}x{ty} kz={kz}: {ms:.4f} ms")
-                if best is None or ms < best["ms"]:
-                    best = {"tx": tx, "ty": ty, "kz": kz, "ms": ms}
+        s.set_schedule(2)
+        if one or not best:
+            best = None
+            for tx, ty in shapes:
+                nxy = -(-shape[0] // tx) * -(-shape[1] // ty)
+                for kz in kz_candidates(shape[2], nxy):
+                    try:
+                        s.set_tiles(tx, ty, kz)
+                    except _lib.QsmError:
+                        continue
+                    ms = _timed(s, st, iters, repeats)
+                    if log:
+                        log(f"{k} one-iteration {tx}x{ty} kz={kz}: {ms:.4f} ms")
+                    if best is None or ms < best["ms"]:
+                        best = {"tx": tx, "ty": ty, "kz": kz, "ms": ms}
+    # the two-iteration pass (even counts: every launch a pair)
+    with Solver(kind, dtype, shape, (1, 1, 1), 1e-3, 2.0, stream=st) as s:
+        s.load(d2f, m)
+        for kz2 in kz2_candidates(shape, dtype):
+            try:
+                s.set_kz2(kz2)
+            except _lib.QsmError:
+                continue
+            ms = _timed(s, st, iters, repeats)
+            if log:
+                log(f"{k} two-iteration kz2={kz2}: {ms:.4f} ms")
+            if "ms2" not in best or ms < best["ms2"]:
+                best["kz2"], best["ms2"] = kz2, ms
+    if "ms2" in best:
+        best["schedule"] = 0 if best["ms2"] < best["ms"] else 2
     if store and best is not None:
         t = _load(USER)
-        t[key(kind, dtype, shape)] = best
+        t[k] = best
         USER.parent.mkdir(parents=True, exist_ok=True)
         USER.write_text(json.dumps(t, indent=1, sort_keys=True))
     return best
+
+
+def best2(kind, dtype, shape):
+    """(kz2, schedule) of this size class's entry, or None when the table
+    has no two-iteration measurement for it (the library's defaults then
+    apply: schedule 0, at most one block per CU)."""
+    e = table().get(key(kind, dtype, shape))
+    if e and "kz2" in e:
+        return int(e["kz2"]), int(e.get("schedule", 0))
+    return None
 
 
 def best(kind, dtype, shape, tiles=None):
@@ -139,13 +197,15 @@ def main(argv=None):
     ap.add_argument("--out", default=None, help="write the table here (default: user table)")
     ap.add_argument("--sizes", default="96x96x96,128x128x128,160x160x96,224x224x128,256x256x176,320x320x192")
     ap.add_argument("--iters", type=int, default=40)
+    ap.add_argument("--two-only", action="store_true",
+                    help="keep the table's one-iteration entries, time only kz2")
     a = ap.parse_args(argv)
     res = {}
     for sz in a.sizes.split(","):
         shape = tuple(int(x) for x in sz.split("x"))
         for kind in (_lib.TGV, _lib.TV):
             for dt in (np.float32, np.float64):
-                b = tune(kind, dt, shape, iters=a.iters, store=False,
+                b = tune(kind, dt, shape, iters=a.iters, store=False, one=not a.two_only,
                          log=lambda s: print(s, file=sys.stderr, flush=True))
                 res[key(kind, dt, shape)] = b
                 print(key(kind, dt, shape), b, flush=True)

@@ -154,11 +154,18 @@ class Solver:
         return self
 
     def autotune(self):
-        """Applies the tuned (tx, ty, kz) of this size class (qsm_amd.autotune
-        table; results are bitwise independent of the choice)."""
+        """Applies the tuned choices of this size class (qsm_amd.autotune
+        table): the one-iteration pass's (tx, ty, kz), and, where measured,
+        the two-iteration pass's kz2 and the faster schedule.  Results are
+        bitwise independent of the choices."""
         from . import autotune
 
-        return self.set_tiles(*autotune.best(self.kind, self.dtype, self.shape))
+        self.set_tiles(*autotune.best(self.kind, self.dtype, self.shape))
+        b2 = autotune.best2(self.kind, self.dtype, self.shape)
+        if b2 is not None:
+            self.set_kz2(b2[0])
+            self.set_schedule(b2[1])
+        return self
 
     def tiles(self):
         a, b, c = C.c_int(), C.c_int(), C.c_int()
