@@ -327,7 +327,7 @@ def _timed_batch(vols, kinds, inputs, iters, warm):
     (seconds for `iters` iterations after `warm`, resident bytes)."""
     from qsm_amd.batch import Batch
 
-    with Batch(np.float32, tiles=(64, 8), hbm_budget=160 << 30, max_volumes=64) as b:
+    with Batch(np.float32, hbm_budget=160 << 30, max_volumes=64) as b:
         for v, k, (d2f, m) in zip(vols, kinds, inputs):
             b.add(k, d2f, m, v["spec"].vsz, v["lam"], v["alpha"], maxiter=warm + iters)
         resident = b.used_bytes
@@ -417,7 +417,7 @@ def _serve_once(specs, inputs, rate, max_live_voxels, chunk):
     reqs = [Request(i, r["arrival"], _lib.TGV if r["kind"] == "tgv" else _lib.TV, d2f, m,
                     r["spec"].vsz, r["lam"], r["alpha"], r["maxiter"])
             for i, (r, (d2f, m)) in enumerate(zip(specs, inputs))]
-    with Batch(np.float32, tiles=(64, 8), hbm_budget=160 << 30, max_volumes=64) as b:
+    with Batch(np.float32, hbm_budget=160 << 30, max_volumes=64) as b:
         stats = Server(b, chunk=chunk, keep_chi=False, max_live_voxels=max_live_voxels).run(reqs)
     assert all(r.iters == s["maxiter"] for r, s in zip(reqs, specs))
     span = float(max(r.arrival for r in reqs))
@@ -432,7 +432,7 @@ def _serve_once(specs, inputs, rate, max_live_voxels, chunk):
 
 
 def serve_leg(n: int, rate: float, sustained_rate: float | None = None,
-              max_live_voxels: int | None = 60_000_000, chunk: int = 25):
+              max_live_voxels: int | None = 60_000_000, chunk: int = 20):
     """C5: Poisson arrivals served by continuous batching, each request with
     its own seeded phantom; latency from arrival to chi on the host.  Two
     rates over the same requests: `rate` (20/s, more than one GPU serves:

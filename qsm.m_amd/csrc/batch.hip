@@ -17,14 +17,24 @@
 // arguments never change: a chunk of iterations is captured once into a
 // hipGraph and replayed; volumes join and leave between chunks by updating
 // the descriptors and the tile list.  Each volume's arithmetic is that of
-// the single-volume solver (same tiles, same reduction order), so a result
-// is bitwise the same alone and in any mix.
+// the single-volume solver (same per-voxel updates in the same order, the
+// convergence sums reduced in a fixed order per volume), so a result is
+// bitwise the same alone and in any mix.
+//
+// A batch created with tile shape 0x0 runs the two-iteration march of
+// fused2_dev.h instead (batch_fused2_kernel): each launch advances a volume
+// by two iterations, or by one where its next iteration is a convergence
+// check (which must see the state after it) or its chunk or budget has one
+// iteration left.  The choice is made on the device per volume
+// (batch_chunk2_kernel at the start of a chunk, batch_advance2_kernel after
+// each launch), so a captured chunk holds a fixed number of launches, the
+// last ones doing nothing for volumes already at the chunk's end.
 #include <algorithm>
 #include <cstring>
 #include <new>
 #include <vector>
 
-#include "fused_dev.h"
+#include "fused2_dev.h"
 
 namespace qsm {
 namespace {
@@ -52,11 +62,18 @@ struct VolDesc {
     // advanced on the device
     uint32_t iter;           // iterations completed
     int live;                // 1 while iterating
+    uint32_t cur;            // bank holding the iterates of `iter`
+    int mode;                // two-iteration march: iterations of the next launch
+    uint32_t chunk_end;      // two-iteration march: iteration ending this chunk
+    int pad;
 };
 
 struct TileList {
     int ntiles[2];   // tiles of TV volumes, of TGV volumes (list order)
     int pad[2];
+    // two-iteration march: tiles claimed so far from each XCD's range of
+    // the list, per kind (TV 0-7, TGV 8-15); zeroed before every launch
+    int claimed[16];
 };
 
 // One persistent launch per kind: kind KIND's tiles are a contiguous part
@@ -82,7 +99,7 @@ __global__ __launch_bounds__(TX *TY) void batch_fused_kernel(
         // the descriptor is read-only here (restrict): its fields stay in
         // registers through the march
         const VolDesc<T> *dp = vols + tv[t];
-        if (!dp->live || dp->ctl->stop)
+        if (!dp->live || __builtin_amdgcn_readfirstlane(dp->ctl->stop))
             continue;
         const int lt = t - dp->tile0;
         const int ntx = dp->ntx, nty = dp->nty;
@@ -154,13 +171,210 @@ __global__ __launch_bounds__(ADV_T) void batch_advance_kernel(VolDesc<T> *vols, 
             if ((n1 < d.tol) && (n2 < 2.0 * d.tol)) {
                 ctl->stop = 1;
                 ctl->stop_iter = n;
+                ctl->stop_bank = n & 1u;  // iteration n wrote bank n & 1
             }
         }
     }
     if (threadIdx.x == 0) {
         d.iter = n;
+        d.cur = n & 1u;
         if (n >= d.target || ctl->stop)
             d.live = 0;
+    }
+}
+
+// ---- the two-iteration march ----------------------------------------------
+
+// One persistent launch per kind over the tile list, as batch_fused_kernel,
+// each tile advancing its volume by the volume's mode (0: nothing).
+// Claims the next tile of the list for the calling block (thread 0) and
+// broadcasts it: first from the block's own XCD's contiguous range (so a
+// volume's neighbouring tiles meet in one L2), then from the other XCDs'
+// ranges; -1 when all are taken.  Tiles of a march differ in cost (sizes,
+// chunk depths, kinds), so claiming balances what a fixed share would not.
+__device__ __forceinline__ int claim_tile(int *claimed, int ntiles, int xcd, int *slot)
+{
+    if (threadIdx.x == 0) {
+        const int span = (ntiles + 7) / 8;
+        int got = -1;
+        for (int k = 0; k < 8 && got < 0; ++k) {
+            const int x = (xcd + k) % 8;
+            const int len = min(ntiles, (x + 1) * span) - x * span;
+            if (len <= 0)
+                continue;
+            if (k > 0 && __hip_atomic_load(&claimed[x], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT) >= len)
+                continue;
+            const int i = atomicAdd(&claimed[x], 1);
+            if (i < len)
+                got = x * span + i;
+        }
+        *slot = got;
+    }
+    __syncthreads();
+    // the same value in every lane: read as a wave-uniform (scalar) value,
+    // so the tile loop and every branch on the tile's descriptor are uniform
+    // control flow around the march's barriers (a value the compiler must
+    // treat as divergent turns the loop into per-lane exec-mask loops, in
+    // which lane 0's next claim and the other lanes' barriers separate)
+    const int t = __builtin_amdgcn_readfirstlane(*slot);
+    __syncthreads();  // every thread has read the slot before it is reused
+    return t;
+}
+
+// The tile t of volume *dp advanced by `mode` iterations (uniform over the
+// block).
+template <typename T, bool TGV, int BX, int BY>
+__device__ __forceinline__ void march2_tile(Lds2<T, TGV, threads2<BX, BY>()> &L,
+                                            const VolDesc<T> *dp, int t, int mode)
+{
+    constexpr int NC = threads2<BX, BY>();
+    const int lt = t - dp->tile0;
+    const int ntx = dp->ntx, nty = dp->nty;
+    const int bx = lt % ntx;
+    const int by = (lt / ntx) % nty;
+    const int bz = lt / (ntx * nty);
+    const bool check = ((dp->iter + uint32_t(mode)) % 10) == 0;  // pd_tgv_mex.c:298,309
+    const T *bi = dp->bank[dp->cur & 1u], *bo = dp->bank[(dp->cur & 1u) ^ 1u];
+    const DualCoef<T> dc = dp->dc;
+    const PrimalCoef<T> pc = dp->pc;
+    const int nx = dp->nx, ny = dp->ny, nz = dp->nz, kz = dp->kz;
+    constexpr uint32_t S = sizeof(T);
+    constexpr uint32_t NARR = TGV ? 15u : 6u;
+    Ctx<T> c;
+    c.nb = dp->N * S;
+    c.in = make_rsrc(bi, NARR * c.nb);
+    c.out = make_rsrc(bo, NARR * c.nb);
+    c.f = make_rsrc(dp->f, c.nb);
+    c.mk = make_rsrc(dp->mk, dp->N);
+    c.sk = nx * ny;
+    c.syb = uint32_t(nx) * S;
+    c.skb = uint32_t(c.sk) * S;
+    c.nz = nz;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    // one specialised march per (mode, check)
+    if (mode == 2) {
+        if (check) {
+            fused2_tile<T, TGV, BX, BY, true, 2>(L, c, dc, pc, nx, ny, nz, bx, by, bz, kz, acc);
+            block_sum4_f<NC>(acc, dp->partials + 4 * lt);
+        } else {
+            fused2_tile<T, TGV, BX, BY, false, 2>(L, c, dc, pc, nx, ny, nz, bx, by, bz, kz, acc);
+        }
+    } else if (check) {
+        fused2_tile<T, TGV, BX, BY, true, 1>(L, c, dc, pc, nx, ny, nz, bx, by, bz, kz, acc);
+        block_sum4_f<NC>(acc, dp->partials + 4 * lt);
+    } else {
+        fused2_tile<T, TGV, BX, BY, false, 1>(L, c, dc, pc, nx, ny, nz, bx, by, bz, kz, acc);
+    }
+}
+
+template <typename T, bool TGV, int BX, int BY>
+__global__ __launch_bounds__((BX * BY + 63) / 64 * 64) void batch_fused2_kernel(
+    const VolDesc<T> *__restrict__ vols, const int *__restrict__ tile_vol,
+    TileList *tl)
+{
+    constexpr int NC = threads2<BX, BY>();
+    __shared__ Lds2<T, TGV, NC> L;
+    __shared__ int slot;
+    const int T_ = tl->ntiles[TGV ? 1 : 0];
+    const int *tv = tile_vol + (TGV ? tl->ntiles[0] : 0);
+    int *claimed = tl->claimed + (TGV ? 8 : 0);
+    const int xcd = int(blockIdx.x) % 8;
+    // every branch below is on wave-uniform values (the claimed index, the
+    // descriptor's scalar fields, the stop flag read through readfirstlane)
+    // and the loop has one exit, at its head: the march's barriers then sit
+    // in uniform control flow
+    for (int t = claim_tile(claimed, T_, xcd, &slot); t >= 0;
+         t = claim_tile(claimed, T_, xcd, &slot)) {
+        const VolDesc<T> *dp = vols + tv[t];
+        const int mode = dp->mode;
+        const bool run = dp->live && mode != 0
+                         && !__builtin_amdgcn_readfirstlane(dp->ctl->stop);
+        if (run)
+            march2_tile<T, TGV, BX, BY>(L, dp, t, mode);
+    }
+}
+
+// Iterations of a volume's next launch: two, unless the first of them is a
+// convergence check or only one is left before min(chunk_end, target).
+__device__ __forceinline__ int next_mode(uint32_t iter, uint32_t chunk_end, uint32_t target)
+{
+    const uint32_t end = min(chunk_end, target);
+    if (iter >= end)
+        return 0;
+    if (end - iter >= 2 && ((iter + 1) % 10) != 0)
+        return 2;
+    return 1;
+}
+
+// Start of a chunk of `chunk` iterations: every live volume may advance
+// that many.
+template <typename T>
+__global__ void batch_chunk2_kernel(VolDesc<T> *vols, int nslots, uint32_t chunk, TileList *tl)
+{
+    const int v = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (v < 16)
+        tl->claimed[v] = 0;
+    if (v >= nslots)
+        return;
+    VolDesc<T> &d = vols[v];
+    if (!d.live)
+        return;
+    d.chunk_end = d.iter + chunk;
+    d.mode = next_mode(d.iter, d.chunk_end, d.target);
+}
+
+// After a launch: finalize the check of volumes that checked (the last
+// iteration the launch ran), advance counters and banks, plan the next.
+template <typename T>
+__global__ __launch_bounds__(ADV_T) void batch_advance2_kernel(VolDesc<T> *vols, int nslots,
+                                                               TileList *tl)
+{
+    if (blockIdx.x == 0 && threadIdx.x < 16)
+        tl->claimed[threadIdx.x] = 0;  // for the next launch
+    VolDesc<T> &d = vols[blockIdx.x];
+    if (!d.live)
+        return;
+    Ctl *ctl = d.ctl;
+    const int mode = d.mode;
+    if (ctl->stop || mode == 0)
+        return;
+    const uint32_t n = d.iter + uint32_t(mode);
+    const uint32_t bank = (d.cur & 1u) ^ 1u;
+    if (n % 10 == 0) {
+        const int nblk = d.ntx * d.nty * d.nch;
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int b = threadIdx.x; b < nblk; b += ADV_T)
+            for (int q = 0; q < 4; ++q)
+                s[q] += d.partials[4 * b + q];
+        __shared__ double tot[4];
+        block_sum4_f<ADV_T>(s, tot);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const double n1 = sqrt(tot[2] / tot[0]);  // pd_tgv_mex.c:1004-1010
+            const double n2 = sqrt(tot[3] / tot[1]);
+            const uint32_t c = ctl->checks;
+            if (d.hist && int(c) < d.hist_cap) {
+                d.hist[2 * c] = n1;
+                d.hist[2 * c + 1] = n2;
+            }
+            ctl->checks = c + 1;
+            ctl->nr[0] = n1;
+            ctl->nr[1] = n2;
+            if ((n1 < d.tol) && (n2 < 2.0 * d.tol)) {
+                ctl->stop = 1;
+                ctl->stop_iter = n;
+                ctl->stop_bank = bank;
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+        d.iter = n;
+        d.cur = bank;
+        const bool done = n >= d.target || ctl->stop;
+        if (done)
+            d.live = 0;
+        d.mode = done ? 0 : next_mode(n, d.chunk_end, d.target);
     }
 }
 
@@ -185,6 +399,7 @@ struct BatchVol {
 
 struct qsm_batch {
     int dtype, tx, ty;
+    bool two = false;            // tiles 0x0: the two-iteration march
     size_t budget, used;
     int max_vols;
     hipStream_t stream;
@@ -234,6 +449,64 @@ int persistent_grids(int out[2])
     return rc ? rc : persistent_grid<T, true, TX, TY>(&out[1]);
 }
 
+template <typename T, bool TGV, int BX, int BY>
+int persistent_grid2(int *out)
+{
+    int dev = 0, ncu = 0, occ = 0;
+    QSM_HIP(hipGetDevice(&dev));
+    QSM_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    QSM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &occ, reinterpret_cast<const void *>(&batch_fused2_kernel<T, TGV, BX, BY>),
+        threads2<BX, BY>(), 0));
+    if (occ < 1) occ = 1;
+    *out = std::max(8, ncu * occ / 8 * 8);
+    return QSM_OK;
+}
+
+template <typename T, int BX, int BY>
+int persistent_grids2(int out[2])
+{
+    int rc = persistent_grid2<T, false, BX, BY>(&out[0]);
+    return rc ? rc : persistent_grid2<T, true, BX, BY>(&out[1]);
+}
+
+// Launches of a captured chunk of `chunk` iterations in the two-iteration
+// march: a volume runs (chunk + singles) / 2 of them, and its singles (a
+// check falling on the first of two iterations, at most one per ten, and
+// the chunk's last odd one) number at most ceil(chunk / 10) + 1.
+inline uint32_t launches2(uint32_t chunk)
+{
+    return (chunk + (chunk + 9) / 10 + 1 + 1) / 2;
+}
+
+template <typename T>
+hipError_t launch_iteration2(qsm_batch *b)
+{
+    auto *vols = static_cast<VolDesc<T> *>(b->d_vols);
+    auto *tl = static_cast<TileList *>(b->d_tl);
+    if constexpr (sizeof(T) == 4) {
+        constexpr int BX = QSM_F2_F32_BX, BY = QSM_F2_F32_BY;
+        batch_fused2_kernel<T, false, BX, BY><<<b->grid[0], threads2<BX, BY>(), 0, b->stream>>>(
+            vols, b->d_tilevol, tl);
+        batch_fused2_kernel<T, true, BX, BY><<<b->grid[1], threads2<BX, BY>(), 0, b->stream>>>(
+            vols, b->d_tilevol, tl);
+    } else {
+        constexpr int BX = QSM_F2_F64_BX, BY = QSM_F2_F64_BY;
+        batch_fused2_kernel<T, false, BX, BY><<<b->grid[0], threads2<BX, BY>(), 0, b->stream>>>(
+            vols, b->d_tilevol, tl);
+        batch_fused2_kernel<T, true, BX, BY><<<b->grid[1], threads2<BX, BY>(), 0, b->stream>>>(
+            vols, b->d_tilevol, tl);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    batch_advance2_kernel<T><<<b->max_vols, ADV_T, 0, b->stream>>>(vols, b->max_vols, tl);
+    return hipGetLastError();
+}
+
+// One chunk of `chunk` iterations (both marches).
+template <typename T>
+hipError_t launch_chunk(qsm_batch *b, uint32_t chunk);
+
 template <typename T>
 hipError_t launch_iteration(qsm_batch *b)
 {
@@ -256,8 +529,31 @@ hipError_t launch_iteration(qsm_batch *b)
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_chunk(qsm_batch *b, uint32_t chunk)
+{
+    if (!b->two) {
+        for (uint32_t i = 0; i < chunk; ++i) {
+            const hipError_t e = launch_iteration<T>(b);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+    auto *vols = static_cast<VolDesc<T> *>(b->d_vols);
+    batch_chunk2_kernel<T><<<(b->max_vols + 255) / 256, 256, 0, b->stream>>>(
+        vols, b->max_vols, chunk, static_cast<TileList *>(b->d_tl));
+    hipError_t e = hipGetLastError();
+    for (uint32_t i = 0; i < launches2(chunk) && e == hipSuccess; ++i)
+        e = launch_iteration2<T>(b);
+    return e;
+}
+
 int grid_for(qsm_batch *b)
 {
+    if (b->two)
+        return b->dtype == QSM_F64
+            ? persistent_grids2<double, QSM_F2_F64_BX, QSM_F2_F64_BY>(b->grid)
+            : persistent_grids2<float, QSM_F2_F32_BX, QSM_F2_F32_BY>(b->grid);
 #define QSM_G(X, Y)                                                            \
     if (b->tx == X && b->ty == Y)                                              \
         return b->dtype == QSM_F64 ? persistent_grids<double, X, Y>(b->grid)   \
@@ -285,9 +581,20 @@ int upload(qsm_batch *b)
     // tile0 is relative to the kind's part of the list
     std::vector<int> map;
     int nk[2] = {0, 0};
+    // the two-iteration march claims tiles in list order: costly tiles
+    // (long marches of large planes) first, so cheap ones fill the end
+    std::vector<int> order;
+    for (int v = 0; v < b->max_vols; ++v)
+        order.push_back(v);
+    if (b->two)
+        std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+            const VolDesc<T> &da = desc<T>(b, a), &dc = desc<T>(b, c);
+            return double(std::min(da.kz, std::max(da.nz - 2, 1)) + 6)
+                       > double(std::min(dc.kz, std::max(dc.nz - 2, 1)) + 6);
+        });
     for (int kind = 0; kind < 2; ++kind) {
         const size_t start = map.size();
-        for (int v = 0; v < b->max_vols; ++v) {
+        for (int v : order) {
             if (!b->vols[v].used || (b->vols[v].kind == QSM_TGV) != (kind == 1)) continue;
             b->vols[v].uploaded = true;
             VolDesc<T> &d = desc<T>(b, v);
@@ -306,7 +613,9 @@ int upload(qsm_batch *b)
     }
     if (!map.empty())
         QSM_HIP(hipMemcpy(b->d_tilevol, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice));
-    TileList tl{{nk[0], nk[1]}, {0, 0}};
+    TileList tl{};
+    tl.ntiles[0] = nk[0];
+    tl.ntiles[1] = nk[1];
     QSM_HIP(hipMemcpy(b->d_tl, &tl, sizeof tl, hipMemcpyHostToDevice));
     QSM_HIP(hipMemcpy(b->d_vols, b->h_desc.data(), b->h_desc.size(), hipMemcpyHostToDevice));
     b->dirty = false;
@@ -328,6 +637,9 @@ int pull(qsm_batch *b)
         VolDesc<T> &d = desc<T>(b, v);
         d.iter = s.iter;
         d.live = s.live;
+        d.cur = s.cur;
+        d.mode = s.mode;
+        d.chunk_end = s.chunk_end;
     }
     return QSM_OK;
 }
@@ -348,7 +660,10 @@ int add_t(qsm_batch *b, int kind, const size_t sz[3], const double h[3], double 
     QSM_ARG(fused_size_supported(kind, b->dtype, N), "volume too large for the batched pass");
     const Geom g{int(sz[0]), int(sz[1]), int(sz[2])};
     const FusedShape sh{b->tx, b->ty, kz};
-    const int ntiles = fused_blocks(g, sh);
+    int ttx = b->tx, tty = b->ty;
+    if (b->two)
+        fused2_tile(b->dtype, ttx, tty);
+    const int ntiles = b->two ? fused2_blocks(g, b->dtype, kz) : fused_blocks(g, sh);
     const int narr = kind == QSM_TGV ? 15 : 6;
     const size_t bank = align_up(size_t(narr) * N * sizeof(T), 256);
     const int hist_cap = int(maxiter / 10) + 1;
@@ -426,8 +741,8 @@ int add_t(qsm_batch *b, int kind, const size_t sz[3], const double h[3], double 
     d.nx = g.nx; d.ny = g.ny; d.nz = g.nz; d.kind = kind;
     d.N = uint32_t(N);
     int ntx, nty, nch;
-    ntx = (g.nx + b->tx - 1) / b->tx;
-    nty = (g.ny + b->ty - 1) / b->ty;
+    ntx = (g.nx + ttx - 1) / ttx;
+    nty = (g.ny + tty - 1) / tty;
     nch = (g.nz - 2 + kz - 1) / kz;
     d.ntx = ntx; d.nty = nty; d.nch = ntiles ? nch : 0; d.kz = kz;
     d.bank[0] = b0; d.bank[1] = b1;
@@ -440,6 +755,9 @@ int add_t(qsm_batch *b, int kind, const size_t sz[3], const double h[3], double 
     d.dc = dc;
     d.pc = pc;
     d.iter = 0;
+    d.cur = 0;
+    d.mode = 0;
+    d.chunk_end = 0;
     d.live = (maxiter > 0 && ntiles > 0) ? 1 : 0;
     if (ntiles == 0)
         d.ntx = d.nty = d.nch = 0;
@@ -458,11 +776,13 @@ int result_t(qsm_batch *b, int v, void *chi, uint32_t *iters, int *stopped, doub
     Ctl c;
     QSM_HIP(hipMemcpy(&c, d.ctl, sizeof c, hipMemcpyDeviceToHost));
     const uint32_t it = c.stop ? c.stop_iter : d.iter;
+    const uint32_t bank = c.stop ? c.stop_bank : d.cur;
     if (iters) *iters = it;
     if (stopped) *stopped = c.stop;
     if (nr) { nr[0] = c.nr[0]; nr[1] = c.nr[1]; }
     if (chi)
-        QSM_HIP(hipMemcpy(chi, d.bank[it & 1u], b->vols[v].N * sizeof(T), hipMemcpyDeviceToHost));
+        QSM_HIP(hipMemcpy(chi, d.bank[bank & 1u], b->vols[v].N * sizeof(T),
+                          hipMemcpyDeviceToHost));
     return QSM_OK;
 }
 
@@ -475,16 +795,13 @@ int step_t(qsm_batch *b, uint32_t chunk)
         if (rc) return rc;
     }
     if (!b->use_graph) {
-        for (uint32_t i = 0; i < chunk; ++i)
-            QSM_HIP(launch_iteration<T>(b));
+        QSM_HIP(launch_chunk<T>(b, chunk));
         return QSM_OK;
     }
     if (!b->gexec || b->graph_chunk != chunk) {
         drop_graph(b);
         QSM_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        hipError_t le = hipSuccess;
-        for (uint32_t i = 0; i < chunk && le == hipSuccess; ++i)
-            le = launch_iteration<T>(b);
+        const hipError_t le = launch_chunk<T>(b, chunk);
         hipGraph_t g = nullptr;
         const hipError_t ee = hipStreamEndCapture(b->stream, &g);
         QSM_HIP(le);
@@ -507,7 +824,8 @@ int qsm_batch_create(qsm_batch **out, int dtype, int tx, int ty, size_t hbm_budg
     QSM_ARG(out, "null handle pointer");
     *out = nullptr;
     QSM_ARG(dtype == QSM_F64 || dtype == QSM_F32, "bad dtype");
-    QSM_ARG(fused_shape_supported(tx, ty), "unsupported tile shape %dx%d", tx, ty);
+    QSM_ARG((tx == 0 && ty == 0) || fused_shape_supported(tx, ty),
+            "unsupported tile shape %dx%d", tx, ty);
     QSM_ARG(max_volumes >= 1 && max_volumes <= 4096, "max_volumes out of range");
     int rc = ensure_device();
     if (rc) return rc;
@@ -516,6 +834,7 @@ int qsm_batch_create(qsm_batch **out, int dtype, int tx, int ty, size_t hbm_budg
     b->dtype = dtype;
     b->tx = tx;
     b->ty = ty;
+    b->two = tx == 0 && ty == 0;
     b->budget = hbm_budget;
     b->used = 0;
     b->max_vols = max_volumes;
@@ -551,9 +870,12 @@ int qsm_batch_create(qsm_batch **out, int dtype, int tx, int ty, size_t hbm_budg
 size_t qsm_batch_volume_bytes(int kind, int dtype, const size_t sz[3], int tx, int ty,
                               int kz, uint32_t maxiter)
 {
+    if (!sz || kz < 1) return 0;
     const size_t N = sz[0] * sz[1] * sz[2];
     const size_t s = dtype_size(dtype);
-    const int ntiles = fused_blocks(Geom{int(sz[0]), int(sz[1]), int(sz[2])}, FusedShape{tx, ty, kz});
+    const Geom g{int(sz[0]), int(sz[1]), int(sz[2])};
+    const int ntiles = (tx == 0 && ty == 0) ? fused2_blocks(g, dtype, kz)
+                                            : fused_blocks(g, FusedShape{tx, ty, kz});
     const size_t bank = align_up(size_t(kind == QSM_TGV ? 15 : 6) * N * s, 256);
     return 2 * bank + align_up(N * s, 256) + align_up(N, 256) + 256
            + align_up(size_t(std::max(ntiles, 1)) * 32, 256)

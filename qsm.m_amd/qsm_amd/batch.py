@@ -39,9 +39,14 @@ class SlotResult:
 
 
 class Batch:
-    """A set of resident inversions advanced together."""
+    """A set of resident inversions advanced together.
 
-    def __init__(self, dtype=np.float32, tiles=(64, 8), hbm_budget=64 << 30,
+    tiles=(0, 0) (the default) runs the two-iteration march (tiles of 32x16
+    voxels in FP32, 16x16 in FP64); a supported one-iteration shape such as
+    (64, 8) runs the one-iteration march instead.  Results are bitwise the
+    same either way."""
+
+    def __init__(self, dtype=np.float32, tiles=(0, 0), hbm_budget=64 << 30,
                  max_volumes=64, stream=None):
         self.dtype = np.dtype(dtype)
         self.tiles = tuple(tiles)
@@ -58,9 +63,11 @@ class Batch:
                                                 self.tiles[0], self.tiles[1], int(kz), int(maxiter)))
 
     def kz_for(self, kind, shape):
-        """The tuned z-chunk depth of this size class for the batch's tiles."""
+        """The tuned z-chunk depth of this size class for the batch's march."""
         from . import autotune
 
+        if tuple(self.tiles) == (0, 0):
+            return autotune.batch_kz2(shape)
         return autotune.best(kind, self.dtype, shape, tiles=self.tiles)[2]
 
     def add(self, kind, d2f, mask, vsz, lam, alpha=0.0, maxiter=1000, tol=0.0, kz=None):

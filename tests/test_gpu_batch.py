@@ -42,9 +42,11 @@ def _oracle(case, d2f, m):
 
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
 @pytest.mark.parametrize("graph", [True, False])
-def test_batch_mix_matches_oracle(cuda, dt, graph):
+@pytest.mark.parametrize("tiles", [(64, 8), (0, 0)])
+def test_batch_mix_matches_oracle(cuda, dt, graph, tiles):
+    """(64, 8): the one-iteration march; (0, 0): the two-iteration march."""
     inputs = _inputs(dt)
-    with Batch(dt, tiles=(64, 8), max_volumes=8) as b:
+    with Batch(dt, tiles=tiles, max_volumes=8) as b:
         b.set_graph(graph)
         slots = []
         # first three join now, the last two after two chunks (continuous)
@@ -112,3 +114,32 @@ def test_batch_admission_budget(cuda):
         b.remove(s0)
         s2 = b.add(_lib.TGV, d2f, m, (1, 1, 1), 1e-3, 2.0, 10, kz=16)
         assert s2 == s0 and b.used_bytes == 2 * one
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_two_iteration_batch_uneven_chunks(cuda, dt):
+    """The two-iteration march with chunks of 7, 13, 1 and 25 iterations (a
+    check then falls on the first iteration of a pair and runs alone, odd
+    budgets end with a single iteration), volumes joining between chunks
+    and a tolerance stop: every volume bitwise the oracle's, same stopping
+    iteration."""
+    inputs = _inputs(dt)
+    with Batch(dt, max_volumes=8) as b:
+        slots = []
+        for case, (d2f, m) in list(zip(CASES, inputs))[:2]:
+            shape, vsz, kind, lam, alpha, iters, tol = case
+            slots.append(b.add(kind, d2f, m, vsz, lam, alpha, iters, tol, kz=6))
+        b.step(7)
+        for case, (d2f, m) in list(zip(CASES, inputs))[2:]:
+            shape, vsz, kind, lam, alpha, iters, tol = case
+            slots.append(b.add(kind, d2f, m, vsz, lam, alpha, iters, tol, kz=11))
+        for chunk in (13, 1, 25, 7) * 100:
+            b.step(chunk)
+            b.sync()
+            if not any(b.state(s)[1] for s in slots):
+                break
+        for s, case, (d2f, m) in zip(slots, CASES, inputs):
+            ref, it, nr = _oracle(case, d2f, m)
+            r = b.result(s)
+            assert r.iters == it, (case, r.iters, it)
+            np.testing.assert_array_equal(r.chi, ref, err_msg=str(case))

@@ -14,8 +14,8 @@
   continuous batching at their full iteration budgets; every result bitwise
   equal to the request solved alone, and the shortest one the oracle's.
 * A bank above 2 GiB: FP64 TGV at 320x320x192 (2.36 GB per bank, byte
-  offsets past 2^31) through the fused pass, the two-sweep schedule, the
-  batch and the oracle, bitwise.
+  offsets past 2^31) through the two- and one-iteration passes, the
+  two-sweep schedule, both batch marches and the oracle, bitwise.
 """
 import numpy as np
 import pytest
@@ -113,7 +113,7 @@ def test_c4_all_48_volumes_batched(cuda):
     inputs = phantom.solver_inputs([v["spec"] for v in vols])
     kinds = [_kind(v["kind"]) for v in vols]
     _say("C4 batch of 48, 20 iterations")
-    with Batch(np.float32, tiles=(64, 8), hbm_budget=160 << 30, max_volumes=64) as b:
+    with Batch(np.float32, hbm_budget=160 << 30, max_volumes=64) as b:
         slots = [b.add(k, d2f, m, v["spec"].vsz, v["lam"], v["alpha"], maxiter=iters)
                  for v, k, (d2f, m) in zip(vols, kinds, inputs)]
         b.step(10)
@@ -152,7 +152,7 @@ def test_c5_requests_equal_solo_solves(cuda):
         reqs.append(Request(j, 0.05 * j, _kind(r["kind"]), d2f.astype(np.float32, order="F"), m,
                             r["spec"].vsz, r["lam"], r["alpha"], r["maxiter"]))
     _say("C5 serving, budgets", [r.maxiter for r in reqs])
-    with Batch(np.float32, tiles=(64, 8), hbm_budget=160 << 30, max_volumes=16) as b:
+    with Batch(np.float32, hbm_budget=160 << 30, max_volumes=16) as b:
         stats = Server(b, chunk=25, max_live_voxels=60_000_000).run(reqs)
     assert stats.n == 8 and stats.peak_live >= 2
     for r in reqs:
@@ -189,15 +189,22 @@ def test_fp64_tgv_bank_above_2gib(cuda):
         s.iterate(iters)
         two = s.chi()
     np.testing.assert_array_equal(fused, two)
-    _say("bank > 2 GiB: batch")
-    with Batch(np.float64, tiles=(64, 8), hbm_budget=160 << 30, max_volumes=2) as b:
-        slot = b.add(_lib.TGV, d2f, m, spec.vsz, lam, alpha, maxiter=iters)
-        b.step(10)
-        b.step(10)
-        b.sync()
-        got = b.result(slot)
-    assert got.iters == iters
-    np.testing.assert_array_equal(got.chi, fused)
+    for tiles in ((64, 8), (0, 0)):
+        _say(f"bank > 2 GiB: batch, tiles {tiles}")
+        with Batch(np.float64, tiles=tiles, hbm_budget=160 << 30, max_volumes=2) as b:
+            slot = b.add(_lib.TGV, d2f, m, spec.vsz, lam, alpha, maxiter=iters)
+            b.step(10)
+            b.step(10)
+            b.sync()
+            got = b.result(slot)
+        assert got.iters == iters
+        np.testing.assert_array_equal(got.chi, fused)
+    _say("bank > 2 GiB: one-iteration passes")
+    with pd.Solver(_lib.TGV, np.float64, shape, spec.vsz, lam, alpha) as s:
+        s.set_schedule(2)
+        s.load(d2f, m)
+        s.iterate(iters)
+        np.testing.assert_array_equal(s.chi(), fused)
     _say("bank > 2 GiB: oracle")
     ref = orc.pd_tgv(d2f, m, spec.vsz, lam, alpha, 0.0, iters)[0]
     np.testing.assert_array_equal(fused, ref)
