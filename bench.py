@@ -6,8 +6,9 @@
 Workload (C3, the size the metric is quoted at): seeded 60-ellipsoid phantom,
 256x256x176 at 1 mm, B0 = z, full-box mask, TGV alpha1 = 1e-3,
 alpha0 = 2*alpha1, tol = 0.  A step is one primal-dual iteration over the whole
-volume (one fused kernel pass; the every-10th-iteration convergence check is
-included).  The input is resident in HBM before timing starts.
+volume (half a two-iteration fused pass; the every-10th-iteration
+convergence check is included).  The input is resident in HBM before timing
+starts.
 
 Reported: FP32 voxel-iters/s (value) and the FP64 drop-in beside it, chi
 error of FP32 against FP64 (the FP64 path is bitwise the oracle's), the
@@ -34,7 +35,7 @@ for _p in (ROOT / "qsm.m_amd", ROOT):
 METRIC = ("TGV-QSM voxel-iters/s at 256²×176 (FP32 vs FP64, same GPU); "
           "p99 solve latency")
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = ROOT / "profiles" / "traffic_fused_tgv_c3.json"
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_fused2_tgv_c3.json"
 
 
 def log(*a):
@@ -63,10 +64,13 @@ def time_solver(kind, dtype, cfg, d2f, mask, steps, warmup):
     s = pd.Solver(kind, dtype, shape, cfg["spec"].vsz, cfg["lam"], cfg["alpha"], stream=stream)
     s.autotune()
     tiles = s.tiles()
+    from qsm_amd import autotune
+    b2 = autotune.best2(kind, dtype, shape)
     s.load(d2f.astype(dtype, order="F"), mask)
     s.iterate(warmup, 0.0)
     stream.synchronize()
     torch.cuda.synchronize()
+    c0 = s.launch_counts()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     w0 = time.perf_counter()
@@ -77,11 +81,14 @@ def time_solver(kind, dtype, cfg, d2f, mask, steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - w0
     gpu = e0.elapsed_time(e1) / 1e3
+    c1 = s.launch_counts()
     it, stopped, _ = s.status()
     chi = s.chi()
     s.close()
     assert it == warmup + steps and not stopped
-    return {"gpu_s": gpu, "wall_s": wall, "chi": chi, "tiles": tiles}
+    launches = {k: c1[k] - c0[k] for k in c1}
+    return {"gpu_s": gpu, "wall_s": wall, "chi": chi, "tiles": tiles, "launches": launches,
+            "kz2": b2[0] if b2 else None, "schedule": b2[1] if b2 else 0}
 
 
 def stream_ceiling_gbs(n=256 * 256 * 176, narr=15):
@@ -801,6 +808,62 @@ def nifti_leg(field, mask, vsz, reps=3):
     return out
 
 
+def _schedule_text(r):
+    tx, ty, kz = r["tiles"]
+    l = r["launches"]
+    if l["pass2"]:
+        return (f"two-iteration fused pass (primal n, dual n+1, primal n+1, dual n+2 per "
+                f"launch), z-chunks of {r['kz2']}; {l['pass2']} two-iteration launches, "
+                f"{l['pass1']} one-iteration, {l['checks']} convergence checks in the timed "
+                f"region (autotuned)")
+    return (f"one-iteration fused pass, {tx}x{ty} tiles, z-chunks of {kz}; {l['pass1']} "
+            f"launches, {l['checks']} convergence checks in the timed region (autotuned)")
+
+
+def roofline(r, N, n_active, s):
+    """Roofline of the dominant kernel of the timed region.
+
+    Algorithmic bytes per voxel-iteration: 41 words of state + 1 mask byte
+    for TGV, the reference's minimal single-pass traffic with the
+    extrapolated chi stored (pd_tgv_mex.c:331-977).  A two-iteration launch
+    processes two voxel-iterations per voxel, so its algorithmic bytes are
+    twice one iteration's.  The launch duration is the HIP-event time of the
+    timed region on the solver's stream over the dominant kernel's launch
+    count (the convergence-check kernels, about 3 us each every tenth
+    iteration, are inside it).  `traffic` is the HBM bytes one launch of that
+    kernel moves, from the committed FETCH_SIZE/WRITE_SIZE summary."""
+    l = r["launches"]
+    two = l["pass2"] > 0 and l["pass1"] == 0
+    per_it = n_active * 41 * s + N
+    if two:
+        its, nl, kern = 2, l["pass2"], "fused2_kernel"
+    else:
+        its, nl, kern = 1, max(1, l["pass1"] + l["sweeps"]), "fused_kernel"
+    t_launch = r["gpu_s"] / nl
+    alg = its * per_it
+    ach = alg / t_launch / 1e9
+    # the bytes the kernel itself cannot avoid: its 31-word read+write of the
+    # state (16 words in, 15 out) and the mask, once per launch
+    own = n_active * 31 * s + N
+    out = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": kern,
+           "iterations_per_launch": its, "launch_ms": t_launch * 1e3,
+           "algorithmic_bytes_per_launch": alg,
+           "own_minimum_bytes_per_launch": own,
+           "own_minimum_gbs": own / t_launch / 1e9}
+    key = "f32" if s == 4 else "f64"
+    if TRAFFIC_FILE.exists():
+        try:
+            t = json.loads(TRAFFIC_FILE.read_text()).get(key, {})
+            if t.get("iterations_per_launch") == its:
+                out["traffic"] = t["bytes_per_launch"]
+                out["traffic_gbs"] = t["bytes_per_launch"] / t_launch / 1e9
+                out["traffic_frac"] = out["traffic_gbs"] / HBM_PEAK_GBS
+        except Exception:
+            pass
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[1])
     ap.add_argument("--gpus", type=int, default=1)
@@ -882,24 +945,16 @@ def main(argv=None) -> int:
                          f"alpha1={cfg['lam']}, alpha0={cfg['alpha']}*alpha1, tol=0"),
             "voxels": N,
             "n_active": n_active,
-            "schedule": ("fused primal(n)+dual(n+1) single pass, {}x{} tiles, z-chunks of {} "
-                         "(autotuned)").format(*r32["tiles"]),
+            "schedule": _schedule_text(r32),
         },
     }
-    # roofline of the fused kernel (algorithmic bytes: 41 words + 1 mask byte
-    # per active voxel per iteration, the reference's minimal single-pass
-    # state traffic)
-    kern_s = r32["gpu_s"] / a.steps
-    alg_bytes = n_active * 41 * 4 + N * 1
-    ach = alg_bytes / kern_s / 1e9
-    traffic = None
-    if TRAFFIC_FILE.exists():
-        try:
-            traffic = json.loads(TRAFFIC_FILE.read_text()).get("f32", {}).get("bytes_per_launch")
-        except Exception:
-            traffic = None
-    out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                       "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic}
+    out["roofline"] = roofline(r32, N, n_active, 4)
+    ach = out["roofline"]["achieved"]
+    rf = out["roofline"]
+    log(f"[bench] roofline FP32 {rf['kernel']}: {rf['launch_ms']:.4f} ms/launch, "
+        f"{rf['iterations_per_launch']} iterations/launch, algorithmic {ach:.0f} GB/s "
+        f"(frac {rf['frac']:.3f}), own minimum {rf['own_minimum_gbs']:.0f} GB/s"
+        + (f", HBM traffic {rf['traffic_gbs']:.0f} GB/s" if rf.get("traffic_gbs") else ""))
     if r64 is not None:
         v64 = N * a.steps / r64["gpu_s"]
         c32 = r32["chi"].astype(np.float64)
@@ -908,7 +963,8 @@ def main(argv=None) -> int:
         out["fp64"] = {
             "value": v64,
             "ms_per_step": r64["wall_s"] / a.steps * 1e3,
-            "roofline_frac": (n_active * 41 * 8 + N) / (r64["gpu_s"] / a.steps) / 1e9 / HBM_PEAK_GBS,
+            "roofline": roofline(r64, N, n_active, 8),
+            "schedule": _schedule_text(r64),
         }
         out["fp32_over_fp64"] = v32 / v64
         out["chi_err_fp32_vs_fp64"] = {
@@ -923,10 +979,9 @@ def main(argv=None) -> int:
         log(f"[bench] copy peak failed: {e}")
     try:
         out["stream15_ceiling_gbs"] = stream_ceiling_gbs()
-        if out["roofline"].get("traffic"):
-            real = out["roofline"]["traffic"] / (out["ms_per_step"] * 1e-3) / 1e9
-            out["roofline"]["traffic_gbs"] = real
-            out["roofline"]["traffic_frac_of_stream15"] = real / out["stream15_ceiling_gbs"]
+        if out["roofline"].get("traffic_gbs"):
+            out["roofline"]["traffic_frac_of_stream15"] = (out["roofline"]["traffic_gbs"]
+                                                           / out["stream15_ceiling_gbs"])
     except Exception as e:  # measurement aid only
         log(f"[bench] stream ceiling failed: {e}")
     if a.dipole:

@@ -141,6 +141,13 @@ int qsm_pd_get_tiles(qsm_pd *s, int *tx, int *ty, int *kz);
  * at most one block per CU.  Results do not depend on it. */
 int qsm_pd_set_kz2(qsm_pd *s, int kz);
 
+/* Kernel launches the handle has enqueued since it was created:
+ * counts[0] two-iteration passes, [1] one-iteration fused passes, [2]
+ * iterations of the two-sweep schedule, [3] convergence checks.  Launches
+ * enqueued after a stop are counted (they exit at once on the device).  A
+ * measurement aid with no counterpart in the reference. */
+int qsm_pd_launch_counts(qsm_pd *s, uint64_t counts[4]);
+
 /* ------------------------------------------------------------------------
  * Allocation-free solve
  * ------------------------------------------------------------------------ */

@@ -116,6 +116,9 @@ struct qsm_pd {
     FusedShape shape;
     int kz2;          // z-chunk depth of the two-iteration pass
     uint32_t cur;     // fused schedules: bank holding the latest iterates
+    // kernel launches enqueued since creation: two-iteration passes,
+    // one-iteration passes, two-sweep iterations, convergence checks
+    uint64_t launches[4];
 
     void *bank(int b) const { return mem + size_t(b) * L.bank_bytes; }
     void *f() const { return mem + L.f_off; }
@@ -202,10 +205,13 @@ int iterate_impl(qsm_pd *s, uint32_t niter, double tol)
                 : launch_fused2<T, false>(in, out, s->N, f, s->mask(), g, s->kz2, dc, pc, stop, part2, s->stream);
             QSM_HIP(e);
             s->cur ^= 1u;
-            if (check2)
+            ++s->launches[0];
+            if (check2) {
                 QSM_HIP(launch_conv_finalize(s->partials(), fused2_blocks(g, s->dtype, s->kz2),
                                              s->ctl(), s->hist(), s->hist_cap, tol, n2, s->cur,
                                              s->stream));
+                ++s->launches[3];
+            }
             s->n = n2;
             ++t;
             continue;
@@ -218,6 +224,7 @@ int iterate_impl(qsm_pd *s, uint32_t niter, double tol)
                 : launch_fused<T, false>(in, out, s->N, f, s->mask(), g, s->shape, dc, pc, stop, part, s->stream);
             QSM_HIP(e);
             s->cur ^= 1u;
+            ++s->launches[1];
             nblk = fused_blocks(g, s->shape);
         } else {
             hipError_t e = tgv
@@ -228,6 +235,7 @@ int iterate_impl(qsm_pd *s, uint32_t niter, double tol)
                 ? launch_primal_sweep<T, true>(b0, bars_of(b1), s->mask(), 1, g, pc, stop, part, s->stream)
                 : launch_primal_sweep<T, false>(b0, bars_of(b1), s->mask(), 1, g, pc, stop, part, s->stream);
             QSM_HIP(e);
+            ++s->launches[2];
             nblk = primal_sweep_blocks(g);
         }
         if (check) {
@@ -235,6 +243,7 @@ int iterate_impl(qsm_pd *s, uint32_t niter, double tol)
             // as the reference computes them.
             QSM_HIP(launch_conv_finalize(s->partials(), nblk, s->ctl(), s->hist(),
                                          s->hist_cap, tol, n1, s->cur, s->stream));
+            ++s->launches[3];
         }
         s->n = n1;
     }
@@ -624,6 +633,14 @@ int qsm_pd_set_kz2(qsm_pd *s, int kz)
             "chunk depth needs %d partial-sum slots, %d allocated",
             fused2_blocks(s->geom(), s->dtype, kz), s->npart);
     s->kz2 = kz;
+    return QSM_OK;
+}
+
+int qsm_pd_launch_counts(qsm_pd *s, uint64_t counts[4])
+{
+    QSM_ARG(s && counts, "null handle or output");
+    for (int i = 0; i < 4; ++i)
+        counts[i] = s->launches[i];
     return QSM_OK;
 }
 

@@ -153,6 +153,12 @@ class Solver:
         check(lib().qsm_pd_set_kz2(self._h, int(kz)))
         return self
 
+    def launch_counts(self) -> dict:
+        """Kernel launches enqueued since creation, by kind."""
+        c = (C.c_uint64 * 4)()
+        check(lib().qsm_pd_launch_counts(self._h, c))
+        return {"pass2": int(c[0]), "pass1": int(c[1]), "sweeps": int(c[2]), "checks": int(c[3])}
+
     def autotune(self):
         """Applies the tuned choices of this size class (qsm_amd.autotune
         table): the one-iteration pass's (tx, ty, kz), and, where measured,

@@ -307,3 +307,27 @@ def test_two_iteration_pass_uneven_calls_and_stop(cuda, dt):
                           dtype=torch.float64 if dt == np.float64 else torch.float32)
         s.chi(out=dev)  # device copy of the bank the stop recorded
         np.testing.assert_array_equal(from_dev(dev, shape), ref)
+
+
+def test_launch_counts_follow_the_schedule(cuda):
+    """The handle's launch counters: pairs of iterations share a launch
+    unless a check falls on the first of them or the call's count is odd;
+    schedule 2 launches once per iteration, schedule 1 sweeps once per
+    iteration; one check every tenth iteration."""
+    shape, h = (24, 22, 20), (1.0, 1.0, 1.0)
+    d2f = _phantom_input(shape, h, 4, np.float32)
+    m = np.ones(shape, np.uint8)
+    with pd.Solver(_lib.TGV, np.float32, shape, h, 1e-3, 2.0) as s:
+        s.load(d2f, m)
+        s.iterate(3)       # (1, 2) paired, 3 alone
+        assert s.launch_counts() == {"pass2": 1, "pass1": 1, "sweeps": 0, "checks": 0}
+        s.iterate(7)       # (4, 5) (6, 7) (8, 9) paired, 10 alone with its check
+        assert s.launch_counts() == {"pass2": 4, "pass1": 2, "sweeps": 0, "checks": 1}
+        s.iterate(10)      # (11, 12) ... (19, 20), the check of 20 after the last pair
+        assert s.launch_counts() == {"pass2": 9, "pass1": 2, "sweeps": 0, "checks": 2}
+        s.set_schedule(2).iterate(20)
+        assert s.launch_counts() == {"pass2": 9, "pass1": 22, "sweeps": 0, "checks": 4}
+        assert s.status()[0] == 40
+    with pd.Solver(_lib.TGV, np.float32, shape, h, 1e-3, 2.0) as s:
+        s.set_schedule(1).load(d2f, m).iterate(10)
+        assert s.launch_counts() == {"pass2": 0, "pass1": 0, "sweeps": 10, "checks": 1}
