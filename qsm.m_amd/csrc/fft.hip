@@ -109,9 +109,15 @@ int make_axis(AxisPlan &ap, size_t M, const char *what)
     AxisPlan np;
     QSM_ARG(factor(int(M), np.pl), "%s %zu has a prime factor above 13", what, M);
     const std::vector<Cx<double>> td = twiddles(np.pl);
-    std::vector<Cx<T>> t(td.size());
-    for (size_t i = 0; i < td.size(); ++i)
+    // the stage table, then the full circle exp(-2 pi i j / M), j < M
+    const size_t nst = td.size();
+    std::vector<Cx<T>> t(nst + M);
+    for (size_t i = 0; i < nst; ++i)
         t[i] = Cx<T>{T(td[i].re), T(td[i].im)};
+    for (size_t j = 0; j < M; ++j) {
+        const Cx<double> w = twiddle(long(j), long(M));
+        t[nst + j] = Cx<T>{T(w.re), T(w.im)};
+    }
     // plan at the head, the table after it on a 256-byte boundary
     constexpr size_t head = (sizeof(Plan) + 255) / 256 * 256;
     void *p = nullptr;
@@ -125,6 +131,7 @@ int make_axis(AxisPlan &ap, size_t M, const char *what)
     }
     np.dpl = static_cast<const Plan *>(p);
     np.dtw = static_cast<const char *>(p) + head;
+    np.dtwM = static_cast<const char *>(p) + head + nst * sizeof(Cx<T>);
     pc.plans.emplace(key, np);
     ap = np;
     return QSM_OK;
@@ -144,18 +151,46 @@ namespace {
 // D(-k))/2, the form a half spectrum can carry.  Ds = D except where an even
 // axis is at its Nyquist index (there -k has the same index, with the other
 // components negated, and a tilted B0 makes D(k) != D(-k)).
+//
+// Evaluated in the transform's precision T: with kz = b.k and kz' the same
+// sum at -k (Nyquist components kept), |k|^2 is common to both, so
+// Ds = 1/3 - (kz^2 + kz'^2) / (2 |k|^2) with one reciprocal (hardware
+// reciprocal in FP32, refined by two Newton steps in FP64), and away from
+// the Nyquist planes kz' = -kz exactly, so Ds = D with no branch.  Ds(0) = 0
+// (dipoleKernel.m:59-66).
 template <typename T>
 struct DipoleOp {
     int M0, M1, M2;
     double dk0, dk1, dk2, b0, b1, b2, scale;
+
+    __device__ static T coord(int m, int n, double dk)
+    {
+        const int low = n / 2;
+        return T(m < n - low ? m : m - n) * T(dk);
+    }
+
+    __device__ static T recip(T a)
+    {
+        if constexpr (sizeof(T) == 4) {
+            return __builtin_amdgcn_rcpf(a);
+        } else {
+            T r = __builtin_amdgcn_rcp(a);
+            r = __builtin_fma(r, __builtin_fma(-a, r, T(1)), r);
+            return __builtin_fma(r, __builtin_fma(-a, r, T(1)), r);
+        }
+    }
+
     __device__ Cx<T> operator()(int x, int y, int z, Cx<T> v) const
     {
-        const double X = kcoord(x, M0, dk0), Y = kcoord(y, M1, dk1), Z = kcoord(z, M2, dk2);
-        double D = dip(X, Y, Z, b0, b1, b2);
-        const bool nx = 2 * x == M0, ny = 2 * y == M1, nz = 2 * z == M2;
-        if (nx || ny || nz)
-            D = 0.5 * (D + dip(nx ? X : -X, ny ? Y : -Y, nz ? Z : -Z, b0, b1, b2));
-        const T d = T(D * scale);
+        const T X = coord(x, M0, dk0), Y = coord(y, M1, dk1), Z = coord(z, M2, dk2);
+        const T bx = T(b0) * X, by = T(b1) * Y, bz = T(b2) * Z;
+        const T kz = (bx + by) + bz;
+        const T kzm = ((2 * x == M0 ? bx : -bx) + (2 * y == M1 ? by : -by))
+                    + (2 * z == M2 ? bz : -bz);
+        const T k2 = (X * X + Y * Y) + Z * Z;
+        const T h = T(0.5) * (kz * kz + kzm * kzm);
+        const T D = k2 > T(0) ? T(1.0 / 3.0) - h * recip(k2) : T(0);
+        const T d = D * T(scale);
         return Cx<T>{v.re * d, v.im * d};
     }
 };

@@ -323,6 +323,14 @@ __device__ __forceinline__ Cx<T> *stockham(Cx<T> *A, Cx<T> *B, Pen pn,
     return A;
 }
 
+}  // namespace fft
+}  // namespace qsm
+
+#include "fft_reg.h"
+
+namespace qsm {
+namespace fft {
+
 // Pass geometry.  The volume is complex, dims M0 x M1 x M2, x fastest.
 // axis 0: pencils are rows (i1, i2); the block takes P consecutive rows of
 // one plane.  axis 1/2: pencils run along y/z; the block takes P
@@ -735,6 +743,8 @@ struct AxisPlan {
     const void *dtw = nullptr;  // device twiddle table in the transform's
                                 // precision (after the plan); the single
                                 // table is the double one rounded
+    const void *dtwM = nullptr; // exp(-2 pi i j / M), j < M, for the
+                                // register passes (after dtw)
 };
 
 template <typename T>
@@ -743,11 +753,62 @@ const Cx<T> *tw_of(const AxisPlan &ap)
     return static_cast<const Cx<T> *>(ap.dtw);
 }
 
+template <typename T>
+const Cx<T> *twM_of(const AxisPlan &ap)
+{
+    return static_cast<const Cx<T> *>(ap.dtwM);
+}
+
 // Plan of one transform length on the current device, built and uploaded
 // on first use and cached until the process ends (fft.hip).  Lengths must
 // factor into 2, 3, 5, 7, 11 and 13 and fit one pencil in LDS_MAX.
 template <typename T>
 int make_axis(AxisPlan &ap, size_t M, const char *what);
+
+// The y pass over planes [k0, k0 + nk) of the half spectrum in the
+// register form (fft_reg.h), when M1 has a split: forward reads rows
+// [lo, hi) and stores every row; inverse reads every row and stores rows
+// [lo, hi).  False: no split, nothing launched.
+template <typename T>
+bool y_reg(Cx<T> *work, const Vol &v, int k0, int nk, const AxisPlan &ap, bool forward, int lo,
+           int hi, const int *skip, hipStream_t st, hipError_t &err)
+{
+    reg::PenGeom pg;
+    pg.H = v.M0;
+    pg.npen = v.M0 * nk;
+    pg.base0 = size_t(k0) * v.M0 * v.M1;
+    pg.rstep = size_t(v.M0) * v.M1;
+    pg.istride = size_t(v.M0);
+    pg.lo_in = forward ? lo : 0;
+    pg.hi_in = forward ? hi : v.M1;
+    pg.lo_out = forward ? 0 : lo;
+    pg.hi_out = forward ? v.M1 : hi;
+    const reg::NoOp<T> none;
+    return forward
+        ? reg::launch<T, reg::FWD>(v.M1, work, work, pg, twM_of<T>(ap), none, skip, st, err)
+        : reg::launch<T, reg::LOAD_K | reg::INV>(v.M1, work, work, pg, twM_of<T>(ap), none,
+                                                 skip, st, err);
+}
+
+// The z pass in the register form, when M2 has a split: pencils (x, y) of
+// the half spectrum, planes [zlo, zhi) read (n-order loads) and stored
+// (n-order stores); MODE as reg::pen4.
+template <typename T, int MODE, typename Op>
+bool z_reg(Cx<T> *dst, const Cx<T> *src, const Vol &v, const AxisPlan &ap, int zlo, int zhi,
+           const Op &op, const int *skip, hipStream_t st, hipError_t &err)
+{
+    reg::PenGeom pg;
+    pg.H = v.M0;
+    pg.npen = v.M0 * v.M1;
+    pg.base0 = 0;
+    pg.rstep = size_t(v.M0);
+    pg.istride = size_t(v.M0) * v.M1;
+    pg.lo_in = zlo;
+    pg.hi_in = zhi;
+    pg.lo_out = zlo;
+    pg.hi_out = zhi;
+    return reg::launch<T, MODE>(v.M2, dst, src, pg, twM_of<T>(ap), op, skip, st, err);
+}
 
 // A real convolution on the stream: out = crop(real(IFFT3(op . FFT3(pad(in)))))
 // over the half spectrum, with the five passes described at the top.  M:
@@ -767,11 +828,16 @@ int inverse_yx(Cx<T> *work, T *out, const RealGeom &g, const Vol &v, const AxisP
     const size_t cs = sizeof(Cx<T>);
     const int H = v.M0, M1 = v.M1, M0 = int(ap[0].pl.M);
     const int ny = g.ny, nz = g.nz;
-    const Pen pny = pencils(M1, cs);
-    const int Py = 1 << pny.logP;
-    cy_pass<T><<<((H + Py - 1) / Py) * nz, FFT_THREADS, lds_bytes(pny, M1, cs), st>>>(
-        work, v, g.o2, pny, ap[1].dpl, tw_of<T>(ap[1]), T(1), 0, M1, g.o1, g.o1 + ny, skip);
-    QSM_HIP(hipGetLastError());
+    hipError_t e = hipSuccess;
+    if (y_reg<T>(work, v, g.o2, nz, ap[1], false, g.o1, g.o1 + ny, skip, st, e)) {
+        QSM_HIP(e);
+    } else {
+        const Pen pny = pencils(M1, cs);
+        const int Py = 1 << pny.logP;
+        cy_pass<T><<<((H + Py - 1) / Py) * nz, FFT_THREADS, lds_bytes(pny, M1, cs), st>>>(
+            work, v, g.o2, pny, ap[1].dpl, tw_of<T>(ap[1]), T(1), 0, M1, g.o1, g.o1 + ny, skip);
+        QSM_HIP(hipGetLastError());
+    }
     const Pen pn = pencils(M0, cs);
     const int P2 = 2 << pn.logP;
     rx_inv<T><<<(ny * nz + P2 - 1) / P2, FFT_THREADS, lds_bytes(pn, M0, cs), st>>>(
@@ -805,14 +871,22 @@ int real_conv(const T *in, T *out, const size_t sz[3], const size_t M[3], const 
             work, in, g, pn, ap[0].dpl, tw_of<T>(ap[0]), skip);
         QSM_HIP(hipGetLastError());
     }
-    const Pen pny = pencils(M1, cs);
-    const int Py = 1 << pny.logP;
-    {   // y forward on data planes
+    hipError_t e = hipSuccess;
+    // y forward on data planes
+    if (y_reg<T>(work, v, g.o2, nz, ap[1], true, g.o1, g.o1 + ny, skip, st, e)) {
+        QSM_HIP(e);
+    } else {
+        const Pen pny = pencils(M1, cs);
+        const int Py = 1 << pny.logP;
         cy_pass<T><<<((H + Py - 1) / Py) * nz, FFT_THREADS, lds_bytes(pny, M1, cs), st>>>(
             work, v, g.o2, pny, ap[1].dpl, tw_of<T>(ap[1]), T(-1), g.o1, g.o1 + ny, 0, M1, skip);
         QSM_HIP(hipGetLastError());
     }
-    {   // z forward, op (, z inverse)
+    // z forward, op (, z inverse)
+    constexpr int ZM = ROUND ? (reg::FWD | reg::OP | reg::INV) : (reg::FWD | reg::OP);
+    if (z_reg<T, ZM>(work, work, v, ap[2], g.o2, g.o2 + nz, op, skip, st, e)) {
+        QSM_HIP(e);
+    } else {
         const Pen pn = pencils(M2, cs);
         const int P = 1 << pn.logP;
         z_pass<T, ROUND, Op><<<((H + P - 1) / P) * M1, FFT_THREADS, lds_bytes(pn, M2, cs), st>>>(
@@ -844,11 +918,17 @@ int real_conv_from(const Cx<T> *S, T *out, const size_t sz[3], const size_t M[3]
     const RealGeom g{int(sz[0]), int(sz[1]), int(sz[2]), int(off[0]), int(off[1]), int(off[2]),
                      H, M1};
     const size_t cs = sizeof(Cx<T>);
-    const Pen pn = pencils(M2, cs);
-    const int P = 1 << pn.logP;
-    z_inv_pass<T, Op><<<((H + P - 1) / P) * M1, FFT_THREADS, lds_bytes(pn, M2, cs), st>>>(
-        S, work, v, pn, ap[2].dpl, tw_of<T>(ap[2]), g.o2, g.o2 + g.nz, op);
-    QSM_HIP(hipGetLastError());
+    hipError_t e = hipSuccess;
+    if (z_reg<T, reg::LOAD_K | reg::OP | reg::INV>(work, S, v, ap[2], g.o2, g.o2 + g.nz, op,
+                                                     nullptr, st, e)) {
+        QSM_HIP(e);
+    } else {
+        const Pen pn = pencils(M2, cs);
+        const int P = 1 << pn.logP;
+        z_inv_pass<T, Op><<<((H + P - 1) / P) * M1, FFT_THREADS, lds_bytes(pn, M2, cs), st>>>(
+            S, work, v, pn, ap[2].dpl, tw_of<T>(ap[2]), g.o2, g.o2 + g.nz, op);
+        QSM_HIP(hipGetLastError());
+    }
     return inverse_yx<T>(work, out, g, v, ap, st, nullptr);
 }
 

@@ -64,6 +64,30 @@ def test_dipole_conv_matches_fft_reference(cuda, dt, shape, vsz, bdir, pad):
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < tol
 
 
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("L", [176, 192, 256, 320, 360, 384, 448, 512, 640])
+def test_dipole_conv_register_pass_lengths(cuda, dt, L):
+    """y and z lengths that run the register-resident four-step passes
+    (fft_reg.h: 176 = 16x11 unpadded, the others padded 2x): the y forward,
+    the fused z forward / multiplier / inverse and the y inverse against
+    numpy's FFT.  A tilted B0 puts D(k) != D(-k) on the Nyquist planes; the
+    half spectrum's 21 x positions are not a multiple of the pencils per
+    block, so blocks straddle rows."""
+    rng = np.random.default_rng(L)
+    n, p = (L, 0) if L == 176 else (L // 2, L // 4)
+    shape, pad = (40, n, n - 2 * (L % 3 == 0)), (0, p, p + (L % 3 == 0))
+    M, _ = dipole.pad_size(shape, pad)
+    assert M[1:] == (L, L)
+    chi = np.asfortranarray(rng.standard_normal(shape).astype(dt))
+    vsz, bdir = (1.0, 0.9, 1.1), (0.3, -0.2, 0.93)
+    got = dipole.dipole_conv(chi, vsz, bdir, pad)
+    up = fastfft.padfastfft(chi.astype(np.float64), pad)
+    D = fastfft.dipole_kernel(up.shape, vsz, bdir)
+    ref = np.real(np.fft.ifftn(D * np.fft.fftn(up)))[fastfft.unpad_slices(up.shape, shape, pad)]
+    tol = 1e-12 if dt == np.float64 else 1e-5
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < tol
+
+
 def test_dipole_conv_pad_sizes_match_padfastfft(cuda):
     for shape, pad in (((24, 22, 20), (12, 11, 10)), ((7, 9, 11), (3, 0, 5)), ((176, 256, 5), (88, 128, 0))):
         M, off = dipole.pad_size(shape, pad)

@@ -153,3 +153,22 @@ def test_rts_device_form_on_side_stream(cuda, dt):
         st.synchronize()
         got = out.permute(2, 1, 0).contiguous().cpu().numpy().reshape(-1).reshape(shape, order="F")
         np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_tv_register_pass_lengths(cuda, dt):
+    """tv.m on an unpadded 24x256x176 grid: y (256 = 16x16) and z (176 =
+    16x11) run the register-resident FFT passes; the oracle to the
+    tolerance with the same rho decisions (FP64)."""
+    shape, vsz, bdir = (24, 256, 176), (1.0, 1.0, 1.0), (0.1, -0.2, 0.97)
+    f, m = _field(shape, vsz, bdir, seed=31)
+    f = f.astype(dt)
+    opts = {"tv": 1, "verbose": 0}
+    want, wtr = tva.tv(f, m, vsz, bdir, opts)
+    got, gtr = gtv.tv(f, m, vsz, bdir, opts, return_trace=True)
+    assert len(gtr) == len(wtr) > 2
+    if dt == np.float64:
+        assert rel_l2(got, want) < 1e-9
+        np.testing.assert_array_equal(gtr[:, 4], wtr[:, 4])
+    else:
+        assert rel_l2(got, want) < 1e-3
