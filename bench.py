@@ -67,6 +67,7 @@ def time_solver(kind, dtype, cfg, d2f, mask, steps, warmup):
     from qsm_amd import autotune
     b2 = autotune.best2(kind, dtype, shape)
     s.load(d2f.astype(dtype, order="F"), mask)
+    items = s.active_items()
     s.iterate(warmup, 0.0)
     stream.synchronize()
     torch.cuda.synchronize()
@@ -88,7 +89,8 @@ def time_solver(kind, dtype, cfg, d2f, mask, steps, warmup):
     assert it == warmup + steps and not stopped
     launches = {k: c1[k] - c0[k] for k in c1}
     return {"gpu_s": gpu, "wall_s": wall, "chi": chi, "tiles": tiles, "launches": launches,
-            "kz2": b2[0] if b2 else None, "schedule": b2[1] if b2 else 0}
+            "kz2": b2[0] if b2 else None, "schedule": b2[1] if b2 else 0,
+            "work_items": items[0], "work_item_kz": items[1]}
 
 
 def stream_ceiling_gbs(n=256 * 256 * 176, narr=15):
@@ -135,6 +137,8 @@ def brain_leg(steps=200, warmup=20):
     t = r["gpu_s"] / steps
     return {"ms_per_iter": t * 1e3, "voxel_iters_per_s": N / t, "n_active": act, "n": N,
             "active_fraction": act / N, "algorithmic_gbs": (act * 41 * 4 + N) / t / 1e9,
+            "frac": (act * 41 * 4 + N) / t / 1e9 / HBM_PEAK_GBS,
+            "work_items": r["work_items"], "work_item_kz": r["work_item_kz"],
             "tiles": r["tiles"], "steps": steps}
 
 

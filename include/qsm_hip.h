@@ -141,6 +141,19 @@ int qsm_pd_get_tiles(qsm_pd *s, int *tx, int *ty, int *kz);
  * at most one block per CU.  Results do not depend on it. */
 int qsm_pd_set_kz2(qsm_pd *s, int kz);
 
+/* Work list of the two-iteration pass for masked volumes.  At each load
+ * the handle finds, per tile, the planes holding active voxels and, when
+ * marching only those (in chunks of near-equal length) takes fewer plane
+ * steps per CU than the uniform grid, launches one block per item instead.
+ * Results are bitwise the same either way.  on: 0 the uniform grid, 1 the
+ * list where the cost model favours it (default), 2 the list whenever a
+ * voxel is active.  Takes effect at once on a loaded handle (the list is
+ * rebuilt synchronously).  No counterpart in the reference (its loop runs
+ * over every voxel, pd_tgv_mex.c:331-977). */
+int qsm_pd_set_active_items(qsm_pd *s, int on);
+/* Items of the current list (0: the uniform grid) and their chunk depth. */
+int qsm_pd_active_items(qsm_pd *s, int *nitems, int *kz);
+
 /* Kernel launches the handle has enqueued since it was created:
  * counts[0] two-iteration passes, [1] one-iteration fused passes, [2]
  * iterations of the two-sweep schedule, [3] convergence checks.  Launches

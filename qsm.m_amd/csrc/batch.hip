@@ -239,6 +239,7 @@ __device__ __forceinline__ void march2_tile(Lds2<T, TGV, threads2<BX, BY>()> &L,
     const DualCoef<T> dc = dp->dc;
     const PrimalCoef<T> pc = dp->pc;
     const int nx = dp->nx, ny = dp->ny, nz = dp->nz, kz = dp->kz;
+    const int k0 = 1 + bz * kz, k1 = min(k0 + kz, nz - 1);  // owned planes
     constexpr uint32_t S = sizeof(T);
     constexpr uint32_t NARR = TGV ? 15u : 6u;
     Ctx<T> c;
@@ -255,16 +256,16 @@ __device__ __forceinline__ void march2_tile(Lds2<T, TGV, threads2<BX, BY>()> &L,
     // one specialised march per (mode, check)
     if (mode == 2) {
         if (check) {
-            fused2_tile<T, TGV, BX, BY, true, 2>(L, c, dc, pc, nx, ny, nz, bx, by, bz, kz, acc);
+            fused2_tile<T, TGV, BX, BY, true, 2>(L, c, dc, pc, nx, ny, nz, bx, by, k0, k1, acc);
             block_sum4_f<NC>(acc, dp->partials + 4 * lt);
         } else {
-            fused2_tile<T, TGV, BX, BY, false, 2>(L, c, dc, pc, nx, ny, nz, bx, by, bz, kz, acc);
+            fused2_tile<T, TGV, BX, BY, false, 2>(L, c, dc, pc, nx, ny, nz, bx, by, k0, k1, acc);
         }
     } else if (check) {
-        fused2_tile<T, TGV, BX, BY, true, 1>(L, c, dc, pc, nx, ny, nz, bx, by, bz, kz, acc);
+        fused2_tile<T, TGV, BX, BY, true, 1>(L, c, dc, pc, nx, ny, nz, bx, by, k0, k1, acc);
         block_sum4_f<NC>(acc, dp->partials + 4 * lt);
     } else {
-        fused2_tile<T, TGV, BX, BY, false, 1>(L, c, dc, pc, nx, ny, nz, bx, by, bz, kz, acc);
+        fused2_tile<T, TGV, BX, BY, false, 1>(L, c, dc, pc, nx, ny, nz, bx, by, k0, k1, acc);
     }
 }
 

@@ -111,15 +111,15 @@ __device__ __forceinline__ void region_cell(int t, int &tx, int &ty, int &dep)
     dep = -1;
 }
 
-// One tile (bx, by) of z-chunk bz of a BX x BY region: MODE 2 advances the
+// One tile (bx, by), owned planes [k0, k1), of a BX x BY region: MODE 2 advances the
 // tile two iterations (primal n and n+1, dual n+1 and n+2), MODE 1 one.
 // Convergence terms of the last primal update of the owned voxels
 // accumulate into acc (CHECK).  Every thread of the block calls it.
 template <typename T, bool TGV, int BX, int BY, bool CHECK, int MODE>
 __device__ __forceinline__ void fused2_tile(Lds2<T, TGV, threads2<BX, BY>()> &L, const Ctx<T> &c,
                                             const DualCoef<T> &dc, const PrimalCoef<T> &pc,
-                                            int nx, int ny, int nz, int bx, int by, int bz,
-                                            int kz, double acc[4])
+                                            int nx, int ny, int nz, int bx, int by, int k0,
+                                            int k1, double acc[4])
 {
     using I = Ix<TGV>;
     constexpr uint32_t S = sizeof(T);
@@ -133,8 +133,6 @@ __device__ __forceinline__ void fused2_tile(Lds2<T, TGV, threads2<BX, BY>()> &L,
     const int x = bx * TX - 3 + tx, y = by * TY - 3 + ty;
     const bool valid = dep >= 0 && x >= 0 && x < nx && y >= 0 && y < ny;
     const bool own = valid && dep >= 3;
-    const int k0 = 1 + bz * kz;
-    const int k1 = min(k0 + kz, nz - 1);  // owned planes [k0, k1)
     // two iterations: stage 1 from k0 - 3 (stage 4's first plane k0 needs
     // stage 3 at k0 - 1, stage 2 at k0 - 2, stage 1 at k0 - 3) to k1 + 2;
     // one iteration: stages 1-2 only, stage 2 on the owned planes

@@ -24,10 +24,13 @@ constexpr int NB_TV = 6;    // chi, eta, nu, p[3]
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 struct Layout {
-    size_t bank_bytes, f_off, mask_off, ctl_off, part_off, hist_off, total;
+    size_t bank_bytes, f_off, mask_off, ctl_off, part_off, hist_off, items_off, total;
 };
 
-Layout layout(int kind, int dtype, const size_t sz[3], int npart, int hist_cap)
+// items: room for a work list of the two-iteration pass (npart items; the
+// allocation-free solve has none)
+Layout layout(int kind, int dtype, const size_t sz[3], int npart, int hist_cap,
+              bool items = true)
 {
     const size_t N = sz[0] * sz[1] * sz[2];
     const size_t s = dtype_size(dtype);
@@ -39,7 +42,8 @@ Layout layout(int kind, int dtype, const size_t sz[3], int npart, int hist_cap)
     L.ctl_off = L.mask_off + align_up(N, 256);
     L.part_off = L.ctl_off + 256;
     L.hist_off = L.part_off + align_up(size_t(npart) * 4 * sizeof(double), 256);
-    L.total = L.hist_off + align_up(size_t(hist_cap) * 2 * sizeof(double), 256);
+    L.items_off = L.hist_off + align_up(size_t(hist_cap) * 2 * sizeof(double), 256);
+    L.total = L.items_off + (items ? align_up(size_t(npart) * sizeof(F2Item), 256) : 0);
     return L;
 }
 
@@ -119,6 +123,9 @@ struct qsm_pd {
     // kernel launches enqueued since creation: two-iteration passes,
     // one-iteration passes, two-sweep iterations, convergence checks
     uint64_t launches[4];
+    // work list of the two-iteration pass (built at load when it beats the
+    // uniform grid): nitems items of at most items_kz planes; 0 = uniform
+    int items_on, nitems, items_kz;
 
     void *bank(int b) const { return mem + size_t(b) * L.bank_bytes; }
     void *f() const { return mem + L.f_off; }
@@ -126,10 +133,95 @@ struct qsm_pd {
     Ctl *ctl() const { return reinterpret_cast<Ctl *>(mem + L.ctl_off); }
     double *partials() const { return reinterpret_cast<double *>(mem + L.part_off); }
     double *hist() const { return reinterpret_cast<double *>(mem + L.hist_off); }
+    F2Item *items() const { return reinterpret_cast<F2Item *>(mem + L.items_off); }
     Geom geom() const { return Geom{int(sz[0]), int(sz[1]), int(sz[2])}; }
 };
 
 namespace {
+
+// Work list of the two-iteration pass for a masked volume.  A tile's
+// owned voxels outside its active plane range [lo, hi) (no mask bit) are
+// never stored by the pass, so marching only [lo, hi) gives the same
+// iterates, and any split of it into chunks (each recomputing its own halo,
+// like the uniform grid's chunks) too.  The ranges are cut on one grid of
+// z-chunks shared by all tiles, chunk j = planes [1 - o + j kz, 1 - o +
+// (j + 1) kz): neighbouring tiles' items then march the same planes at the
+// same time and their ring reads meet in L2 (ranges split per tile instead
+// start at different planes: 50 % more HBM reads, a third of the gain).
+// Items are ordered by (chunk, tile row, tile column), so consecutive items
+// -- one XCD's blocks -- are patches of neighbours.  An item of len planes
+// costs about len + 6 plane steps (the four-stage pipeline's fill) and
+// `slots` items run at a time (CUs x resident blocks); the grid (kz >= 8
+// or the interior, offset o) taken is the one with the fewest rounds, then
+// the shortest longest item (offsets sampled at kz/16 steps), and the list is kept when its rounds x (len +
+// 6) beat the uniform grid's by 10 % (items_on == 2: whenever a voxel is
+// active).  Synchronous (the item count sizes the launches).
+int build_items(qsm_pd *s)
+{
+    s->nitems = 0;
+    s->items_kz = 0;
+    const Geom g = s->geom();
+    if (!s->items_on || g.nx < 3 || g.ny < 3 || g.nz < 3 || !fused_size_supported(s->kind, s->dtype, s->N))
+        return QSM_OK;
+    int tx, ty;
+    fused2_tile(s->dtype, tx, ty);
+    const int ntx = (g.nx + tx - 1) / tx, nty = (g.ny + ty - 1) / ty, nt = ntx * nty;
+    int *dr = reinterpret_cast<int *>(s->items());  // 2 nt ints <= npart items
+    QSM_HIP(launch_tile_zrange(s->mask(), g, s->dtype, dr, s->stream));
+    std::vector<int> r(2 * size_t(nt));
+    QSM_HIP(hipMemcpyAsync(r.data(), dr, r.size() * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    QSM_HIP(hipStreamSynchronize(s->stream));
+    const long slots = long(cu_count()) * fused2_resident(s->kind, s->dtype);
+    const long nb = fused2_blocks(g, s->dtype, s->kz2);
+    const long cost_u = (nb + slots - 1) / slots * (s->kz2 + 6);
+    long act = 0;
+    for (int t = 0; t < nt; ++t)
+        act += r[2 * t + 1] > r[2 * t];
+    if (act == 0)
+        return QSM_OK;  // no active voxel: the uniform grid
+    const long rmin = (act + slots - 1) / slots;  // every active tile needs an item
+    long best = -1;
+    int bkz = 0, bo = 0;
+    for (int kz = std::min(8, g.nz - 2); kz <= g.nz - 2; ++kz)
+        for (int o = 0; o < kz; o += std::max(1, kz / 16)) {  // 16 offsets at most
+            long cnt = 0;
+            int len = 0;
+            for (int t = 0; t < nt; ++t) {
+                const int lo = r[2 * t], hi = r[2 * t + 1];
+                if (hi <= lo) continue;
+                const int j0 = (lo - 1 + o) / kz, j1 = (hi - 2 + o) / kz;
+                cnt += j1 - j0 + 1;
+                for (int j = j0; j <= j1; ++j)
+                    len = std::max(len, std::min(hi, 1 - o + (j + 1) * kz)
+                                            - std::max(lo, 1 - o + j * kz));
+            }
+            const long rounds = (cnt + slots - 1) / slots;
+            if (rounds > rmin) continue;
+            const long cost = rounds * (len + 6);
+            if (best < 0 || cost < best) { best = cost; bkz = kz; bo = o; }
+        }
+    if (bkz == 0 || (s->items_on == 1 && best * 10 >= cost_u * 9))
+        return QSM_OK;
+    std::vector<F2Item> it;
+    for (int j = 0; 1 - bo + j * bkz < g.nz - 1; ++j) {
+        const int c0 = 1 - bo + j * bkz, c1 = c0 + bkz;
+        for (int by = 0; by < nty; ++by)
+            for (int bx = 0; bx < ntx; ++bx) {
+                const int t = bx + ntx * by;
+                const int k0 = std::max(c0, r[2 * t]), k1 = std::min(c1, r[2 * t + 1]);
+                if (k0 < k1)
+                    it.push_back(F2Item{bx, by, k0, k1});
+            }
+    }
+    if (it.size() > size_t(s->npart))
+        return QSM_OK;
+    QSM_HIP(hipMemcpyAsync(s->items(), it.data(), it.size() * sizeof(F2Item),
+                           hipMemcpyHostToDevice, s->stream));
+    QSM_HIP(hipStreamSynchronize(s->stream));
+    s->nitems = int(it.size());
+    s->items_kz = bkz;
+    return QSM_OK;
+}
 
 template <typename T>
 Bank<T> make_bank(void *base, int kind, size_t N)
@@ -200,16 +292,19 @@ int iterate_impl(qsm_pd *s, uint32_t niter, double tol)
             double *part2 = check2 ? s->partials() : nullptr;
             const T *in = static_cast<const T *>(s->bank(s->cur));
             T *out = static_cast<T *>(s->bank(s->cur ^ 1u));
+            const F2Item *items = s->nitems ? s->items() : nullptr;
             const hipError_t e = tgv
-                ? launch_fused2<T, true>(in, out, s->N, f, s->mask(), g, s->kz2, dc, pc, stop, part2, s->stream)
-                : launch_fused2<T, false>(in, out, s->N, f, s->mask(), g, s->kz2, dc, pc, stop, part2, s->stream);
+                ? launch_fused2<T, true>(in, out, s->N, f, s->mask(), g, s->kz2, dc, pc, stop, part2,
+                                         items, s->nitems, s->stream)
+                : launch_fused2<T, false>(in, out, s->N, f, s->mask(), g, s->kz2, dc, pc, stop, part2,
+                                          items, s->nitems, s->stream);
             QSM_HIP(e);
             s->cur ^= 1u;
             ++s->launches[0];
             if (check2) {
-                QSM_HIP(launch_conv_finalize(s->partials(), fused2_blocks(g, s->dtype, s->kz2),
-                                             s->ctl(), s->hist(), s->hist_cap, tol, n2, s->cur,
-                                             s->stream));
+                const int nb2 = s->nitems ? s->nitems : fused2_blocks(g, s->dtype, s->kz2);
+                QSM_HIP(launch_conv_finalize(s->partials(), nb2, s->ctl(), s->hist(), s->hist_cap,
+                                             tol, n2, s->cur, s->stream));
                 ++s->launches[3];
             }
             s->n = n2;
@@ -311,6 +406,7 @@ int create_impl(qsm_pd **out, int kind, int dtype, const size_t sz[3],
     const Geom g = s->geom();
     s->npart = max_partials(g);
     s->hist_cap = hist_cap;
+    s->items_on = 1;
     s->L = layout(kind, dtype, sz, s->npart, hist_cap);
     const hipError_t e = hipMalloc(reinterpret_cast<void **>(&s->mem), s->L.total);
     if (e != hipSuccess) {
@@ -461,7 +557,7 @@ int pd_solve_static(int kind, int dtype, const void *d2f, const uint8_t *mask,
     qsm_step_sizes(h, &s.tau, &s.sigma);
     s.npart = max_partials(s.geom());
     s.hist_cap = 1;
-    s.L = layout(kind, dtype, sz, s.npart, s.hist_cap);
+    s.L = layout(kind, dtype, sz, s.npart, s.hist_cap, false);
     QSM_ARG(s.L.total <= size_t(QSM_PD_STATIC_BYTES),
             "volume %zux%zux%zu needs %zu bytes of solver state, more than the %zu-byte "
             "static work space", sz[0], sz[1], sz[2], s.L.total,
@@ -536,6 +632,8 @@ int qsm_pd_load(qsm_pd *s, const void *d2f, const uint8_t *mask, int on_device)
     QSM_HIP(launch_ctl_reset(s->ctl(), 1, s->stream));
     if (!on_device)
         QSM_HIP(hipStreamSynchronize(s->stream));  // host buffers may go away
+    const int rc = build_items(s);
+    if (rc) return rc;
     s->n = 0;
     s->dual_ahead = 0;
     s->cur = 0;
@@ -633,6 +731,27 @@ int qsm_pd_set_kz2(qsm_pd *s, int kz)
             "chunk depth needs %d partial-sum slots, %d allocated",
             fused2_blocks(s->geom(), s->dtype, kz), s->npart);
     s->kz2 = kz;
+    return QSM_OK;
+}
+
+int qsm_pd_set_active_items(qsm_pd *s, int on)
+{
+    QSM_ARG(s, "null handle");
+    QSM_ARG(on >= 0 && on <= 2, "on must be 0, 1 or 2");
+    s->items_on = on;
+    if (!s->items_on) {
+        s->nitems = 0;
+        s->items_kz = 0;
+        return QSM_OK;
+    }
+    return s->loaded ? build_items(s) : QSM_OK;
+}
+
+int qsm_pd_active_items(qsm_pd *s, int *nitems, int *kz)
+{
+    QSM_ARG(s, "null handle");
+    if (nitems) *nitems = s->nitems;
+    if (kz) *kz = s->items_kz;
     return QSM_OK;
 }
 

@@ -80,17 +80,31 @@ hipError_t launch_fused(const T *in, T *out, size_t N, const T *f,
 int fused_blocks(Geom g, FusedShape sh);
 bool fused_shape_supported(int tx, int ty);
 
+// One work item of the two-iteration pass: tile (bx, by), owned planes
+// [k0, k1).
+struct F2Item {
+    int bx, by, k0, k1;
+};
+
 // Two-iteration pass (fused2.hip): reads bank `in` (primal n-1, dual n)
 // and writes bank `out` (primal n+1, dual n+2); partials (if non-null)
-// receive the convergence sums of primal n+1.  Tiles of fused2_tile(dtype)
-// marching kz planes.
+// receive the convergence sums of primal n+1, one slot per block.  Tiles of
+// fused2_tile(dtype) marching kz planes, or, with items, one block per
+// item (nitems blocks).
 template <typename T, bool TGV>
 hipError_t launch_fused2(const T *in, T *out, size_t N, const T *f,
                          const uint8_t *mk, Geom g, int kz,
                          const DualCoef<T> &dc, const PrimalCoef<T> &pc,
-                         const int *stop, double *partials, hipStream_t st);
+                         const int *stop, double *partials, const F2Item *items, int nitems,
+                         hipStream_t st);
 void fused2_tile(int dtype, int &tx, int &ty);
 int fused2_blocks(Geom g, int dtype, int kz);
+// Blocks of the two-iteration kernel one CU holds at once.
+int fused2_resident(int kind, int dtype);
+// Per tile of fused2_tile(dtype), row-major: out[2t], out[2t+1] = the plane
+// range [lo, hi) in [1, nz - 1) holding a voxel with a mask bit (lo == hi:
+// none).
+hipError_t launch_tile_zrange(const uint8_t *mask, Geom g, int dtype, int *out, hipStream_t st);
 // The fused pass addresses a bank with 32-bit byte offsets (< 4 GiB).
 bool fused_size_supported(int kind, int dtype, size_t N);
 

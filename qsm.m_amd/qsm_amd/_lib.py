@@ -61,6 +61,8 @@ _SIGS = {
     "qsm_pd_set_tiles": (C.c_int, [_p, C.c_int, C.c_int, C.c_int]),
     "qsm_pd_set_kz2": (C.c_int, [_p, C.c_int]),
     "qsm_pd_launch_counts": (C.c_int, [_p, C.POINTER(C.c_uint64)]),
+    "qsm_pd_set_active_items": (C.c_int, [_p, C.c_int]),
+    "qsm_pd_active_items": (C.c_int, [_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "qsm_pd_get_tiles": (C.c_int, [_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "qsm_tgv_dual_dev": (C.c_int, [C.c_int, _p, _p, _p, _p, _p, _p, _p, _u8p,
                                    C.POINTER(_sz_t), _dp, C.c_double, C.c_double,

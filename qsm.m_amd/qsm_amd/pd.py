@@ -153,6 +153,18 @@ class Solver:
         check(lib().qsm_pd_set_kz2(self._h, int(kz)))
         return self
 
+    def set_active_items(self, on: int):
+        """Work list of active tiles for masked volumes: 0 off, 1 where the
+        cost model favours it (default), 2 always; identical results."""
+        check(lib().qsm_pd_set_active_items(self._h, int(on)))
+        return self
+
+    def active_items(self):
+        """(items, chunk depth) of the current work list; (0, 0): uniform grid."""
+        n, kz = C.c_int(), C.c_int()
+        check(lib().qsm_pd_active_items(self._h, C.byref(n), C.byref(kz)))
+        return n.value, kz.value
+
     def launch_counts(self) -> dict:
         """Kernel launches enqueued since creation, by kind."""
         c = (C.c_uint64 * 4)()

@@ -331,3 +331,73 @@ def test_launch_counts_follow_the_schedule(cuda):
     with pd.Solver(_lib.TGV, np.float32, shape, h, 1e-3, 2.0) as s:
         s.set_schedule(1).load(d2f, m).iterate(10)
         assert s.launch_counts() == {"pass2": 0, "pass1": 0, "sweeps": 10, "checks": 1}
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+@pytest.mark.parametrize("kind", [_lib.TGV, _lib.TV])
+def test_active_items_match_the_uniform_grid(cuda, dt, kind):
+    """A brain-like mask filling half the box: the two-iteration pass over
+    the work list of active tiles (forced on) gives chi bitwise the uniform
+    grid's and the oracle's, after an odd count with convergence checks and
+    at a tolerance stop (same iteration)."""
+    shape, h = (90, 70, 60), (1.0, 0.9, 1.1)
+    d2f = _phantom_input(shape, h, 21, dt)
+    m = brainish_mask(shape, seed=3, frac=0.5)
+    alpha = 2.0 if kind == _lib.TGV else 0.0
+
+    def oracle(tol, iters):
+        if kind == _lib.TGV:
+            return orc.pd_tgv(d2f, m, h, 1e-3, alpha, tol, iters)
+        return orc.pd_tv(d2f, m, h, 1e-3, tol, iters)
+
+    def solve(on, tol, iters, calls=None):
+        with pd.Solver(kind, dt, shape, h, 1e-3, alpha) as s:
+            s.set_active_items(on)
+            s.load(d2f, m)
+            n, kz = s.active_items()
+            for c in (calls or [iters]):
+                s.iterate(c, tol)
+            it, stopped, _ = s.status()
+            return s.chi(), n, kz, it, stopped
+
+    chi_l, n, kz, it, _ = solve(2, 0.0, 37, [3, 24, 10])
+    assert n > 0 and kz >= 8 and it == 37
+    chi_u, n0, _, _, _ = solve(0, 0.0, 37)
+    assert n0 == 0
+    np.testing.assert_array_equal(chi_l, chi_u)
+    np.testing.assert_array_equal(chi_l, oracle(0.0, 37)[0])
+    ref, it_ref, _ = oracle(5e-3, 2000)
+    assert 10 <= it_ref < 2000
+    chi_s, _, _, it_s, stopped = solve(2, 5e-3, 2000)
+    assert stopped and it_s == it_ref
+    np.testing.assert_array_equal(chi_s, ref)
+
+
+def test_active_items_cost_model_and_edge_masks(cuda):
+    """The cost model keeps the uniform grid for a full-box mask (nothing to
+    skip) and for an empty one, and picks the list for a small blob in a
+    large box; switching the list off and on again on a loaded handle keeps
+    the iterates bitwise."""
+    shape, h = (160, 160, 120), (1.0, 1.0, 1.0)
+    d2f = _phantom_input(shape, h, 5, np.float32)
+    full = np.ones(shape, np.uint8)
+    empty = np.zeros(shape, np.uint8)
+    blob = brainish_mask(shape, seed=4, frac=0.35)
+    with pd.Solver(_lib.TGV, np.float32, shape, h, 1e-3, 2.0) as s:
+        assert s.load(d2f, full).active_items() == (0, 0)
+        assert s.load(d2f, empty).active_items() == (0, 0)
+        s.iterate(4)
+        assert np.all(s.chi() == 0)
+        s.load(d2f, blob)
+        n, kz = s.active_items()
+        assert n > 0 and kz >= 8
+        s.iterate(6)
+        s.set_active_items(0)
+        assert s.active_items() == (0, 0)
+        s.iterate(6)
+        s.set_active_items(1)
+        assert s.active_items()[0] == n
+        s.iterate(9)
+        got = s.chi()
+    ref = orc.pd_tgv(d2f, blob, h, 1e-3, 2.0, 0.0, 21)[0]
+    np.testing.assert_array_equal(got, ref)
