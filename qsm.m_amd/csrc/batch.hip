@@ -65,7 +65,11 @@ struct VolDesc {
     uint32_t cur;            // bank holding the iterates of `iter`
     int mode;                // two-iteration march: iterations of the next launch
     uint32_t chunk_end;      // two-iteration march: iteration ending this chunk
-    int pad;
+    // two-iteration march: the volume's entries of the tile list as work
+    // items (tile, owned planes) when its mask leaves planes to skip, else
+    // null (the uniform tiles x chunks); nitems entries
+    int nitems;
+    const F2Item *items;
 };
 
 struct TileList {
@@ -231,15 +235,24 @@ __device__ __forceinline__ void march2_tile(Lds2<T, TGV, threads2<BX, BY>()> &L,
     constexpr int NC = threads2<BX, BY>();
     const int lt = t - dp->tile0;
     const int ntx = dp->ntx, nty = dp->nty;
-    const int bx = lt % ntx;
-    const int by = (lt / ntx) % nty;
-    const int bz = lt / (ntx * nty);
     const bool check = ((dp->iter + uint32_t(mode)) % 10) == 0;  // pd_tgv_mex.c:298,309
     const T *bi = dp->bank[dp->cur & 1u], *bo = dp->bank[(dp->cur & 1u) ^ 1u];
     const DualCoef<T> dc = dp->dc;
     const PrimalCoef<T> pc = dp->pc;
     const int nx = dp->nx, ny = dp->ny, nz = dp->nz, kz = dp->kz;
-    const int k0 = 1 + bz * kz, k1 = min(k0 + kz, nz - 1);  // owned planes
+    int bx, by, k0, k1;  // tile and owned planes
+    if (dp->items) {
+        const F2Item it = dp->items[lt];
+        bx = it.bx;
+        by = it.by;
+        k0 = it.k0;
+        k1 = it.k1;
+    } else {
+        bx = lt % ntx;
+        by = (lt / ntx) % nty;
+        k0 = 1 + (lt / (ntx * nty)) * kz;
+        k1 = min(k0 + kz, nz - 1);
+    }
     constexpr uint32_t S = sizeof(T);
     constexpr uint32_t NARR = TGV ? 15u : 6u;
     Ctx<T> c;
@@ -343,7 +356,7 @@ __global__ __launch_bounds__(ADV_T) void batch_advance2_kernel(VolDesc<T> *vols,
     const uint32_t n = d.iter + uint32_t(mode);
     const uint32_t bank = (d.cur & 1u) ^ 1u;
     if (n % 10 == 0) {
-        const int nblk = d.ntx * d.nty * d.nch;
+        const int nblk = d.items ? d.nitems : d.ntx * d.nty * d.nch;
         double s[4] = {0.0, 0.0, 0.0, 0.0};
         for (int b = threadIdx.x; b < nblk; b += ADV_T)
             for (int q = 0; q < 4; ++q)
@@ -673,7 +686,8 @@ int add_t(qsm_batch *b, int kind, const size_t sz[3], const double h[3], double 
     const size_t off_c = off_m + align_up(N, 256);
     const size_t off_p = off_c + 256;
     const size_t off_h = off_p + align_up(size_t(std::max(ntiles, 1)) * 4 * sizeof(double), 256);
-    const size_t bytes = off_h + align_up(size_t(hist_cap) * 2 * sizeof(double), 256);
+    const size_t off_i = off_h + align_up(size_t(hist_cap) * 2 * sizeof(double), 256);
+    const size_t bytes = off_i + (b->two ? align_up(size_t(std::max(ntiles, 1)) * sizeof(F2Item), 256) : 0);
     if (b->used + bytes > b->budget) {
         set_error("admission refused: %zu bytes needed, %zu of %zu in use", bytes, b->used, b->budget);
         return QSM_EFULL;
@@ -735,17 +749,50 @@ int add_t(qsm_batch *b, int kind, const size_t sz[3], const double h[3], double 
         QSM_HIP(launch_dual_sweep<T, true>(k0, bars, f, mk, 2, g, dc, nullptr, b->stream));
     else
         QSM_HIP(launch_dual_sweep<T, false>(k0, bars, f, mk, 2, g, dc, nullptr, b->stream));
+    const int ntx = (g.nx + ttx - 1) / ttx, nty = (g.ny + tty - 1) / tty;
+    const int nch = (g.nz - 2 + kz - 1) / kz;
+    // Two-iteration march: each (tile, chunk) of the uniform list trimmed to
+    // the planes where the tile holds active voxels, empty ones dropped.  A
+    // tile's voxels outside those planes are never stored by the march, so
+    // the iterates are the same; the list keeps its order (chunk, row,
+    // column), so neighbours still march the same planes together.
+    F2Item *items = b->two ? reinterpret_cast<F2Item *>(mem + off_i) : nullptr;
+    int nitems = 0;
+    if (items && ntiles > 0) {
+        int *dr = reinterpret_cast<int *>(mem + off_p);  // 2 ntx nty ints fit the partials
+        QSM_HIP(launch_tile_zrange(mk, g, b->dtype, dr, b->stream));
+        std::vector<int> r(2 * size_t(ntx) * nty);
+        QSM_HIP(hipMemcpyAsync(r.data(), dr, r.size() * sizeof(int), hipMemcpyDeviceToHost,
+                               b->stream));
+        QSM_HIP(hipStreamSynchronize(b->stream));
+        std::vector<F2Item> it;
+        for (int j = 0; j < nch; ++j) {
+            const int c0 = 1 + j * kz, c1 = std::min(c0 + kz, g.nz - 1);
+            for (int by = 0; by < nty; ++by)
+                for (int bx = 0; bx < ntx; ++bx) {
+                    const int t = bx + ntx * by;
+                    const int a = std::max(c0, r[2 * t]), e = std::min(c1, r[2 * t + 1]);
+                    if (a < e)
+                        it.push_back(F2Item{bx, by, a, e});
+                }
+        }
+        if (!it.empty() && int(it.size()) <= ntiles) {
+            QSM_HIP(hipMemcpyAsync(items, it.data(), it.size() * sizeof(F2Item),
+                                   hipMemcpyHostToDevice, b->stream));
+            nitems = int(it.size());
+        }
+    }
     QSM_HIP(hipStreamSynchronize(b->stream));
+    if (nitems)
+        bv.ntiles = nitems;
 
     VolDesc<T> &d = desc<T>(b, v);
     std::memset(&d, 0, sizeof d);
     d.nx = g.nx; d.ny = g.ny; d.nz = g.nz; d.kind = kind;
     d.N = uint32_t(N);
-    int ntx, nty, nch;
-    ntx = (g.nx + ttx - 1) / ttx;
-    nty = (g.ny + tty - 1) / tty;
-    nch = (g.nz - 2 + kz - 1) / kz;
     d.ntx = ntx; d.nty = nty; d.nch = ntiles ? nch : 0; d.kz = kz;
+    d.nitems = nitems;
+    d.items = nitems ? items : nullptr;
     d.bank[0] = b0; d.bank[1] = b1;
     d.f = f; d.mk = mk; d.ctl = ctl;
     d.partials = reinterpret_cast<double *>(mem + off_p);
@@ -878,9 +925,11 @@ size_t qsm_batch_volume_bytes(int kind, int dtype, const size_t sz[3], int tx, i
     const int ntiles = (tx == 0 && ty == 0) ? fused2_blocks(g, dtype, kz)
                                             : fused_blocks(g, FusedShape{tx, ty, kz});
     const size_t bank = align_up(size_t(kind == QSM_TGV ? 15 : 6) * N * s, 256);
+    const bool two = tx == 0 && ty == 0;
     return 2 * bank + align_up(N * s, 256) + align_up(N, 256) + 256
            + align_up(size_t(std::max(ntiles, 1)) * 32, 256)
-           + align_up(size_t(maxiter / 10 + 1) * 16, 256);
+           + align_up(size_t(maxiter / 10 + 1) * 16, 256)
+           + (two ? align_up(size_t(std::max(ntiles, 1)) * sizeof(F2Item), 256) : 0);
 }
 
 int qsm_batch_add(qsm_batch *b, int kind, const size_t sz[3], const double h[3],
