@@ -323,6 +323,13 @@ __device__ __forceinline__ Cx<T> *stockham(Cx<T> *A, Cx<T> *B, Pen pn,
     return A;
 }
 
+// Where the real volume (nx, ny, nz) sits in the transform grid: offsets
+// (o0, o1, o2) of zero padding before it; H = M0/2 + 1 half-spectrum
+// columns; M1 the y length.
+struct RealGeom {
+    int nx, ny, nz, o0, o1, o2, H, M1;
+};
+
 }  // namespace fft
 }  // namespace qsm
 
@@ -460,13 +467,6 @@ __global__ __launch_bounds__(FFT_THREADS) void pass_kernel(
 }
 
 // ---- real-convolution passes ---------------------------------------------
-
-// Where the real volume (nx, ny, nz) sits in the transform grid: offsets
-// (o0, o1, o2) of zero padding before it; H = M0/2 + 1 half-spectrum
-// columns; M1 the y length.
-struct RealGeom {
-    int nx, ny, nz, o0, o1, o2, H, M1;
-};
 
 // x pass, forward: reads the real input for rows (j, k) inside the data
 // region only.  Rows 2p and 2p+1 of the block travel as the real and
@@ -838,6 +838,10 @@ int inverse_yx(Cx<T> *work, T *out, const RealGeom &g, const Vol &v, const AxisP
             work, v, g.o2, pny, ap[1].dpl, tw_of<T>(ap[1]), T(1), 0, M1, g.o1, g.o1 + ny, skip);
         QSM_HIP(hipGetLastError());
     }
+    if (reg::launch_x<T>(M0, false, work, nullptr, out, g, twM_of<T>(ap[0]), skip, st, e)) {
+        QSM_HIP(e);
+        return QSM_OK;
+    }
     const Pen pn = pencils(M0, cs);
     const int P2 = 2 << pn.logP;
     rx_inv<T><<<(ny * nz + P2 - 1) / P2, FFT_THREADS, lds_bytes(pn, M0, cs), st>>>(
@@ -864,14 +868,17 @@ int real_conv(const T *in, T *out, const size_t sz[3], const size_t M[3], const 
                      H, M1};
     const int ny = g.ny, nz = g.nz;
     const size_t cs = sizeof(Cx<T>);
-    {   // x forward on data rows, two per transform
+    hipError_t e = hipSuccess;
+    // x forward on data rows, two per transform
+    if (reg::launch_x<T>(M0, true, work, in, nullptr, g, twM_of<T>(ap[0]), skip, st, e)) {
+        QSM_HIP(e);
+    } else {
         const Pen pn = pencils(M0, cs);
         const int P2 = 2 << pn.logP;
         rx_fwd<T><<<(ny * nz + P2 - 1) / P2, FFT_THREADS, lds_bytes(pn, M0, cs), st>>>(
             work, in, g, pn, ap[0].dpl, tw_of<T>(ap[0]), skip);
         QSM_HIP(hipGetLastError());
     }
-    hipError_t e = hipSuccess;
     // y forward on data planes
     if (y_reg<T>(work, v, g.o2, nz, ap[1], true, g.o1, g.o1 + ny, skip, st, e)) {
         QSM_HIP(e);

@@ -354,6 +354,206 @@ bool launch(int M, Cx<T> *dst, const Cx<T> *src, const PenGeom &g, const Cx<T> *
     }
 }
 
+// ---- x passes of the real convolution --------------------------------------
+//
+// Pencils run along x, so the lanes of a wave take consecutive transform
+// indices of one pencil (thread index m fastest): every global access is a
+// run of a row.  Each complex pencil carries two real rows, 2p and 2p + 1 of
+// the block, as its real and imaginary parts.  The exchange buffer holds
+// entry (a, b) of a pencil at a (N2 + 1) + b: writes run along b, reads
+// along a with an odd stride, both on distinct LDS banks.
+
+template <int N1, int N2>
+constexpr int xbuf() { return N1 * (N2 + 1) > N1 * N2 + N1 ? N1 * (N2 + 1) : N1 * N2 + N1; }
+
+// Pencils (row pairs) per block of an x pass.
+template <typename T>
+constexpr int xpencils() { return sizeof(T) == 4 ? 8 : 4; }
+
+// Forward: rows (j, k) of the real input inside the data region (zero
+// padding fused into the load), stored as the half spectra 2A(k), 2B(k),
+// k < H, of the two rows (A = (Z(k) + conj Z(M-k))/2, B = (Z(k) - conj
+// Z(M-k))/2i with Z the spectrum of the packed pencil), as the Stockham
+// rx_fwd stores them.
+template <typename T, int N1, int N2, int P>
+__global__ __launch_bounds__(P * (N1 > N2 ? N1 : N2)) void rx_fwd_reg(
+    Cx<T> *data, const T *in, RealGeom g, const Cx<T> *twM, const int *skip)
+{
+#pragma clang fp contract(fast)
+    constexpr int M = N1 * N2, TM = team<N1, N2>(), NT = P * TM, XB = xbuf<N1, N2>();
+    __shared__ Cx<T> X[P * XB];
+    __shared__ Cx<T> W[M];
+    if (skip && *skip)
+        return;
+    const int t = int(threadIdx.x), m = t % TM, p = t / TM;
+    const int rows = g.ny * g.nz;
+    const int ra = int(blockIdx.x) * 2 * P + 2 * p;  // rows ra, ra + 1
+    Cx<T> *Xp = X + p * XB;
+    for (int e = t; e < M; e += NT)
+        W[e] = twM[e];
+    Cx<T> a[N1];
+    if (m < N2) {
+        const size_t b0 = size_t(g.nx) * ra, b1 = b0 + g.nx;  // rows ra, ra + 1
+        sfor<0, N1>([&](auto c) {
+            constexpr int n1 = decltype(c)::value;
+            const int i = N2 * n1 + m;
+            T re = T(0), im = T(0);
+            if (i >= g.o0 && i < g.o0 + g.nx) {
+                if (ra < rows) re = in[b0 + (i - g.o0)];
+                if (ra + 1 < rows) im = in[b1 + (i - g.o0)];
+            }
+            a[n1] = Cx<T>{re, im};
+        });
+    }
+    __syncthreads();
+    if (m < N2) {
+        dft<T, N1, -1>(a);
+        sfor<0, N1>([&](auto c) {
+            constexpr int k1 = decltype(c)::value;
+            Xp[k1 * (N2 + 1) + m] = k1 == 0 ? a[0] : cmul(a[k1], W[m * k1]);
+        });
+    }
+    __syncthreads();
+    Cx<T> b[N2];
+    if (m < N1) {
+        sfor<0, N2>([&](auto c) {
+            constexpr int n2 = decltype(c)::value;
+            b[n2] = Xp[m * (N2 + 1) + n2];
+        });
+        dft<T, N2, -1>(b);
+    }
+    __syncthreads();
+    if (m < N1) {
+        sfor<0, N2>([&](auto c) {
+            constexpr int k2 = decltype(c)::value;
+            Xp[m + N1 * k2] = b[k2];  // natural order
+        });
+    }
+    __syncthreads();
+    if (ra >= rows)
+        return;
+    const int H = g.H;
+    const int ja = ra % g.ny + g.o1, ka = ra / g.ny + g.o2;
+    const int jb = (ra + 1) % g.ny + g.o1, kb = (ra + 1) / g.ny + g.o2;
+    Cx<T> *oa = data + size_t(H) * (size_t(ja) + size_t(g.M1) * ka);
+    Cx<T> *ob = data + size_t(H) * (size_t(jb) + size_t(g.M1) * kb);
+    for (int k = m; k < H; k += TM) {
+        const Cx<T> z = Xp[k];
+        const Cx<T> w = Xp[k == 0 ? 0 : M - k];  // conj taken below
+        oa[k] = Cx<T>{z.re + w.re, z.im - w.im};
+        if (ra + 1 < rows)
+            ob[k] = Cx<T>{z.im + w.im, w.re - z.re};
+    }
+}
+
+// Inverse: two rows' half spectra A, B (k < H) rebuilt into the pencil
+// Z(i) = A(i) + i B(i) (i < H) and conj A(M-i) + i conj B(M-i) above; the
+// inverse transform is a + i b; the rows are cropped into out.
+template <typename T, int N1, int N2, int P>
+__global__ __launch_bounds__(P * (N1 > N2 ? N1 : N2)) void rx_inv_reg(
+    const Cx<T> *data, T *out, RealGeom g, const Cx<T> *twM, const int *skip)
+{
+#pragma clang fp contract(fast)
+    constexpr int M = N1 * N2, TM = team<N1, N2>(), NT = P * TM, XB = xbuf<N1, N2>();
+    __shared__ Cx<T> X[P * XB];
+    __shared__ Cx<T> W[M];
+    if (skip && *skip)
+        return;
+    const int t = int(threadIdx.x), m = t % TM, p = t / TM;
+    const int rows = g.ny * g.nz;
+    const int ra = int(blockIdx.x) * 2 * P + 2 * p;
+    const int H = g.H;
+    Cx<T> *Xp = X + p * XB;
+    for (int e = t; e < M; e += NT)
+        W[e] = twM[e];
+    Cx<T> a[N1];
+    if (m < N2) {
+        const Cx<T> *sa = nullptr, *sb = nullptr;
+        if (ra < rows) {
+            const int j = ra % g.ny + g.o1, k = ra / g.ny + g.o2;
+            sa = data + size_t(H) * (size_t(j) + size_t(g.M1) * k);
+        }
+        if (ra + 1 < rows) {
+            const int j = (ra + 1) % g.ny + g.o1, k = (ra + 1) / g.ny + g.o2;
+            sb = data + size_t(H) * (size_t(j) + size_t(g.M1) * k);
+        }
+        sfor<0, N1>([&](auto c) {
+            constexpr int n1 = decltype(c)::value;
+            const int i = N2 * n1 + m;
+            const bool lo = i < H;
+            const int kk = lo ? i : M - i;
+            Cx<T> x{T(0), T(0)}, y{T(0), T(0)};
+            if (sa) x = sa[kk];
+            if (sb) y = sb[kk];
+            // lo: x + i y; above H: conj(x) + i conj(y)
+            a[n1] = lo ? Cx<T>{x.re - y.im, x.im + y.re} : Cx<T>{x.re + y.im, y.re - x.im};
+        });
+    }
+    __syncthreads();
+    if (m < N2) {
+        dft<T, N1, 1>(a);
+        sfor<0, N1>([&](auto c) {
+            constexpr int k1 = decltype(c)::value;
+            const Cx<T> w = W[m * k1];
+            Xp[k1 * (N2 + 1) + m] = k1 == 0 ? a[0] : cmul(a[k1], Cx<T>{w.re, -w.im});
+        });
+    }
+    __syncthreads();
+    if (m < N1 && ra < rows) {
+        Cx<T> b[N2];
+        sfor<0, N2>([&](auto c) {
+            constexpr int n2 = decltype(c)::value;
+            b[n2] = Xp[m * (N2 + 1) + n2];
+        });
+        dft<T, N2, 1>(b);
+        const size_t b0 = size_t(g.nx) * ra, b1 = b0 + g.nx;  // rows ra, ra + 1
+        sfor<0, N2>([&](auto c) {
+            constexpr int k2 = decltype(c)::value;
+            const int x = m + N1 * k2;
+            if (x >= g.o0 && x < g.o0 + g.nx) {
+                out[b0 + (x - g.o0)] = b[k2].re;
+                if (ra + 1 < rows)
+                    out[b1 + (x - g.o0)] = b[k2].im;
+            }
+        });
+    }
+}
+
+template <typename T, int N1, int N2>
+hipError_t launch_x_one(bool forward, Cx<T> *data, const T *in, T *out, const RealGeom &g,
+                        const Cx<T> *twM, const int *skip, hipStream_t st)
+{
+    constexpr int P = xpencils<T>(), NT = P * team<N1, N2>();
+    const int rows = g.ny * g.nz;
+    const int nblk = (rows + 2 * P - 1) / (2 * P);
+    if (nblk > 0) {
+        if (forward)
+            rx_fwd_reg<T, N1, N2, P><<<nblk, NT, 0, st>>>(data, in, g, twM, skip);
+        else
+            rx_inv_reg<T, N1, N2, P><<<nblk, NT, 0, st>>>(data, out, g, twM, skip);
+    }
+    return hipGetLastError();
+}
+
+// The x pass of length M in the register form; false when M has no split.
+template <typename T>
+bool launch_x(int M, bool forward, Cx<T> *data, const T *in, T *out, const RealGeom &g,
+              const Cx<T> *twM, const int *skip, hipStream_t st, hipError_t &err)
+{
+    switch (M) {
+    case 176: err = launch_x_one<T, 16, 11>(forward, data, in, out, g, twM, skip, st); return true;
+    case 192: err = launch_x_one<T, 16, 12>(forward, data, in, out, g, twM, skip, st); return true;
+    case 256: err = launch_x_one<T, 16, 16>(forward, data, in, out, g, twM, skip, st); return true;
+    case 320: err = launch_x_one<T, 20, 16>(forward, data, in, out, g, twM, skip, st); return true;
+    case 360: err = launch_x_one<T, 20, 18>(forward, data, in, out, g, twM, skip, st); return true;
+    case 384: err = launch_x_one<T, 24, 16>(forward, data, in, out, g, twM, skip, st); return true;
+    case 448: err = launch_x_one<T, 28, 16>(forward, data, in, out, g, twM, skip, st); return true;
+    case 512: err = launch_x_one<T, 32, 16>(forward, data, in, out, g, twM, skip, st); return true;
+    case 640: err = launch_x_one<T, 32, 20>(forward, data, in, out, g, twM, skip, st); return true;
+    default: return false;
+    }
+}
+
 // The identity op of passes without one.
 template <typename T>
 struct NoOp {

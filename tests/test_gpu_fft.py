@@ -88,6 +88,27 @@ def test_dipole_conv_register_pass_lengths(cuda, dt, L):
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < tol
 
 
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("L", [176, 192, 256, 320, 360, 384, 448, 512, 640])
+def test_dipole_conv_register_x_pass_lengths(cuda, dt, L):
+    """x lengths that run the register-resident x passes (two real rows per
+    pencil, the half-spectrum split and its rebuild): 7 x 5 = 35 rows, so
+    the last pencil carries one row, and an odd leading pad."""
+    rng = np.random.default_rng(L + 1)
+    n, p = (L, 0) if L == 176 else (L // 2 - 2, L // 4 + 1)
+    shape, pad = (n, 7, 5), (p, 0, 0)
+    M, _ = dipole.pad_size(shape, pad)
+    assert M[0] == L
+    chi = np.asfortranarray(rng.standard_normal(shape).astype(dt))
+    vsz, bdir = (0.9, 1.0, 1.1), (0.4, -0.1, 0.91)
+    got = dipole.dipole_conv(chi, vsz, bdir, pad)
+    up = fastfft.padfastfft(chi.astype(np.float64), pad)
+    D = fastfft.dipole_kernel(up.shape, vsz, bdir)
+    ref = np.real(np.fft.ifftn(D * np.fft.fftn(up)))[fastfft.unpad_slices(up.shape, shape, pad)]
+    tol = 1e-12 if dt == np.float64 else 1e-5
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < tol
+
+
 def test_dipole_conv_pad_sizes_match_padfastfft(cuda):
     for shape, pad in (((24, 22, 20), (12, 11, 10)), ((7, 9, 11), (3, 0, 5)), ((176, 256, 5), (88, 128, 0))):
         M, off = dipole.pad_size(shape, pad)

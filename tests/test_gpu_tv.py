@@ -156,11 +156,13 @@ def test_rts_device_form_on_side_stream(cuda, dt):
 
 
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
-def test_tv_register_pass_lengths(cuda, dt):
-    """tv.m on an unpadded 24x256x176 grid: y (256 = 16x16) and z (176 =
-    16x11) run the register-resident FFT passes; the oracle to the
-    tolerance with the same rho decisions (FP64)."""
-    shape, vsz, bdir = (24, 256, 176), (1.0, 1.0, 1.0), (0.1, -0.2, 0.97)
+@pytest.mark.parametrize("shape", [(24, 256, 176), (192, 20, 16)])
+def test_tv_register_pass_lengths(cuda, dt, shape):
+    """tv.m on unpadded grids whose lengths run the register-resident FFT
+    passes: y (256 = 16x16) and z (176 = 16x11), or x (192 = 16x12, the
+    real-row x passes); the oracle to the tolerance with the same rho
+    decisions (FP64)."""
+    vsz, bdir = (1.0, 1.0, 1.0), (0.1, -0.2, 0.97)
     f, m = _field(shape, vsz, bdir, seed=31)
     f = f.astype(dt)
     opts = {"tv": 1, "verbose": 0}
