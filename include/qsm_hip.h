@@ -96,7 +96,9 @@ size_t qsm_pd_footprint(int kind, int dtype, const size_t sz[3]);
 
 /* Loads the input (N values of dtype) and the mask (N bytes), zeroes the
  * iterates and scales f by sigma (pd_tgv_mex.c:280-283).  on_device != 0:
- * d2f and mask are device pointers (device-to-device copies on the stream). */
+ * d2f and mask are device pointers (device-to-device copies on the stream).
+ * Returns after the handle's stream has drained: the work list of active
+ * tiles (qsm_pd_set_active_items) is built from the loaded mask. */
 int qsm_pd_load(qsm_pd *s, const void *d2f, const uint8_t *mask,
                 int on_device);
 
