@@ -55,7 +55,18 @@ def make_input(cfg_name: str):
             np.asfortranarray(ph["field"]))
 
 
-def time_solver(kind, dtype, cfg, d2f, mask, steps, warmup):
+def time_solver(kind, dtype, cfg, d2f, mask, steps, warmup, settle=500):
+    """Times `steps` iterations after `warmup` untimed ones, with HIP events
+    on the solver's stream.  Before the warmup the same solve runs `settle`
+    (or settle + 1) untimed iterations: the GPU's clocks settle over the
+    first tens of milliseconds under load (launch times of the C3 pass vary
+    by +-8 % there), which a short timed region would otherwise sample
+    instead of the steady state a 1000-iteration solve runs at.  The count's
+    parity makes settle + warmup even, so the timed region starts right
+    after a pair of iterations and every convergence check falls on the
+    second iteration of a two-iteration launch: the region holds
+    two-iteration launches only (plus one single iteration at the end when
+    `steps` is odd)."""
     import torch
     from qsm_amd import pd
 
@@ -68,6 +79,8 @@ def time_solver(kind, dtype, cfg, d2f, mask, steps, warmup):
     b2 = autotune.best2(kind, dtype, shape)
     s.load(d2f.astype(dtype, order="F"), mask)
     items = s.active_items()
+    settled = settle + ((settle + warmup) & 1)
+    s.iterate(settled, 0.0)
     s.iterate(warmup, 0.0)
     stream.synchronize()
     torch.cuda.synchronize()
@@ -86,11 +99,12 @@ def time_solver(kind, dtype, cfg, d2f, mask, steps, warmup):
     it, stopped, _ = s.status()
     chi = s.chi()
     s.close()
-    assert it == warmup + steps and not stopped
+    assert it == settled + warmup + steps and not stopped
     launches = {k: c1[k] - c0[k] for k in c1}
     return {"gpu_s": gpu, "wall_s": wall, "chi": chi, "tiles": tiles, "launches": launches,
             "kz2": b2[0] if b2 else None, "schedule": b2[1] if b2 else 0,
-            "work_items": items[0], "work_item_kz": items[1]}
+            "work_items": items[0], "work_item_kz": items[1], "settle_iterations": settled,
+            "iterations": settled + warmup + steps}
 
 
 def stream_ceiling_gbs(n=256 * 256 * 176, narr=15):
@@ -837,13 +851,18 @@ def roofline(r, N, n_active, s):
     iteration, are inside it).  `traffic` is the HBM bytes one launch of that
     kernel moves, from the committed FETCH_SIZE/WRITE_SIZE summary."""
     l = r["launches"]
-    two = l["pass2"] > 0 and l["pass1"] == 0
+    two = l["pass2"] > 0
     per_it = n_active * 41 * s + N
     if two:
+        # iterations of the region in two-iteration launches; a trailing
+        # single iteration (odd step count) is priced at the same time per
+        # iteration
         its, nl, kern = 2, l["pass2"], "fused2_kernel"
+        share = 2 * l["pass2"] / (2 * l["pass2"] + l["pass1"])
     else:
         its, nl, kern = 1, max(1, l["pass1"] + l["sweeps"]), "fused_kernel"
-    t_launch = r["gpu_s"] / nl
+        share = 1.0
+    t_launch = r["gpu_s"] * share / nl
     alg = its * per_it
     ach = alg / t_launch / 1e9
     # the bytes the kernel itself cannot avoid: its 31-word read+write of the
@@ -950,6 +969,9 @@ def main(argv=None) -> int:
             "voxels": N,
             "n_active": n_active,
             "schedule": _schedule_text(r32),
+            "settle": ("before the warmup the same solve runs {} untimed iterations, so "
+                       "the timed region sees the GPU's steady clocks and starts on a "
+                       "two-iteration boundary").format(r32["settle_iterations"]),
         },
     }
     out["roofline"] = roofline(r32, N, n_active, 4)
@@ -974,7 +996,7 @@ def main(argv=None) -> int:
         out["chi_err_fp32_vs_fp64"] = {
             "rel_l2": float(np.linalg.norm(c32 - c64) / n64) if n64 > 0 else 0.0,
             "max_abs": float(np.max(np.abs(c32 - c64))),
-            "iterations": a.warmup + a.steps,
+            "iterations": r32["iterations"],
         }
     try:
         out["copy_peak_gbs"] = copy_peak_gbs()
