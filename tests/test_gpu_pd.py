@@ -3,8 +3,8 @@ src/inversion/mex/pd_tgv_mex.c and pd_tv_mex.c).
 
 FP64 and FP32 are compared BIT FOR BIT: the kernels keep the reference's
 association order and are built without FMA contraction, so any difference
-is a bug.  (The north-star bars, 1e-12 per operator and 1e-10 on chi, are
-asserted as well so a failure message shows both.)
+is a bug.  (The drop-in's tolerances, relative L2 1e-12 per operator and
+1e-10 on chi, are asserted as well so a failure message shows both.)
 """
 import ctypes as C
 
