@@ -88,18 +88,6 @@ hipError_t launch2_shape(const T *in, T *out, size_t N, const T *f, const uint8_
     return hipGetLastError();
 }
 
-// Blocks of the kernel one CU holds at once (LDS and registers).
-template <typename T, bool TGV, int BX, int BY>
-int resident2()
-{
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &n, reinterpret_cast<const void *>(&fused2_kernel<T, TGV, BX, BY, false>),
-            threads2<BX, BY>(), 0) != hipSuccess || n < 1)
-        n = 1;
-    return n;
-}
-
 // Tile (bx, by) of the two-iteration pass: [lo, hi) = the planes in
 // [1, nz - 1) where one of its owned voxels has a mask bit (hi = lo when
 // none).  One block per tile; each thread scans its columns from both ends.
@@ -177,15 +165,6 @@ hipError_t launch_fused2(const T *in, T *out, size_t N, const T *f, const uint8_
     else
         return launch2_shape<T, TGV, QSM_F2_F64_BX, QSM_F2_F64_BY>(
             in, out, N, f, mk, g, kz, dc, pc, stop, partials, items, nitems, st);
-}
-
-int fused2_resident(int kind, int dtype)
-{
-    if (dtype == QSM_F32)
-        return kind == QSM_TGV ? resident2<float, true, QSM_F2_F32_BX, QSM_F2_F32_BY>()
-                               : resident2<float, false, QSM_F2_F32_BX, QSM_F2_F32_BY>();
-    return kind == QSM_TGV ? resident2<double, true, QSM_F2_F64_BX, QSM_F2_F64_BY>()
-                           : resident2<double, false, QSM_F2_F64_BX, QSM_F2_F64_BY>();
 }
 
 hipError_t launch_tile_zrange(const uint8_t *mask, Geom g, int dtype, int *out, hipStream_t st)

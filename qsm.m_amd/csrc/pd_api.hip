@@ -150,12 +150,13 @@ namespace {
 // start at different planes: 50 % more HBM reads, a third of the gain).
 // Items are ordered by (chunk, tile row, tile column), so consecutive items
 // -- one XCD's blocks -- are patches of neighbours.  An item of len planes
-// costs about len + 6 plane steps (the four-stage pipeline's fill) and
-// `slots` items run at a time (CUs x resident blocks); the grid (kz >= 8
-// or the interior, offset o) taken is the one with the fewest rounds, then
-// the shortest longest item (offsets sampled at kz/16 steps), and the list is kept when its rounds x (len +
-// 6) beat the uniform grid's by 10 % (items_on == 2: whenever a voxel is
-// active).  Synchronous (the item count sizes the launches).
+// costs about len + 6 plane steps (the four-stage pipeline's fill) and one
+// item per CU runs at a time; the grid (kz >= 8 or the interior, offset o,
+// offsets sampled at kz/16 steps) taken is the one with the fewest rounds,
+// then the shortest longest item.  The list is built only where it skips
+// at least a tenth of the tiles' plane steps, and kept when its rounds x
+// (len + 6) beat the uniform grid's by 10 % (items_on == 2: whenever a
+// voxel is active).  Synchronous (the item count sizes the launches).
 int build_items(qsm_pd *s)
 {
     s->nitems = 0;
@@ -171,7 +172,16 @@ int build_items(qsm_pd *s)
     std::vector<int> r(2 * size_t(nt));
     QSM_HIP(hipMemcpyAsync(r.data(), dr, r.size() * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     QSM_HIP(hipStreamSynchronize(s->stream));
-    const long slots = long(cu_count()) * fused2_resident(s->kind, s->dtype);
+    // A list only pays where it skips planes: with (nearly) every plane of
+    // every tile active the uniform grid's tuned chunking is kept.
+    long act_planes = 0;
+    for (int t = 0; t < nt; ++t)
+        act_planes += std::max(0, r[2 * t + 1] - r[2 * t]);
+    if (s->items_on == 1 && act_planes * 10 >= long(nt) * (g.nz - 2) * 9)
+        return QSM_OK;
+    // rounds of one block per CU: a second resident block (TV's smaller LDS
+    // allows three) shares the CU's issue slots rather than adding to them
+    const long slots = long(cu_count());
     const long nb = fused2_blocks(g, s->dtype, s->kz2);
     const long cost_u = (nb + slots - 1) / slots * (s->kz2 + 6);
     long act = 0;

@@ -99,8 +99,6 @@ hipError_t launch_fused2(const T *in, T *out, size_t N, const T *f,
                          hipStream_t st);
 void fused2_tile(int dtype, int &tx, int &ty);
 int fused2_blocks(Geom g, int dtype, int kz);
-// Blocks of the two-iteration kernel one CU holds at once.
-int fused2_resident(int kind, int dtype);
 // Per tile of fused2_tile(dtype), row-major: out[2t], out[2t+1] = the plane
 // range [lo, hi) in [1, nz - 1) holding a voxel with a mask bit (lo == hi:
 // none).
