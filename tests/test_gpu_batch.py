@@ -143,3 +143,35 @@ def test_two_iteration_batch_uneven_chunks(cuda, dt):
             r = b.result(s)
             assert r.iters == it, (case, r.iters, it)
             np.testing.assert_array_equal(r.chi, ref, err_msg=str(case))
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_two_iteration_batch_masked_and_empty_volumes(cuda, dt):
+    """The two-iteration march's entries trimmed to each tile's active
+    planes: a volume masked to two thin slabs and one with an empty mask
+    (nothing to trim to: its uniform entries stay) batched together; the
+    slab volume bitwise its solo solve and the oracle, the empty one all
+    zeros after its full budget, as alone."""
+    shape, vsz = (60, 40, 34), (1.0, 1.0, 1.0)
+    rng = np.random.default_rng(17)
+    d2f = np.asfortranarray((0.01 * rng.standard_normal(shape)).astype(dt))
+    slab = np.zeros(shape, np.uint8)
+    slab[4:40, 3:30, 2:9] = 1
+    slab[20:58, 10:38, 24:32] = 1
+    slab = np.asfortranarray(slab)
+    empty = np.zeros(shape, np.uint8, order="F")
+    iters = 24
+    with Batch(dt, tiles=(0, 0), max_volumes=4) as b:
+        s1 = b.add(_lib.TGV, d2f, slab, vsz, 1e-3, 2.0, iters, 0.0, kz=16)
+        s2 = b.add(_lib.TV, d2f, empty, vsz, 1e-3, 0.0, iters, 0.0, kz=16)
+        while b.state(s1)[1] or b.state(s2)[1]:
+            b.step(10)
+            b.sync()
+        r1, r2 = b.result(s1), b.result(s2)
+    assert r1.iters == iters and r2.iters == iters
+    with Solver(_lib.TGV, dt, shape, vsz, 1e-3, 2.0) as s:
+        s.load(d2f, slab).iterate(iters)
+        solo = s.chi()
+    np.testing.assert_array_equal(r1.chi, solo)
+    np.testing.assert_array_equal(r1.chi, orc.pd_tgv(d2f, slab, vsz, 1e-3, 2.0, 0.0, iters)[0])
+    assert np.all(r2.chi == 0)

@@ -401,3 +401,33 @@ def test_active_items_cost_model_and_edge_masks(cuda):
         got = s.chi()
     ref = orc.pd_tgv(d2f, blob, h, 1e-3, 2.0, 0.0, 21)[0]
     np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+def test_active_items_at_the_volume_ends(cuda, dt):
+    """Masks active only near z = 1 and z = nz - 2 (the items' first and
+    last planes clip to the interior), a tile active on one plane only, and
+    an x/y edge tile: the work list bitwise the uniform grid and the
+    oracle."""
+    shape, h = (70, 40, 30), (1.0, 1.0, 1.0)
+    d2f = _phantom_input(shape, h, 33, dt)
+    m = np.zeros(shape, np.uint8)
+    m[5:30, 4:20, 0:6] = 1          # bottom slab (z = 0 is outside the interior)
+    m[40:69, 25:40, 26:30] = 1      # top slab, touching x and y edges
+    m[50, 10, 15] = 1               # one voxel: its tile is active on one plane
+    m = np.asfortranarray(m)
+
+    def solve(on):
+        with pd.Solver(_lib.TGV, dt, shape, h, 1e-3, 2.0) as s:
+            s.set_active_items(on)
+            s.load(d2f, m)
+            n = s.active_items()[0]
+            s.iterate(23)
+            return s.chi(), n
+
+    got, n = solve(2)
+    assert n > 0
+    ref_u, n0 = solve(0)
+    assert n0 == 0
+    np.testing.assert_array_equal(got, ref_u)
+    np.testing.assert_array_equal(got, orc.pd_tgv(d2f, m, h, 1e-3, 2.0, 0.0, 23)[0])
