@@ -565,7 +565,8 @@ def admm_leg(cfg, field, mask, iters=20, cpu_iters=2):
     hbm_bytes per ADMM iteration: right-hand side 7 N s (v, l read with -1
     neighbours, r written), the half-spectrum solve 2 N s + 9 S (S = H ny nz
     c, the spectrum; passes: x 1+1, y 2, z 3 with F, y 2, x 1+1), the v/l
-    update 16 N s (u, v, l read; v, l, dv written), the dual norms 6 N s.
+    update with both residuals' sums 13 N s (u, v, l read; the other v, l
+    set written; the -1 neighbours' terms recomputed, not stored).
     rts adds its step 1 once (4 LSMR iterations and two full complex 3D
     FFTs).  cpu: the oracle (numpy restatement, one thread for its FFTs) on
     cpu_iters ADMM iterations of the same input, setup included."""
@@ -603,7 +604,7 @@ def admm_leg(cfg, field, mask, iters=20, cpu_iters=2):
             t = e0.elapsed_time(e1) / 1e3
             H = shape[0] // 2 + 1
             S = H * shape[1] * shape[2] * 2 * s
-            hbm = (7 + 2 + 16 + 6) * N * s + 9 * S
+            hbm = (7 + 2 + 13) * N * s + 9 * S
             res[name] = {"ms_total": t * 1e3, "ms_per_iter": t / it * 1e3,
                          "voxel_iters_per_s": N * it / t, "hbm_bytes_per_iter": hbm,
                          "hbm_gbs_per_iter": hbm / (t / it) / 1e9}

@@ -14,13 +14,14 @@
 //
 // Device layout: every real field is a column-major volume of the solver
 // grid; the spectra F and the work spectrum hold the half spectrum along x
-// ((nx/2 + 1) ny nz complex).  Per iteration four launches touch HBM: the
+// ((nx/2 + 1) ny nz complex).  Per iteration three launches touch HBM: the
 // right-hand side (gradAdj of rho v - l, 6 reads + 1 write per voxel), the
 // half-spectrum solve (five FFT passes, the k-space multipliers generated
-// in-kernel from per-axis tables), the v/l update with its residual partial
-// sums (7 reads + 9 writes), and the dual residual norms (6 reads).  Sums
-// are accumulated in double per block and reduced in a fixed order, so
-// they are reproducible.
+// in-kernel from per-axis tables), and the v/l update with the partial sums
+// of both residuals (7 reads + 6 writes; (v, l) kept in two sets so that
+// the dual residual's -1 neighbour terms are recomputed from the previous
+// set instead of stored).  Sums are accumulated in double per block and
+// reduced in a fixed order, so they are reproducible.
 //
 // The loop runs on the device: the stopping rule and the penalty update
 // (tv.m:279-306) are taken by a one-block kernel after each iteration's
@@ -203,83 +204,108 @@ __device__ __forceinline__ T sgn_of(T x)
     return x > T(0) ? T(1) : (x < T(0) ? T(-1) : x);  // sign(): 0 -> 0, NaN -> NaN
 }
 
-// tv.m:205-260: the v-subproblem, the multiplier update and the primal
-// residual sums.  Writes v (new), l (new), dv = v - v_old; partial sums of
-// |grad u - v| (nr), |grad u| (na), |v| (nb) per block.
+// tv.m:205-252 at one voxel (i, j, k) = l: the gradient d of u (gradfp
+// with 1/h), the over-relaxed d1, the shrunk v (new), the multiplier l
+// (new) and dv = v_new - v_old, from u and the previous v, l.
+template <typename T>
+struct Upd {
+    T d[3], n[3], m[3], dv[3];
+};
+
 template <typename T, bool ISO>
-__global__ __launch_bounds__(NT) void shrink_kernel(Vec3<T> v, Vec3<T> lm, Vec3<T> dv,
-                                                    const T *u, G3 g, T hx, T hy, T hz,
-                                                    T alpha, T oma, const Ctl *ctl, double *part)
+__device__ __forceinline__ Upd<T> update_at(const T *u, CVec3<T> v, CVec3<T> lm, G3 g, int i,
+                                            int j, int k, int l, T hx, T hy, T hz, T alpha,
+                                            T oma, T rho, T irho)
+{
+    Upd<T> r;
+    int px, py, pz;
+    plus_nbrs(g, i, j, k, l, px, py, pz);
+    const T c = u[l];
+    r.d[0] = hx * (u[px] - c);
+    r.d[1] = hy * (u[py] - c);
+    r.d[2] = hz * (u[pz] - c);
+    const T vo[3] = {v.x[l], v.y[l], v.z[l]};
+    const T lo[3] = {lm.x[l], lm.y[l], lm.z[l]};
+    // over-relaxation, then the shrinkage argument
+    T d1[3], q[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        d1[a] = alpha * r.d[a] + oma * vo[a];
+        q[a] = d1[a] + irho * lo[a];
+    }
+    if constexpr (!ISO) {
+        // max(abs(r)-irho, 0) .* sign(r)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            r.n[a] = fmax(fabs(q[a]) - irho, T(0)) * sgn_of(q[a]);
+    } else {
+        // s = (s - irho) .* (s > irho) ./ s, non-finite -> 0
+        const T sn = sqrt((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]);
+        T f = ((sn - irho) * T(sn > irho ? 1 : 0)) / sn;
+        if (!isfinite(f)) f = T(0);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            r.n[a] = q[a] * f;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        r.m[a] = lo[a] + rho * (d1[a] - r.n[a]);
+        r.dv[a] = r.n[a] - vo[a];
+    }
+    return r;
+}
+
+// One ADMM update after the u-subproblem: the v-subproblem and the
+// multipliers (tv.m:205-256) and both residuals' partial sums (tv.m:
+// 258-277).  Reads u and the previous (v, l), writes the new (vn, ln) to
+// separate buffers: the dual terms need dv and the new l at the periodic -1
+// neighbours, which each thread recomputes from the unchanged previous
+// values (the same operations on the same inputs, so the same bits as the
+// neighbour's own update), so dv is never stored.  Partial sums per block:
+// |grad u - v| (nr), |grad u| (na), |v| (nb), |rho gradAdj(dv)|^2 (dual
+// residual) and |rho gradAdj(l)|^2 (its scale); gradAdj with the -1/h
+// coefficients (ax, ay, az) as gradfp_adj_mex forms it.
+template <typename T, bool ISO>
+__global__ __launch_bounds__(NT) void update_kernel(CVec3<T> v, CVec3<T> lm, Vec3<T> vn,
+                                                    Vec3<T> ln, const T *u, G3 g, T hx, T hy,
+                                                    T hz, T ax, T ay, T az, T alpha, T oma,
+                                                    const Ctl *ctl, double *part)
 {
     if (ctl->stop) return;
     const T rho = T(ctl->rho), irho = T(ctl->irho);
     int i, j, k, l;
-    double s[3] = {0.0, 0.0, 0.0};
-    if (vox(g, i, j, k, l)) {
-        int px, py, pz;
-        plus_nbrs(g, i, j, k, l, px, py, pz);
-        const T c = u[l];
-        const T dx = hx * (u[px] - c), dy = hy * (u[py] - c), dz = hz * (u[pz] - c);
-        const T v1 = v.x[l], v2 = v.y[l], v3 = v.z[l];
-        const T l1 = lm.x[l], l2 = lm.y[l], l3 = lm.z[l];
-        // over-relaxation
-        const T dx1 = alpha * dx + oma * v1;
-        const T dy1 = alpha * dy + oma * v2;
-        const T dz1 = alpha * dz + oma * v3;
-        const T r1 = dx1 + irho * l1, r2 = dy1 + irho * l2, r3 = dz1 + irho * l3;
-        T n1, n2, n3;
-        if constexpr (!ISO) {
-            // max(abs(r)-irho, 0) .* sign(r)
-            n1 = fmax(fabs(r1) - irho, T(0)) * sgn_of(r1);
-            n2 = fmax(fabs(r2) - irho, T(0)) * sgn_of(r2);
-            n3 = fmax(fabs(r3) - irho, T(0)) * sgn_of(r3);
-        } else {
-            // s = (s - irho) .* (s > irho) ./ s, non-finite -> 0
-            const T sn = sqrt((r1 * r1 + r2 * r2) + r3 * r3);
-            T f = ((sn - irho) * T(sn > irho ? 1 : 0)) / sn;
-            if (!isfinite(f)) f = T(0);
-            n1 = r1 * f;
-            n2 = r2 * f;
-            n3 = r3 * f;
-        }
-        v.x[l] = n1;
-        v.y[l] = n2;
-        v.z[l] = n3;
-        lm.x[l] = l1 + rho * (dx1 - n1);
-        lm.y[l] = l2 + rho * (dy1 - n2);
-        lm.z[l] = l3 + rho * (dz1 - n3);
-        dv.x[l] = n1 - v1;
-        dv.y[l] = n2 - v2;
-        dv.z[l] = n3 - v3;
-        const T e1 = dx - n1, e2 = dy - n2, e3 = dz - n3;
-        s[0] = double(sqrt((e1 * e1 + e2 * e2) + e3 * e3));
-        s[1] = double(sqrt((dx * dx + dy * dy) + dz * dz));
-        s[2] = double(sqrt((n1 * n1 + n2 * n2) + n3 * n3));
-    }
-    block_sum<3>(s, part + NPART * flat_block());
-}
-
-// tv.m:262-277: squared norms of rho gradAdj(dv) (dual residual) and of
-// rho gradAdj(l) (the dual tolerance's scale).
-template <typename T>
-__global__ __launch_bounds__(NT) void dual_kernel(CVec3<T> dv, CVec3<T> lm, G3 g, const Ctl *ctl,
-                                                  T hx, T hy, T hz, double *part)
-{
-    if (ctl->stop) return;
-    const T rho = T(ctl->rho);
-    int i, j, k, l;
-    double s[2] = {0.0, 0.0};
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (vox(g, i, j, k, l)) {
         int mx, my, mz;
         minus_nbrs(g, i, j, k, l, mx, my, mz);
-        const T a = rho * adj_sum(hx * (dv.x[l] - dv.x[mx]), hy * (dv.y[l] - dv.y[my]),
-                                  hz * (dv.z[l] - dv.z[mz]), j, k);
-        const T b = rho * adj_sum(hx * (lm.x[l] - lm.x[mx]), hy * (lm.y[l] - lm.y[my]),
-                                  hz * (lm.z[l] - lm.z[mz]), j, k);
-        s[0] = double(a) * double(a);
-        s[1] = double(b) * double(b);
+        const int im = i > 0 ? i - 1 : g.nx - 1, jm = j > 0 ? j - 1 : g.ny - 1,
+                  km = k > 0 ? k - 1 : g.nz - 1;
+        const Upd<T> c = update_at<T, ISO>(u, v, lm, g, i, j, k, l, hx, hy, hz, alpha, oma, rho,
+                                           irho);
+        const Upd<T> X = update_at<T, ISO>(u, v, lm, g, im, j, k, mx, hx, hy, hz, alpha, oma,
+                                           rho, irho);
+        const Upd<T> Y = update_at<T, ISO>(u, v, lm, g, i, jm, k, my, hx, hy, hz, alpha, oma,
+                                           rho, irho);
+        const Upd<T> Z = update_at<T, ISO>(u, v, lm, g, i, j, km, mz, hx, hy, hz, alpha, oma,
+                                           rho, irho);
+        vn.x[l] = c.n[0];
+        vn.y[l] = c.n[1];
+        vn.z[l] = c.n[2];
+        ln.x[l] = c.m[0];
+        ln.y[l] = c.m[1];
+        ln.z[l] = c.m[2];
+        const T e1 = c.d[0] - c.n[0], e2 = c.d[1] - c.n[1], e3 = c.d[2] - c.n[2];
+        s[0] = double(sqrt((e1 * e1 + e2 * e2) + e3 * e3));
+        s[1] = double(sqrt((c.d[0] * c.d[0] + c.d[1] * c.d[1]) + c.d[2] * c.d[2]));
+        s[2] = double(sqrt((c.n[0] * c.n[0] + c.n[1] * c.n[1]) + c.n[2] * c.n[2]));
+        const T a = rho * adj_sum(ax * (c.dv[0] - X.dv[0]), ay * (c.dv[1] - Y.dv[1]),
+                                  az * (c.dv[2] - Z.dv[2]), j, k);
+        const T b = rho * adj_sum(ax * (c.m[0] - X.m[0]), ay * (c.m[1] - Y.m[1]),
+                                  az * (c.m[2] - Z.m[2]), j, k);
+        s[3] = double(a) * double(a);
+        s[4] = double(b) * double(b);
     }
-    block_sum<2>(s, part + NPART * flat_block() + 3);
+    block_sum<5>(s, part + NPART * flat_block());
 }
 
 // First reduction level: block b of RB sums the NPART partials of a
@@ -415,8 +441,10 @@ struct KeepOp {
 // -laplaceKernel (laplaceKernel.m:14-24) padded and circularly shifted to
 // the origin (tv.m:163-165), with L(0) = 0 exactly.  The k coordinates and
 // the per-axis terms of L come from tables (tab: k and L along x for H
-// entries, then y for M1, then z for M2); the rest is formed in double and
-// rounded once to T.
+// entries, then y for M1, then z for M2).  The weights are formed in the
+// class the reference forms iA in: double, except for tv.m with a single
+// f, where K and iA are single; then rounded once to T with the transform
+// scale.
 template <typename T, bool RTS>
 struct SolveOp {
     const Cx<T> *F;    // 2 fft3(source) on the half spectrum
@@ -441,20 +469,49 @@ struct SolveOp {
         a2 = iA;
     }
 
+    // tv.m in single (tv.m:89, 159-168): D formed in double and cast to
+    // single, K = mu |D|^2 and iA = 1 ./ (K + rho L) in single (rho L
+    // formed in double, then cast), data factor mu conj(D) in single.
+    __device__ void weights_single(float D, double L, double rho, float &a1, float &a2) const
+    {
+        const float K = float(mu) * (D * D);
+        float iA = 1.0f / (K + float(rho * L));
+        if (!isfinite(iA)) iA = 0.0f;
+        a1 = iA * (float(mu) * D);
+        a2 = iA;
+    }
+
     __device__ Cx<T> operator()(int x, int y, int z, Cx<T> fr) const
     {
         const double *ty = tab + 2 * H, *tz = ty + 2 * M1;
         const double X = tab[x], Y = ty[y], Z = tz[z];
         const double L = (tab[H + x] + ty[M1 + y]) + tz[M2 + z];
         const double rho = ctl->rho;
-        double a1, a2;
-        weights(dip(X, Y, Z, b0, b1, b2), L, rho, a1, a2);
         const bool nx = 2 * x == M0, ny = 2 * y == M1, nz = 2 * z == M2;
-        if (nx || ny || nz) {
-            double m1, m2;
-            weights(dip(nx ? X : -X, ny ? Y : -Y, nz ? Z : -Z, b0, b1, b2), L, rho, m1, m2);
-            a1 = 0.5 * (a1 + m1);
-            a2 = 0.5 * (a2 + m2);
+        double a1, a2;
+        if constexpr (!RTS && sizeof(T) == 4) {
+            // rts.m's M = mu (|D| > delta) is double, so only tv.m's
+            // weights follow the single class
+            float s1, s2;
+            weights_single(float(dip(X, Y, Z, b0, b1, b2)), L, rho, s1, s2);
+            if (nx || ny || nz) {
+                float m1, m2;
+                weights_single(float(dip(nx ? X : -X, ny ? Y : -Y, nz ? Z : -Z, b0, b1, b2)), L,
+                               rho, m1, m2);
+                s1 = 0.5f * (s1 + m1);
+                s2 = 0.5f * (s2 + m2);
+            }
+            a1 = s1;
+            a2 = s2;
+        } else {
+            weights(dip(X, Y, Z, b0, b1, b2), L, rho, a1, a2);
+            if (nx || ny || nz) {
+                double m1, m2;
+                weights(dip(nx ? X : -X, ny ? Y : -Y, nz ? Z : -Z, b0, b1, b2), L, rho, m1,
+                        m2);
+                a1 = 0.5 * (a1 + m1);
+                a2 = 0.5 * (a2 + m2);
+            }
         }
         const T c1 = T(a1 * scale), c2 = T(a2 * scale);
         const Cx<T> ff = F[size_t(x) + size_t(H) * (size_t(y) + size_t(M1) * z)];
@@ -482,7 +539,7 @@ template <typename T>
 struct Work {
     Cx<T> *F, *W;
     T *r, *u;
-    Vec3<T> v, l, dv;
+    Vec3<T> v, l, v2, l2;  // (v, l) and the second set the update writes
     double *part, *p1, *tab, *trace;
     Ctl *ctl;
     // rts step 1 (LSMR on the full spectrum): u, v, h, hbar, x and D
@@ -504,10 +561,11 @@ size_t work_layout(const size_t sz[3], Work<T> *w, char *base, bool lsmr = false
     size_t off = 0;
     auto take = [&](size_t n) { char *p = base ? base + off : nullptr; off += n; return p; };
     char *pF = take(spec), *pW = take(spec), *pr = take(vol), *pu = take(vol);
-    char *pv[3], *pl[3], *pd[3];
+    char *pv[3], *pl[3], *pv2[3], *pl2[3];
     for (auto &p : pv) p = take(vol);
     for (auto &p : pl) p = take(vol);
-    for (auto &p : pd) p = take(vol);
+    for (auto &p : pv2) p = take(vol);
+    for (auto &p : pl2) p = take(vol);
     char *pp = take(part), *p1 = take(align256(RB * NPART * sizeof(double)));
     char *ptab = take(align256(2 * (sz[0] / 2 + 1 + sz[1] + sz[2]) * sizeof(double)));
     char *ptr = take(align256(size_t(MAX_ITERS) * 5 * sizeof(double)));
@@ -530,7 +588,8 @@ size_t work_layout(const size_t sz[3], Work<T> *w, char *base, bool lsmr = false
         w->u = reinterpret_cast<T *>(pu);
         w->v = {reinterpret_cast<T *>(pv[0]), reinterpret_cast<T *>(pv[1]), reinterpret_cast<T *>(pv[2])};
         w->l = {reinterpret_cast<T *>(pl[0]), reinterpret_cast<T *>(pl[1]), reinterpret_cast<T *>(pl[2])};
-        w->dv = {reinterpret_cast<T *>(pd[0]), reinterpret_cast<T *>(pd[1]), reinterpret_cast<T *>(pd[2])};
+        w->v2 = {reinterpret_cast<T *>(pv2[0]), reinterpret_cast<T *>(pv2[1]), reinterpret_cast<T *>(pv2[2])};
+        w->l2 = {reinterpret_cast<T *>(pl2[0]), reinterpret_cast<T *>(pl2[1]), reinterpret_cast<T *>(pl2[2])};
         w->part = reinterpret_cast<double *>(pp);
         w->p1 = reinterpret_cast<double *>(p1);
         w->tab = reinterpret_cast<double *>(ptab);
@@ -620,26 +679,29 @@ int admm_run(const T *src, const uint8_t *mask, const size_t sz[3], const double
     start_kernel<<<1, blk, 0, st>>>(w.p1, w.ctl, o.rho);
     QSM_HIP(hipGetLastError());
 
-    const CVec3<T> cv{w.v.x, w.v.y, w.v.z}, cl{w.l.x, w.l.y, w.l.z}, cd{w.dv.x, w.dv.y, w.dv.z};
+    // (v, l) alternate between the two sets: the update reads one, writes
+    // the other.  Once the rule stops the loop every later launch returns
+    // at once, so which set holds the last (v, l) is immaterial.
+    Vec3<T> vs[2] = {w.v, w.v2}, ls[2] = {w.l, w.l2};
     const int *stop = &w.ctl->stop;
     for (int it = 1; it <= o.maxit; ++it) {
+        const int p = (it - 1) & 1;
+        const CVec3<T> cv{vs[p].x, vs[p].y, vs[p].z}, cl{ls[p].x, ls[p].y, ls[p].z};
         // u-subproblem (tv.m:196-203)
         rhs_kernel<T><<<grid, blk, 0, st>>>(w.r, cv, cl, g, w.ctl, ax, ay, az);
         QSM_HIP(hipGetLastError());
         if ((rc = real_conv<T, true>(w.r, w.u, sz, M, off, w.W, op, st, stop))) return rc;
-        // v-subproblem, multipliers, primal sums (tv.m:205-256)
+        // v-subproblem, multipliers, primal and dual sums (tv.m:205-277)
         if (o.tv == 1)
-            shrink_kernel<T, false><<<grid, blk, 0, st>>>(w.v, w.l, w.dv, w.u, g, gx, gy, gz,
-                                                         T(o.alpha), T(1.0 - o.alpha), w.ctl,
-                                                         w.part);
+            update_kernel<T, false><<<grid, blk, 0, st>>>(cv, cl, vs[p ^ 1], ls[p ^ 1], w.u, g,
+                                                         gx, gy, gz, ax, ay, az, T(o.alpha),
+                                                         T(1.0 - o.alpha), w.ctl, w.part);
         else
-            shrink_kernel<T, true><<<grid, blk, 0, st>>>(w.v, w.l, w.dv, w.u, g, gx, gy, gz,
-                                                        T(o.alpha), T(1.0 - o.alpha), w.ctl,
-                                                        w.part);
+            update_kernel<T, true><<<grid, blk, 0, st>>>(cv, cl, vs[p ^ 1], ls[p ^ 1], w.u, g,
+                                                        gx, gy, gz, ax, ay, az, T(o.alpha),
+                                                        T(1.0 - o.alpha), w.ctl, w.part);
         QSM_HIP(hipGetLastError());
-        // dual norms (tv.m:262-277), then the rule (tv.m:279-306)
-        dual_kernel<T><<<grid, blk, 0, st>>>(cd, cl, g, w.ctl, ax, ay, az, w.part);
-        QSM_HIP(hipGetLastError());
+        // the rule (tv.m:279-306)
         reduce1_kernel<<<RB, blk, 0, st>>>(w.part, nblk, w.p1, w.ctl);
         QSM_HIP(hipGetLastError());
         decide_kernel<<<1, blk, 0, st>>>(w.p1, w.ctl, rule, w.trace, it);
