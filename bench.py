@@ -565,8 +565,11 @@ def admm_leg(cfg, field, mask, iters=20, cpu_iters=2):
     hbm_bytes per ADMM iteration: right-hand side 7 N s (v, l read with -1
     neighbours, r written), the half-spectrum solve 2 N s + 9 S (S = H ny nz
     c, the spectrum; passes: x 1+1, y 2, z 3 with F, y 2, x 1+1), the v/l
-    update with both residuals' sums 13 N s (u, v, l read; the other v, l
-    set written; the -1 neighbours' terms recomputed, not stored).
+    update with both residuals' sums and the next right-hand side at the
+    same rho 14 N s (u, v, l read; the other v, l set and r written; the -1
+    neighbours' terms recomputed, not stored).  The right-hand side runs
+    (7 N s) only in the first iteration and after those whose rule changed
+    rho, counted from the returned trace.
     rts adds its step 1 once (4 LSMR iterations and two full complex 3D
     FFTs).  cpu: the oracle (numpy restatement, one thread for its FFTs) on
     cpu_iters ADMM iterations of the same input, setup included."""
@@ -597,17 +600,30 @@ def admm_leg(cfg, field, mask, iters=20, cpu_iters=2):
             st.synchronize()
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
+            tr = np.zeros(5 * iters, dtype=np.float64)
             e0.record(st)
-            it = run(f, m, vsz, bdir, opts, x, work=work, stream=st)
+            it = run(f, m, vsz, bdir, opts, x, work=work, stream=st, trace=tr)
             e1.record(st)
             e1.synchronize()
             t = e0.elapsed_time(e1) / 1e3
             H = shape[0] // 2 + 1
             S = H * shape[1] * shape[2] * 2 * s
-            hbm = (7 + 2 + 13) * N * s + 9 * S
+            rho = np.concatenate(([mod.make_opts(opts).rho], tr[4::5][:it]))
+            n_rhs = 1 + int(np.count_nonzero(rho[1:it] != rho[:it - 1]))
+            hbm = ((2 + 14) * N * s + 9 * S) * it + 7 * N * s * n_rhs
             res[name] = {"ms_total": t * 1e3, "ms_per_iter": t / it * 1e3,
-                         "voxel_iters_per_s": N * it / t, "hbm_bytes_per_iter": hbm,
-                         "hbm_gbs_per_iter": hbm / (t / it) / 1e9}
+                         "voxel_iters_per_s": N * it / t, "rhs_launches_run": n_rhs,
+                         "hbm_bytes_per_iter": hbm / it,
+                         "hbm_gbs_per_iter": hbm / t / 1e9}
+            # the defaults' tolerances: the rule stops the loop early and the
+            # remaining maxit - iterations launches return at once
+            e0.record(st)
+            itd = run(f, m, vsz, bdir, {"verbose": 0}, x, work=work, stream=st)
+            e1.record(st)
+            e1.synchronize()
+            res[name]["default_opts"] = {"ms_total": e0.elapsed_time(e1),
+                                         "iterations": itd,
+                                         "maxit": mod.make_opts(None).maxit}
             del f, m, x, work
         torch.cuda.empty_cache()
         if cpu_iters > 0:
@@ -1020,8 +1036,13 @@ def main(argv=None) -> int:
     if a.admm_iters > 0:
         out["admm"] = admm_leg(cfg, field, mask, a.admm_iters, 2 if a.cpu_iters > 0 else 0)
         for k in ("tv", "rts"):
-            log(f"[bench] {k}.m (ADMM) f32 {out['admm'][k]['f32']['ms_total']:.2f} ms, "
-                f"f64 {out['admm'][k]['f64']['ms_total']:.2f} ms for {a.admm_iters} iterations")
+            r32, r64 = out['admm'][k]['f32'], out['admm'][k]['f64']
+            log(f"[bench] {k}.m (ADMM) f32 {r32['ms_total']:.2f} ms, "
+                f"f64 {r64['ms_total']:.2f} ms for {a.admm_iters} iterations; default "
+                f"tolerances f32 {r32['default_opts']['ms_total']:.2f} ms "
+                f"({r32['default_opts']['iterations']} of {r32['default_opts']['maxit']}), "
+                f"f64 {r64['default_opts']['ms_total']:.2f} ms "
+                f"({r64['default_opts']['iterations']} of {r64['default_opts']['maxit']})")
     if a.bgremove:
         out["bgremove"] = bgremove_leg(cfg, field, mask, cpu=a.cpu_iters > 0)
         log(f"[bench] sharp f32 {out['bgremove']['sharp']['f32']['device_ms']:.2f} ms on device, "

@@ -14,14 +14,17 @@
 //
 // Device layout: every real field is a column-major volume of the solver
 // grid; the spectra F and the work spectrum hold the half spectrum along x
-// ((nx/2 + 1) ny nz complex).  Per iteration three launches touch HBM: the
-// right-hand side (gradAdj of rho v - l, 6 reads + 1 write per voxel), the
-// half-spectrum solve (five FFT passes, the k-space multipliers generated
-// in-kernel from per-axis tables), and the v/l update with the partial sums
-// of both residuals (7 reads + 6 writes; (v, l) kept in two sets so that
-// the dual residual's -1 neighbour terms are recomputed from the previous
-// set instead of stored).  Sums are accumulated in double per block and
-// reduced in a fixed order, so they are reproducible.
+// ((nx/2 + 1) ny nz complex).  Per iteration up to three launches touch
+// HBM: the right-hand side (gradAdj of rho v - l, 6 reads + 1 write per
+// voxel), the half-spectrum solve (five FFT passes, the k-space multipliers
+// generated in-kernel from per-axis tables), and the v/l update with the
+// partial sums of both residuals (7 reads + 7 writes; (v, l) kept in two
+// sets so that the dual residual's -1 neighbour terms are recomputed from
+// the previous set instead of stored).  The update also writes the next
+// right-hand side at the current rho; the right-hand-side launch does work
+// only when the rule has changed rho (and in the first iteration).  Sums
+// are accumulated in double per block and reduced in a fixed order, so
+// they are reproducible.
 //
 // The loop runs on the device: the stopping rule and the penalty update
 // (tv.m:279-306) are taken by a one-block kernel after each iteration's
@@ -60,7 +63,7 @@ struct Ctl {
     int flag;          // penalty-update counter (tv.m:287-306)
     int stop;          // set when the rule stops: later launches return
     int iters;         // iterations done
-    int pad;
+    int rkeep;         // rho unchanged: r already holds the next right-hand side
 };
 
 // Constants of the stopping rule and the penalty update.
@@ -92,6 +95,20 @@ int vox_blocks(G3 g)
 __device__ __forceinline__ int flat_block()
 {
     return int(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+}
+
+// Tile t of the voxel grid (BX x BY voxels of one plane; t = bx + gx (by +
+// gy k), the block index vox_grid would give it): the thread's voxel,
+// false outside the volume.
+__device__ __forceinline__ bool tile_vox(G3 g, int t, int &i, int &j, int &k, int &l)
+{
+    const int gx = (g.nx + BX - 1) / BX, gy = (g.ny + BY - 1) / BY;
+    const int r = t / gx;
+    i = (t - r * gx) * BX + threadIdx.x;
+    j = (r % gy) * BY + threadIdx.y;
+    k = r / gy;
+    l = i + g.nx * (j + g.ny * k);
+    return i < g.nx && j < g.ny;
 }
 
 // Sum of v[q] over the block's NT threads into out[q] for q < NQ, in a
@@ -182,20 +199,24 @@ __global__ __launch_bounds__(NT) void init_kernel(Vec3<T> v, Vec3<T> lm, const T
 
 // tv.m:196-200: r = gradAdj(rho v - l) (gradfp_adj_mex with -1/h), the
 // differences rho*v - l formed per sample as the reference forms r1..r3.
+// Returns at once when the previous update already wrote r at this rho.
 template <typename T>
 __global__ __launch_bounds__(NT) void rhs_kernel(T *r, CVec3<T> v, CVec3<T> lm, G3 g,
-                                                 const Ctl *ctl, T hx, T hy, T hz)
+                                                 const Ctl *ctl, T hx, T hy, T hz,
+                                                 int ntiles)
 {
-    if (ctl->stop) return;
+    if (ctl->stop || ctl->rkeep) return;
     const T rho = T(ctl->rho);
-    int i, j, k, l;
-    if (!vox(g, i, j, k, l)) return;
-    int mx, my, mz;
-    minus_nbrs(g, i, j, k, l, mx, my, mz);
-    const T X = hx * ((rho * v.x[l] - lm.x[l]) - (rho * v.x[mx] - lm.x[mx]));
-    const T Y = hy * ((rho * v.y[l] - lm.y[l]) - (rho * v.y[my] - lm.y[my]));
-    const T Z = hz * ((rho * v.z[l] - lm.z[l]) - (rho * v.z[mz] - lm.z[mz]));
-    r[l] = adj_sum(X, Y, Z, j, k);
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        int i, j, k, l;
+        if (!tile_vox(g, t, i, j, k, l)) continue;
+        int mx, my, mz;
+        minus_nbrs(g, i, j, k, l, mx, my, mz);
+        const T X = hx * ((rho * v.x[l] - lm.x[l]) - (rho * v.x[mx] - lm.x[mx]));
+        const T Y = hy * ((rho * v.y[l] - lm.y[l]) - (rho * v.y[my] - lm.y[my]));
+        const T Z = hz * ((rho * v.z[l] - lm.z[l]) - (rho * v.z[mz] - lm.z[mz]));
+        r[l] = adj_sum(X, Y, Z, j, k);
+    }
 }
 
 template <typename T>
@@ -261,51 +282,60 @@ __device__ __forceinline__ Upd<T> update_at(const T *u, CVec3<T> v, CVec3<T> lm,
 // separate buffers: the dual terms need dv and the new l at the periodic -1
 // neighbours, which each thread recomputes from the unchanged previous
 // values (the same operations on the same inputs, so the same bits as the
-// neighbour's own update), so dv is never stored.  Partial sums per block:
+// neighbour's own update), so dv is never stored.  Partial sums per tile:
 // |grad u - v| (nr), |grad u| (na), |v| (nb), |rho gradAdj(dv)|^2 (dual
 // residual) and |rho gradAdj(l)|^2 (its scale); gradAdj with the -1/h
-// coefficients (ax, ay, az) as gradfp_adj_mex forms it.
+// coefficients (ax, ay, az) as gradfp_adj_mex forms it.  Also writes r =
+// gradAdj(rho v - l) of the new (v, l) at this rho, bitwise what
+// rhs_kernel would form, for the next iteration when the rule keeps rho.
 template <typename T, bool ISO>
 __global__ __launch_bounds__(NT) void update_kernel(CVec3<T> v, CVec3<T> lm, Vec3<T> vn,
-                                                    Vec3<T> ln, const T *u, G3 g, T hx, T hy,
-                                                    T hz, T ax, T ay, T az, T alpha, T oma,
-                                                    const Ctl *ctl, double *part)
+                                                    Vec3<T> ln, T *r, const T *u, G3 g, T hx,
+                                                    T hy, T hz, T ax, T ay, T az, T alpha,
+                                                    T oma, const Ctl *ctl, double *part,
+                                                    int ntiles)
 {
     if (ctl->stop) return;
     const T rho = T(ctl->rho), irho = T(ctl->irho);
-    int i, j, k, l;
-    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (vox(g, i, j, k, l)) {
-        int mx, my, mz;
-        minus_nbrs(g, i, j, k, l, mx, my, mz);
-        const int im = i > 0 ? i - 1 : g.nx - 1, jm = j > 0 ? j - 1 : g.ny - 1,
-                  km = k > 0 ? k - 1 : g.nz - 1;
-        const Upd<T> c = update_at<T, ISO>(u, v, lm, g, i, j, k, l, hx, hy, hz, alpha, oma, rho,
-                                           irho);
-        const Upd<T> X = update_at<T, ISO>(u, v, lm, g, im, j, k, mx, hx, hy, hz, alpha, oma,
-                                           rho, irho);
-        const Upd<T> Y = update_at<T, ISO>(u, v, lm, g, i, jm, k, my, hx, hy, hz, alpha, oma,
-                                           rho, irho);
-        const Upd<T> Z = update_at<T, ISO>(u, v, lm, g, i, j, km, mz, hx, hy, hz, alpha, oma,
-                                           rho, irho);
-        vn.x[l] = c.n[0];
-        vn.y[l] = c.n[1];
-        vn.z[l] = c.n[2];
-        ln.x[l] = c.m[0];
-        ln.y[l] = c.m[1];
-        ln.z[l] = c.m[2];
-        const T e1 = c.d[0] - c.n[0], e2 = c.d[1] - c.n[1], e3 = c.d[2] - c.n[2];
-        s[0] = double(sqrt((e1 * e1 + e2 * e2) + e3 * e3));
-        s[1] = double(sqrt((c.d[0] * c.d[0] + c.d[1] * c.d[1]) + c.d[2] * c.d[2]));
-        s[2] = double(sqrt((c.n[0] * c.n[0] + c.n[1] * c.n[1]) + c.n[2] * c.n[2]));
-        const T a = rho * adj_sum(ax * (c.dv[0] - X.dv[0]), ay * (c.dv[1] - Y.dv[1]),
-                                  az * (c.dv[2] - Z.dv[2]), j, k);
-        const T b = rho * adj_sum(ax * (c.m[0] - X.m[0]), ay * (c.m[1] - Y.m[1]),
-                                  az * (c.m[2] - Z.m[2]), j, k);
-        s[3] = double(a) * double(a);
-        s[4] = double(b) * double(b);
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        int i, j, k, l;
+        double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (tile_vox(g, t, i, j, k, l)) {
+            int mx, my, mz;
+            minus_nbrs(g, i, j, k, l, mx, my, mz);
+            const int im = i > 0 ? i - 1 : g.nx - 1, jm = j > 0 ? j - 1 : g.ny - 1,
+                      km = k > 0 ? k - 1 : g.nz - 1;
+            const Upd<T> c = update_at<T, ISO>(u, v, lm, g, i, j, k, l, hx, hy, hz, alpha,
+                                               oma, rho, irho);
+            const Upd<T> X = update_at<T, ISO>(u, v, lm, g, im, j, k, mx, hx, hy, hz, alpha,
+                                               oma, rho, irho);
+            const Upd<T> Y = update_at<T, ISO>(u, v, lm, g, i, jm, k, my, hx, hy, hz, alpha,
+                                               oma, rho, irho);
+            const Upd<T> Z = update_at<T, ISO>(u, v, lm, g, i, j, km, mz, hx, hy, hz, alpha,
+                                               oma, rho, irho);
+            vn.x[l] = c.n[0];
+            vn.y[l] = c.n[1];
+            vn.z[l] = c.n[2];
+            ln.x[l] = c.m[0];
+            ln.y[l] = c.m[1];
+            ln.z[l] = c.m[2];
+            const T e1 = c.d[0] - c.n[0], e2 = c.d[1] - c.n[1], e3 = c.d[2] - c.n[2];
+            s[0] = double(sqrt((e1 * e1 + e2 * e2) + e3 * e3));
+            s[1] = double(sqrt((c.d[0] * c.d[0] + c.d[1] * c.d[1]) + c.d[2] * c.d[2]));
+            s[2] = double(sqrt((c.n[0] * c.n[0] + c.n[1] * c.n[1]) + c.n[2] * c.n[2]));
+            const T a = rho * adj_sum(ax * (c.dv[0] - X.dv[0]), ay * (c.dv[1] - Y.dv[1]),
+                                      az * (c.dv[2] - Z.dv[2]), j, k);
+            const T b = rho * adj_sum(ax * (c.m[0] - X.m[0]), ay * (c.m[1] - Y.m[1]),
+                                      az * (c.m[2] - Z.m[2]), j, k);
+            s[3] = double(a) * double(a);
+            s[4] = double(b) * double(b);
+            // the next right-hand side at this rho, as rhs_kernel forms it
+            r[l] = adj_sum(ax * ((rho * c.n[0] - c.m[0]) - (rho * X.n[0] - X.m[0])),
+                           ay * ((rho * c.n[1] - c.m[1]) - (rho * Y.n[1] - Y.m[1])),
+                           az * ((rho * c.n[2] - c.m[2]) - (rho * Z.n[2] - Z.m[2])), j, k);
+        }
+        block_sum<5>(s, part + NPART * t);
     }
-    block_sum<5>(s, part + NPART * flat_block());
 }
 
 // First reduction level: block b of RB sums the NPART partials of a
@@ -360,6 +390,7 @@ __global__ __launch_bounds__(NT) void start_kernel(const double *p1, Ctl *ctl, d
         ctl->flag = 0;
         ctl->stop = 0;
         ctl->iters = 0;
+        ctl->rkeep = 0;
     }
 }
 
@@ -398,6 +429,7 @@ __global__ __launch_bounds__(NT) void decide_kernel(const double *p1, Ctl *ctl, 
     row[3] = ed;
     row[4] = rho;
     ctl->nr = nr;
+    ctl->rkeep = rho == ctl->rho ? 1 : 0;
     ctl->rho = rho;
     ctl->irho = 1.0 / rho;
     ctl->flag = flag;
@@ -549,6 +581,12 @@ struct Work {
 
 size_t align256(size_t n) { return (n + 255) / 256 * 256; }
 
+// Grid of a tile-walking kernel: each block walks TPB tiles, gridDim.x
+// apart, so that a launch that returns at once (rule stopped, r current)
+// dispatches TPB times fewer blocks.
+constexpr int TPB = 4;
+int tile_grid(int ntiles) { return (ntiles + TPB - 1) / TPB; }
+
 template <typename T>
 size_t work_layout(const size_t sz[3], Work<T> *w, char *base, bool lsmr = false)
 {
@@ -586,10 +624,14 @@ size_t work_layout(const size_t sz[3], Work<T> *w, char *base, bool lsmr = false
         w->W = reinterpret_cast<Cx<T> *>(pW);
         w->r = reinterpret_cast<T *>(pr);
         w->u = reinterpret_cast<T *>(pu);
-        w->v = {reinterpret_cast<T *>(pv[0]), reinterpret_cast<T *>(pv[1]), reinterpret_cast<T *>(pv[2])};
-        w->l = {reinterpret_cast<T *>(pl[0]), reinterpret_cast<T *>(pl[1]), reinterpret_cast<T *>(pl[2])};
-        w->v2 = {reinterpret_cast<T *>(pv2[0]), reinterpret_cast<T *>(pv2[1]), reinterpret_cast<T *>(pv2[2])};
-        w->l2 = {reinterpret_cast<T *>(pl2[0]), reinterpret_cast<T *>(pl2[1]), reinterpret_cast<T *>(pl2[2])};
+        auto vec = [](char *const *q) {
+            return Vec3<T>{reinterpret_cast<T *>(q[0]), reinterpret_cast<T *>(q[1]),
+                           reinterpret_cast<T *>(q[2])};
+        };
+        w->v = vec(pv);
+        w->l = vec(pl);
+        w->v2 = vec(pv2);
+        w->l2 = vec(pl2);
         w->part = reinterpret_cast<double *>(pp);
         w->p1 = reinterpret_cast<double *>(p1);
         w->tab = reinterpret_cast<double *>(ptab);
@@ -679,6 +721,8 @@ int admm_run(const T *src, const uint8_t *mask, const size_t sz[3], const double
     start_kernel<<<1, blk, 0, st>>>(w.p1, w.ctl, o.rho);
     QSM_HIP(hipGetLastError());
 
+    const int rgrid = tile_grid(nblk), ugrid = rgrid;
+
     // (v, l) alternate between the two sets: the update reads one, writes
     // the other.  Once the rule stops the loop every later launch returns
     // at once, so which set holds the last (v, l) is immaterial.
@@ -688,18 +732,20 @@ int admm_run(const T *src, const uint8_t *mask, const size_t sz[3], const double
         const int p = (it - 1) & 1;
         const CVec3<T> cv{vs[p].x, vs[p].y, vs[p].z}, cl{ls[p].x, ls[p].y, ls[p].z};
         // u-subproblem (tv.m:196-203)
-        rhs_kernel<T><<<grid, blk, 0, st>>>(w.r, cv, cl, g, w.ctl, ax, ay, az);
+        rhs_kernel<T><<<rgrid, blk, 0, st>>>(w.r, cv, cl, g, w.ctl, ax, ay, az, nblk);
         QSM_HIP(hipGetLastError());
         if ((rc = real_conv<T, true>(w.r, w.u, sz, M, off, w.W, op, st, stop))) return rc;
         // v-subproblem, multipliers, primal and dual sums (tv.m:205-277)
         if (o.tv == 1)
-            update_kernel<T, false><<<grid, blk, 0, st>>>(cv, cl, vs[p ^ 1], ls[p ^ 1], w.u, g,
-                                                         gx, gy, gz, ax, ay, az, T(o.alpha),
-                                                         T(1.0 - o.alpha), w.ctl, w.part);
+            update_kernel<T, false><<<ugrid, blk, 0, st>>>(cv, cl, vs[p ^ 1], ls[p ^ 1], w.r,
+                                                          w.u, g, gx, gy, gz, ax, ay, az,
+                                                          T(o.alpha), T(1.0 - o.alpha), w.ctl,
+                                                          w.part, nblk);
         else
-            update_kernel<T, true><<<grid, blk, 0, st>>>(cv, cl, vs[p ^ 1], ls[p ^ 1], w.u, g,
-                                                        gx, gy, gz, ax, ay, az, T(o.alpha),
-                                                        T(1.0 - o.alpha), w.ctl, w.part);
+            update_kernel<T, true><<<ugrid, blk, 0, st>>>(cv, cl, vs[p ^ 1], ls[p ^ 1], w.r,
+                                                         w.u, g, gx, gy, gz, ax, ay, az,
+                                                         T(o.alpha), T(1.0 - o.alpha), w.ctl,
+                                                         w.part, nblk);
         QSM_HIP(hipGetLastError());
         // the rule (tv.m:279-306)
         reduce1_kernel<<<RB, blk, 0, st>>>(w.part, nblk, w.p1, w.ctl);
