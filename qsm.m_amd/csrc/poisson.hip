@@ -7,9 +7,10 @@
 // around the volume (mgpcg_mex.c: mx_pad_boundary).  PCG (mgpcg_mex.c:
 // mx_pcg) stops when ||r|| <= tol ||f|| or after maxit iterations; its
 // preconditioner is mgopts.maxit V-cycles (mx_mg, mx_cycle): pre-smoothing
-// on G and on the boundary band B (2^l nboundary sweeps on level l), the
-// residual, full-weighting restriction onto the coarse mask, the coarse
-// cycle, trilinear correction, the mirrored post-smoothing; on the
+// on G and on the boundary band B (2^l nboundary sweeps on level l, from
+// per-colour index lists of the band), the residual, full-weighting
+// restriction onto the coarse mask, the coarse cycle, trilinear
+// correction, the mirrored post-smoothing; on the
 // coarsest level the reference sweeps to a relative residual of 1e-8.
 // The grid hierarchy (the voxel size doubles per level), the masks (a coarse
 // voxel is active iff its 8 fine children are: coarsen_grid_mex.c), the
@@ -178,6 +179,99 @@ __global__ __launch_bounds__(NT) void band_kernel(uint8_t *B, const uint8_t *b, 
     B[l] = v;
 }
 
+// ---- the band as index lists ---------------------------------------------
+//
+// The band is a thin shell, but a sweep over its mask visits the whole
+// grid.  Each level's band voxels are listed once per solve, per colour, in
+// increasing index order: a count per row (one wave per row of the grid),
+// an exclusive scan over the rows, a fill per row.  A band sweep then takes
+// one lane per listed voxel.  Voxels of one colour do not read each other,
+// so the order within a colour does not change the result.
+
+// One wave per row (j, k): the row's band voxels of colour 0 and 1, as
+// cnt[2 row + c].
+__global__ __launch_bounds__(NT) void band_count_kernel(int *cnt, const uint8_t *B, Dim d)
+{
+    const int lane = int(threadIdx.x) & 63;
+    const int row = int(blockIdx.x) * (NT / 64) + int(threadIdx.x) / 64;
+    if (row >= d.ny * d.nz) return;  // whole waves
+    const int j = row % d.ny, k = row / d.ny;
+    const uint8_t *b = B + size_t(d.nx) * row;
+    int c0 = 0, c1 = 0;
+    for (int x0 = 0; x0 < d.nx; x0 += 64) {
+        const int i = x0 + lane;
+        const bool on = i < d.nx && b[i];
+        const bool odd = ((i + j + k) & 1) != 0;
+        c0 += __popcll(__ballot(on && !odd));
+        c1 += __popcll(__ballot(on && odd));
+    }
+    if (lane == 0) {
+        cnt[2 * row] = c0;
+        cnt[2 * row + 1] = c1;
+    }
+}
+
+// One block: cnt[2 row + c] becomes the exclusive prefix over rows of
+// colour c; tot[0..1] the totals.
+__global__ __launch_bounds__(1024) void band_scan_kernel(int *cnt, int rows, int *tot)
+{
+    __shared__ int part[2][1024];
+    const int t = int(threadIdx.x), per = (rows + 1023) / 1024;
+    const int r0 = min(rows, t * per), r1 = min(rows, r0 + per);
+    int s0 = 0, s1 = 0;
+    for (int r = r0; r < r1; ++r) {
+        s0 += cnt[2 * r];
+        s1 += cnt[2 * r + 1];
+    }
+    part[0][t] = s0;
+    part[1][t] = s1;
+    __syncthreads();
+    // inclusive scan of the 1024 thread sums (Hillis-Steele)
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int a0 = t >= o ? part[0][t - o] : 0, a1 = t >= o ? part[1][t - o] : 0;
+        __syncthreads();
+        part[0][t] += a0;
+        part[1][t] += a1;
+        __syncthreads();
+    }
+    int e0 = part[0][t] - s0, e1 = part[1][t] - s1;
+    for (int r = r0; r < r1; ++r) {
+        const int n0 = cnt[2 * r], n1 = cnt[2 * r + 1];
+        cnt[2 * r] = e0;
+        cnt[2 * r + 1] = e1;
+        e0 += n0;
+        e1 += n1;
+    }
+    if (t == 1023) {
+        tot[0] = part[0][1023];
+        tot[1] = part[1][1023];
+    }
+}
+
+// One wave per row: the row's band voxels of colour c go to list[c] from
+// the row's offset, in increasing i.
+__global__ __launch_bounds__(NT) void band_fill_kernel(int *list0, int *list1, const int *off,
+                                                       const uint8_t *B, Dim d)
+{
+    const int lane = int(threadIdx.x) & 63;
+    const int row = int(blockIdx.x) * (NT / 64) + int(threadIdx.x) / 64;
+    if (row >= d.ny * d.nz) return;  // whole waves
+    const int j = row % d.ny, k = row / d.ny;
+    const size_t base = size_t(d.nx) * row;
+    int o0 = off[2 * row], o1 = off[2 * row + 1];
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int x0 = 0; x0 < d.nx; x0 += 64) {
+        const int i = x0 + lane;
+        const bool on = i < d.nx && B[base + i];
+        const bool odd = ((i + j + k) & 1) != 0;
+        const unsigned long long m0 = __ballot(on && !odd), m1 = __ballot(on && odd);
+        if (on && !odd) list0[o0 + __popcll(m0 & below)] = int(base + i);
+        if (on && odd) list1[o1 + __popcll(m1 & below)] = int(base + i);
+        o0 += __popcll(m0);
+        o1 += __popcll(m1);
+    }
+}
+
 // ---- smoothing, residual, transfers --------------------------------------
 
 // One colour of a red-black Gauss-Seidel sweep (gauss_seidel_mex.c):
@@ -197,6 +291,21 @@ __global__ __launch_bounds__(NT) void gs_kernel(T *v, const T *f, const uint8_t 
     if (i >= d.nx || !interior(d, i, j, k)) return;
     const size_t l = size_t(i) + size_t(d.nx) * (size_t(j) + size_t(d.ny) * k);
     if (!M[l]) return;
+    const size_t sy = size_t(d.nx), sz = size_t(d.nx) * d.ny;
+    const T hh = T(1.0 / (2.0 * (c.hx + c.hy + c.hz)));
+    v[l] = hh * (f[l] + ((T(c.hx) * (v[l - 1] + v[l + 1]) + T(c.hy) * (v[l - sy] + v[l + sy])) +
+                         T(c.hz) * (v[l - sz] + v[l + sz])));
+}
+
+// gs_kernel on the n voxels of one colour listed in idx (interior voxels of
+// the band): the same update, one lane per listed voxel.
+template <typename T>
+__global__ __launch_bounds__(NT) void gs_list_kernel(T *v, const T *f, const int *idx, int n,
+                                                     Dim d, Lap c)
+{
+    const int t = int(blockIdx.x) * NT + int(threadIdx.x);
+    if (t >= n) return;
+    const size_t l = size_t(idx[t]);
     const size_t sy = size_t(d.nx), sz = size_t(d.nx) * d.ny;
     const T hh = T(1.0 / (2.0 * (c.hx + c.hy + c.hz)));
     v[l] = hh * (f[l] + ((T(c.hx) * (v[l - 1] + v[l + 1]) + T(c.hy) * (v[l - sy] + v[l + sy])) +
@@ -427,6 +536,8 @@ struct Level {
     Lap c;
     T *v, *f, *r;
     uint8_t *G, *B;
+    int *bl;    // band voxels of colour 0 from bl, of colour 1 from bl + (n + 1) / 2
+    int bn[2];  // their counts (host copy)
 };
 
 template <typename T>
@@ -436,6 +547,7 @@ struct Solver {
     T *x, *p;           // PCG iterate and direction (level-0 size)
     T *cp, *cq;         // coarsest-level CG scratch
     uint8_t *tmp;       // band seed scratch (level-0 size)
+    int *rowcnt, *btot; // band list construction: per-row counts, totals per level
     double *part, *sum; // reductions
 };
 
@@ -461,12 +573,18 @@ size_t solver_layout(const size_t sz[3], const double vsz[3], int L, Solver<T> *
         const size_t n = d[i].n();
         char *pv = take(n * sizeof(T)), *pf = take(n * sizeof(T)), *pr = take(n * sizeof(T));
         char *pG = take(n), *pB = take(n);
-        if (s) s->lv[i] = Level<T>{d[i], lap_at(vsz, i), reinterpret_cast<T *>(pv), reinterpret_cast<T *>(pf),
-                                   reinterpret_cast<T *>(pr), reinterpret_cast<uint8_t *>(pG),
-                                   reinterpret_cast<uint8_t *>(pB)};
+        // band lists on the levels that smooth (not the coarsest)
+        char *pl = i < L - 1 ? take((n + 1) * sizeof(int)) : nullptr;
+        if (s)
+            s->lv[i] = Level<T>{d[i], lap_at(vsz, i), reinterpret_cast<T *>(pv),
+                                reinterpret_cast<T *>(pf), reinterpret_cast<T *>(pr),
+                                reinterpret_cast<uint8_t *>(pG), reinterpret_cast<uint8_t *>(pB),
+                                reinterpret_cast<int *>(pl), {0, 0}};
     }
     const size_t n0 = d[0].n();
     char *px = take(n0 * sizeof(T)), *pp = take(n0 * sizeof(T)), *pt = take(n0);
+    char *prc = take(2 * size_t(d[0].ny) * d[0].nz * sizeof(int));
+    char *pbt = take(2 * MAX_LEVELS * sizeof(int));
     const size_t nc = d[L - 1].n();
     char *pcp = take(nc * sizeof(T)), *pcq = take(nc * sizeof(T));
     char *ppart = take(DOT_BLOCKS * sizeof(double)), *psum = take(sizeof(double));
@@ -477,6 +595,8 @@ size_t solver_layout(const size_t sz[3], const double vsz[3], int L, Solver<T> *
         s->cp = reinterpret_cast<T *>(pcp);
         s->cq = reinterpret_cast<T *>(pcq);
         s->tmp = reinterpret_cast<uint8_t *>(pt);
+        s->rowcnt = reinterpret_cast<int *>(prc);
+        s->btot = reinterpret_cast<int *>(pbt);
         s->part = reinterpret_cast<double *>(ppart);
         s->sum = reinterpret_cast<double *>(psum);
     }
@@ -506,6 +626,24 @@ int smooth(Level<T> &L, const uint8_t *M, int n, bool reverse, hipStream_t st)
     for (int s = 0; s < n; ++s)
         for (int h = 0; h < 2; ++h) {
             gs_kernel<T><<<g, b, 0, st>>>(L.v, L.f, M, L.d, L.c, first ^ h);
+            QSM_HIP(hipGetLastError());
+        }
+    return QSM_OK;
+}
+
+// n red-black sweeps on the level's band, from its index lists; the colour
+// order of smooth().
+template <typename T>
+int smooth_band(Level<T> &L, int n, bool reverse, hipStream_t st)
+{
+    const int first = reverse ? ((L.d.nx + 1) & 1) : 0;
+    const int *list[2] = {L.bl, L.bl + (L.d.n() + 1) / 2};
+    for (int s = 0; s < n; ++s)
+        for (int h = 0; h < 2; ++h) {
+            const int c = first ^ h;
+            if (L.bn[c] == 0) continue;
+            gs_list_kernel<T><<<(L.bn[c] + NT - 1) / NT, NT, 0, st>>>(L.v, L.f, list[c], L.bn[c],
+                                                                     L.d, L.c);
             QSM_HIP(hipGetLastError());
         }
     return QSM_OK;
@@ -572,7 +710,7 @@ int cycle(Solver<T> &s, int l, const MgParams &m, hipStream_t st)
     if (l < s.L - 1) {
         const int bs = (1 << l) * m.nboundary;
         if ((rc = smooth(L, L.G, m.npre, false, st))) return rc;
-        if ((rc = smooth(L, L.B, bs, false, st))) return rc;
+        if ((rc = smooth_band(L, bs, false, st))) return rc;
         residual_kernel<T><<<grid3(L.d), b, 0, st>>>(L.r, L.f, L.v, L.G, L.d, L.c);
         QSM_HIP(hipGetLastError());
         Level<T> &C = s.lv[l + 1];
@@ -584,7 +722,7 @@ int cycle(Solver<T> &s, int l, const MgParams &m, hipStream_t st)
         }
         correct_kernel<T><<<grid3(C.d), b, 0, st>>>(L.v, C.v, L.G, L.d, C.d);
         QSM_HIP(hipGetLastError());
-        if ((rc = smooth(L, L.B, bs, true, st))) return rc;
+        if ((rc = smooth_band(L, bs, true, st))) return rc;
         if ((rc = smooth(L, L.G, m.npost, true, st))) return rc;
     } else {
         if ((rc = coarse_solve(s, L, st))) return rc;
@@ -622,6 +760,25 @@ int mgpcg_t(const T *src, bool unwrap, const uint8_t *mask, const size_t sz[3],
         band_seed_kernel<<<grid3(L.d), b, 0, st>>>(s.tmp, L.G, L.d);
         band_kernel<<<grid3(L.d), b, 0, st>>>(L.B, s.tmp, L.G, L.d);
         QSM_HIP(hipGetLastError());
+        if (l < s.L - 1) {
+            // the band's index lists per colour
+            const int rows = L.d.ny * L.d.nz, rb = (rows + NT / 64 - 1) / (NT / 64);
+            band_count_kernel<<<rb, NT, 0, st>>>(s.rowcnt, L.B, L.d);
+            band_scan_kernel<<<1, 1024, 0, st>>>(s.rowcnt, rows, s.btot + 2 * l);
+            band_fill_kernel<<<rb, NT, 0, st>>>(L.bl, L.bl + (L.d.n() + 1) / 2, s.rowcnt, L.B,
+                                                L.d);
+            QSM_HIP(hipGetLastError());
+        }
+    }
+    if (s.L > 1) {
+        int tot[2 * MAX_LEVELS];
+        QSM_HIP(hipMemcpyAsync(tot, s.btot, 2 * (s.L - 1) * sizeof(int), hipMemcpyDeviceToHost,
+                               st));
+        QSM_HIP(hipStreamSynchronize(st));
+        for (int l = 0; l < s.L - 1; ++l) {
+            s.lv[l].bn[0] = tot[2 * l];
+            s.lv[l].bn[1] = tot[2 * l + 1];
+        }
     }
     // PCG (mgpcg_mex.c: mx_pcg): reltol from ||f||, then r = f - A x0
     zero_kernel<T><<<g1(n0), NT, 0, st>>>(s.p, n0);

@@ -128,12 +128,15 @@ def test_mgpcg_argument_checks_without_gpu(kw, msg):
 
 
 def test_mgpcg_work_bytes_cover_the_hierarchy():
-    """Work space: per level v, f, r and two mask bytes; PCG x, p; coarse
-    CG scratch; band seed; reduction partials."""
+    """Work space: per level v, f, r and two mask bytes, and on the levels
+    that smooth (all but the coarsest) the band's index lists (n + 1 ints);
+    PCG x, p; coarse CG scratch; band seed; the lists' per-row counts
+    (2 ints per level-0 row); reduction partials."""
     n = [(34, 30, 26), (18, 16, 14), (10, 9, 8)]      # 32x28x24 padded, 3 levels
     vox = [a * b * c for a, b, c in n]
     w = _lib.lib().qsm_mgpcg_work_bytes(_lib.F64, (C.c_size_t * 3)(32, 28, 24), 3)
-    lo = sum(3 * 8 * v + 2 * v for v in vox) + 2 * 8 * vox[0] + 2 * 8 * vox[-1] + vox[0]
+    lo = (sum(3 * 8 * v + 2 * v for v in vox) + sum(4 * (v + 1) for v in vox[:-1])
+          + 2 * 8 * vox[0] + 2 * 8 * vox[-1] + vox[0] + 2 * 4 * n[0][1] * n[0][2])
     assert lo <= w <= lo + 64 * 1024
 
 
