@@ -69,4 +69,22 @@ __device__ __forceinline__ int xcd_block()
     return (xcd < rem) ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
 }
 
+// xcd_block for a 3-D grid: the (x, y, z) block this block runs as, each
+// XCD taking one contiguous range of the grid's blocks in x-fastest order.
+// For a grid of (x, y) tiles by planes that is a slab of planes per XCD,
+// so a plane's -z and +z neighbours are read through the same L2.
+__device__ __forceinline__ void xcd_block3(int &bx, int &by, int &bz)
+{
+    const int gx = int(gridDim.x), gy = int(gridDim.y);
+    const int nblk = gx * gy * int(gridDim.z);
+    const int b = int(blockIdx.x) + gx * (int(blockIdx.y) + gy * int(blockIdx.z));
+    const int per = nblk / 8, rem = nblk % 8;
+    const int xcd = b % 8, idx = b / 8;
+    const int lb =
+        (xcd < rem) ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+    bx = lb % gx;
+    by = (lb / gx) % gy;
+    bz = lb / (gx * gy);
+}
+
 }  // namespace qsm

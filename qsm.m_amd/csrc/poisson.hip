@@ -283,9 +283,11 @@ template <typename T>
 __global__ __launch_bounds__(NT) void gs_kernel(T *v, const T *f, const uint8_t *M, Dim d, Lap c,
                                                 int colour)
 {
-    const int t = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    const int k = blockIdx.z;
+    int bx, by, bz;
+    xcd_block3(bx, by, bz);  // a slab of planes per XCD: the z neighbours meet in its L2
+    const int t = bx * BX + threadIdx.x;
+    const int j = by * BY + threadIdx.y;
+    const int k = bz;
     if (j >= d.ny) return;
     const int i = 2 * t + ((colour + j + k) & 1);
     if (i >= d.nx || !interior(d, i, j, k)) return;
