@@ -60,9 +60,11 @@ dim3 grid_half(Dim d) { return dim3(((d.nx + 1) / 2 + BX - 1) / BX, (d.ny + BY -
 
 __device__ __forceinline__ bool vox(Dim d, int &i, int &j, int &k, size_t &l)
 {
-    i = blockIdx.x * BX + threadIdx.x;
-    j = blockIdx.y * BY + threadIdx.y;
-    k = blockIdx.z;
+    int bx, by, bz;
+    xcd_block3(bx, by, bz);  // a slab of planes per XCD
+    i = bx * BX + threadIdx.x;
+    j = by * BY + threadIdx.y;
+    k = bz;
     l = size_t(i) + size_t(d.nx) * (size_t(j) + size_t(d.ny) * k);
     return i < d.nx && j < d.ny;
 }
