@@ -349,84 +349,77 @@ int sorted_radii(const double *rs, size_t nrs, std::vector<double> &out)
     return QSM_OK;
 }
 
-// cropIndices.m: the non-empty slice indices along each axis.
-void crop_indices(const uint8_t *m, const size_t sz[3], std::vector<size_t> ix[3])
-{
-    std::vector<char> any[3] = {std::vector<char>(sz[0], 0), std::vector<char>(sz[1], 0),
-                                std::vector<char>(sz[2], 0)};
-    size_t l = 0;
-    for (size_t k = 0; k < sz[2]; ++k)
-        for (size_t j = 0; j < sz[1]; ++j)
-            for (size_t i = 0; i < sz[0]; ++i, ++l)
-                if (m[l]) any[0][i] = any[1][j] = any[2][k] = 1;
-    for (int a = 0; a < 3; ++a) {
-        ix[a].clear();
-        for (size_t i = 0; i < sz[a]; ++i)
-            if (any[a][i]) ix[a].push_back(i);
-    }
-}
-
 // Host forms: crop to the mask's non-empty slices (sharp.m:51-56), run on
 // the device, uncrop into zeros (sharp.m:81-84).  work_bytes(szc) sizes the
 // device work space; run(fc, mc, szc, flc, mo, work) is the device form.
+// The crop and the uncrop run on the device: the whole field and mask go
+// up, the whole results come down into the caller's arrays.
 template <typename WorkBytes, typename Run>
 int host_form(int dtype, const void *f, const uint8_t *mask, const size_t sz[3], size_t nech,
               void *fl, uint8_t *mask_out, WorkBytes work_bytes, Run run)
 {
     const size_t es = dtype_size(dtype), N = sz[0] * sz[1] * sz[2];
-    std::vector<size_t> ix[3];
-    crop_indices(mask, sz, ix);
-    std::memset(fl, 0, N * nech * es);
-    if (mask_out) std::memset(mask_out, 0, N);
-    if (ix[0].empty()) return QSM_OK;  // empty mask: nothing to remove
-    const size_t szc[3] = {ix[0].size(), ix[1].size(), ix[2].size()};
-    const size_t n = szc[0] * szc[1] * szc[2];
-    std::vector<char> fc(n * nech * es), flc(n * nech * es);
-    std::vector<uint8_t> mc(n), mo(n);
-    // runs of consecutive x indices are copied whole
-    std::vector<std::pair<size_t, size_t>> runs;  // (first x, length)
-    for (size_t a = 0; a < ix[0].size();) {
-        size_t b = a + 1;
-        while (b < ix[0].size() && ix[0][b] == ix[0][b - 1] + 1) ++b;
-        runs.emplace_back(ix[0][a], b - a);
-        a = b;
+    std::vector<char> any[3];
+    axis_any(mask, sz, any);
+    std::vector<int> ix;  // the x, then y, then z indices of non-empty slices
+    size_t szc[3];
+    for (int a = 0; a < 3; ++a) {
+        const size_t before = ix.size();
+        for (size_t i = 0; i < sz[a]; ++i)
+            if (any[a][i]) ix.push_back(int(i));
+        szc[a] = ix.size() - before;
     }
-    size_t c = 0;
-    for (size_t k : ix[2])
-        for (size_t j : ix[1])
-            for (const auto &rn : runs) {
-                const size_t l = rn.first + sz[0] * (j + sz[1] * k);
-                for (size_t t = 0; t < rn.second; ++t) mc[c + t] = mask[l + t] ? 1 : 0;
-                for (size_t e = 0; e < nech; ++e)
-                    std::memcpy(&fc[(e * n + c) * es],
-                                static_cast<const char *>(f) + (e * N + l) * es, rn.second * es);
-                c += rn.second;
-            }
+    if (szc[0] == 0) {  // empty mask: nothing to remove
+        std::memset(fl, 0, N * nech * es);
+        if (mask_out) std::memset(mask_out, 0, N);
+        return QSM_OK;
+    }
+    const size_t n = szc[0] * szc[1] * szc[2];
     int rc;
     if ((rc = ensure_device())) return rc;
-    DevBuf df, dm, dfl, dmo, dw;
-    if ((rc = df.alloc(n * nech * es)) || (rc = dm.alloc(n)) || (rc = dfl.alloc(n * nech * es))
-        || (rc = dmo.alloc(n)) || (rc = dw.alloc(work_bytes(szc))))
+    // dF, dM: the whole field and mask, then the whole results
+    DevBuf dF, dM, dix, df, dm, dfl, dmo, dw;
+    if ((rc = dF.alloc(N * nech * es)) || (rc = dM.alloc(N)) ||
+        (rc = dix.alloc(ix.size() * sizeof(int))) || (rc = df.alloc(n * nech * es)) ||
+        (rc = dm.alloc(n)) || (rc = dfl.alloc(n * nech * es)) || (rc = dmo.alloc(n)) ||
+        (rc = dw.alloc(work_bytes(szc))))
         return rc;
-    QSM_HIP(hipMemcpy(df.p, fc.data(), fc.size(), hipMemcpyHostToDevice));
-    QSM_HIP(hipMemcpy(dm.p, mc.data(), n, hipMemcpyHostToDevice));
+    QSM_HIP(hipMemcpy(dF.p, f, N * nech * es, hipMemcpyHostToDevice));
+    QSM_HIP(hipMemcpy(dM.p, mask, N, hipMemcpyHostToDevice));
+    QSM_HIP(hipMemcpy(dix.p, ix.data(), ix.size() * sizeof(int), hipMemcpyHostToDevice));
+    const int *dixp = static_cast<const int *>(dix.p);
+    const CropMap cm{dixp, dixp + szc[0], dixp + szc[0] + szc[1], int(szc[0]), int(szc[1]),
+                     int(szc[2]), sz[0], sz[1], N, n};
+    const int ne = int(nech);
+    uint8_t *Mb = static_cast<uint8_t *>(dM.p);
+    crop_gather_kernel<uint8_t, true><<<crop_blocks(cm, 1), 256>>>(
+        static_cast<uint8_t *>(dm.p), Mb, cm, 1);
+    if (dtype == QSM_F32)
+        crop_gather_kernel<float><<<crop_blocks(cm, ne), 256>>>(
+            static_cast<float *>(df.p), static_cast<const float *>(dF.p), cm, ne);
+    else
+        crop_gather_kernel<double><<<crop_blocks(cm, ne), 256>>>(
+            static_cast<double *>(df.p), static_cast<const double *>(dF.p), cm, ne);
+    QSM_HIP(hipGetLastError());
     if ((rc = run(df.p, static_cast<const uint8_t *>(dm.p), szc, dfl.p,
                   static_cast<uint8_t *>(dmo.p), dw.p)))
         return rc;
-    QSM_HIP(hipStreamSynchronize(nullptr));
-    QSM_HIP(hipMemcpy(flc.data(), dfl.p, flc.size(), hipMemcpyDeviceToHost));
-    QSM_HIP(hipMemcpy(mo.data(), dmo.p, n, hipMemcpyDeviceToHost));
-    c = 0;
-    for (size_t k : ix[2])
-        for (size_t j : ix[1])
-            for (const auto &rn : runs) {
-                const size_t l = rn.first + sz[0] * (j + sz[1] * k);
-                if (mask_out) std::memcpy(mask_out + l, &mo[c], rn.second);
-                for (size_t e = 0; e < nech; ++e)
-                    std::memcpy(static_cast<char *>(fl) + (e * N + l) * es,
-                                &flc[(e * n + c) * es], rn.second * es);
-                c += rn.second;
-            }
+    // uncrop into zeros
+    QSM_HIP(hipMemsetAsync(dF.p, 0, N * nech * es, nullptr));
+    if (dtype == QSM_F32)
+        crop_scatter_kernel<float><<<crop_blocks(cm, ne), 256>>>(
+            static_cast<float *>(dF.p), static_cast<const float *>(dfl.p), cm, ne);
+    else
+        crop_scatter_kernel<double><<<crop_blocks(cm, ne), 256>>>(
+            static_cast<double *>(dF.p), static_cast<const double *>(dfl.p), cm, ne);
+    if (mask_out) {
+        QSM_HIP(hipMemsetAsync(Mb, 0, N, nullptr));
+        crop_scatter_kernel<uint8_t><<<crop_blocks(cm, 1), 256>>>(
+            Mb, static_cast<const uint8_t *>(dmo.p), cm, 1);
+    }
+    QSM_HIP(hipGetLastError());
+    QSM_HIP(hipMemcpy(fl, dF.p, N * nech * es, hipMemcpyDeviceToHost));
+    if (mask_out) QSM_HIP(hipMemcpy(mask_out, Mb, N, hipMemcpyDeviceToHost));
     return QSM_OK;
 }
 

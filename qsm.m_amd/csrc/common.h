@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 #include "qsm_hip.h"
 
@@ -51,6 +52,57 @@ struct DevBuf {
     ~DevBuf() { if (p) (void)hipFree(p); }
     int alloc(size_t bytes);
 };
+
+// Per-axis occupancy of a mask (sz[0] x sz[1] x sz[2], x fastest):
+// any[a][i] = 1 where slice i of axis a holds a nonzero voxel.  One pass: a
+// byte OR per x position and a word OR per row, both vectorisable.
+void axis_any(const uint8_t *m, const size_t sz[3], std::vector<char> any[3]);
+
+// A crop of a volume by per-axis index lists (device arrays), for the
+// host-pointer entry points, which crop and uncrop on the device.
+struct CropMap {
+    const int *ix, *iy, *iz;  // listed x, y, z indices of the volume
+    int cx, cy, cz;           // crop size (list lengths)
+    size_t nx, ny;            // the volume's x and y size
+    size_t N, n;              // voxels of the volume and of the crop
+};
+
+__device__ __forceinline__ size_t crop_source(const CropMap &c, size_t q)
+{
+    const size_t cxy = size_t(c.cx) * c.cy;
+    const size_t k = q / cxy, r = q - k * cxy, j = r / c.cx, i = r - j * c.cx;
+    return size_t(c.ix[i]) + c.nx * (size_t(c.iy[j]) + c.ny * size_t(c.iz[k]));
+}
+
+// out[e n + q] = in[e N + source(q)] for nech volumes (NORM: a byte mask
+// made 0/1).
+template <typename T, bool NORM = false>
+__global__ __launch_bounds__(256) void crop_gather_kernel(T *out, const T *in, CropMap c,
+                                                          int nech)
+{
+    const size_t t = size_t(blockIdx.x) * 256 + threadIdx.x;
+    if (t >= c.n * size_t(nech)) return;
+    const size_t e = t / c.n, q = t - e * c.n;
+    const T v = in[e * c.N + crop_source(c, q)];
+    out[t] = NORM ? T(v ? 1 : 0) : v;
+}
+
+// The inverse: out[e N + source(q)] = in[e n + q] (the rest of out is
+// left as it is).
+template <typename T>
+__global__ __launch_bounds__(256) void crop_scatter_kernel(T *out, const T *in, CropMap c,
+                                                           int nech)
+{
+    const size_t t = size_t(blockIdx.x) * 256 + threadIdx.x;
+    if (t >= c.n * size_t(nech)) return;
+    const size_t e = t / c.n, q = t - e * c.n;
+    out[e * c.N + crop_source(c, q)] = in[t];
+}
+
+inline unsigned crop_blocks(const CropMap &c, int nech)
+{
+    return unsigned((c.n * size_t(nech) + 255) / 256);
+}
 
 // Correctly rounded square root for both storage types (the reference calls
 // sqrt on the double path and sqrtf on the single path).

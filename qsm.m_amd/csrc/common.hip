@@ -51,6 +51,25 @@ int ensure_device()
     return QSM_OK;
 }
 
+void axis_any(const uint8_t *m, const size_t sz[3], std::vector<char> any[3])
+{
+    for (int a = 0; a < 3; ++a) any[a].assign(sz[a], 0);
+    std::vector<uint8_t> col(sz[0], 0);
+    const size_t nx = sz[0];
+    size_t l = 0;
+    for (size_t k = 0; k < sz[2]; ++k)
+        for (size_t j = 0; j < sz[1]; ++j, l += nx) {
+            const uint8_t *row = m + l;
+            uint8_t acc = 0;
+            for (size_t i = 0; i < nx; ++i) {
+                col[i] |= row[i];
+                acc |= row[i];
+            }
+            if (acc) any[1][j] = any[2][k] = 1;
+        }
+    for (size_t i = 0; i < nx; ++i) any[0][i] = col[i] ? 1 : 0;
+}
+
 int DevBuf::alloc(size_t bytes)
 {
     if (p) { (void)hipFree(p); p = nullptr; }

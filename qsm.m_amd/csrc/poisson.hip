@@ -1015,26 +1015,20 @@ int qsm_unwrap_laplacian(int dtype, const void *phas, const uint8_t *mask, const
     int rc = check_grid(sz, vsz, 1);
     if (rc) return rc;
     const size_t es = dtype_size(dtype), N = sz[0] * sz[1] * sz[2];
-    std::memset(uphas, 0, N * nechoes * es);
-    if (d2phas) std::memset(d2phas, 0, N * nechoes * es);
-    size_t lo[3] = {sz[0], sz[1], sz[2]}, hi[3] = {0, 0, 0};
-    size_t l = 0;
-    for (size_t k = 0; k < sz[2]; ++k)
-        for (size_t j = 0; j < sz[1]; ++j)
-            for (size_t i = 0; i < sz[0]; ++i, ++l)
-                if (mask[l]) {
-                    const size_t x[3] = {i, j, k};
-                    for (int a = 0; a < 3; ++a) {
-                        lo[a] = std::min(lo[a], x[a]);
-                        hi[a] = std::max(hi[a], x[a]);
-                    }
-                }
-    if (lo[0] > hi[0]) return QSM_OK;  // empty mask
+    std::vector<char> any[3];
+    axis_any(mask, sz, any);
     size_t c0[3], cs[3];
     for (int a = 0; a < 3; ++a) {
-        c0[a] = lo[a] > 0 ? lo[a] - 1 : 0;
-        const size_t c1 = std::min(sz[a] - 1, hi[a] + 1);
-        cs[a] = c1 - c0[a] + 1;
+        size_t lo = 0, hi = sz[a];
+        while (lo < sz[a] && !any[a][lo]) ++lo;
+        while (hi > lo && !any[a][hi - 1]) --hi;
+        if (lo == hi) {  // empty mask
+            std::memset(uphas, 0, N * nechoes * es);
+            if (d2phas) std::memset(d2phas, 0, N * nechoes * es);
+            return QSM_OK;
+        }
+        c0[a] = lo > 0 ? lo - 1 : 0;
+        cs[a] = std::min(sz[a] - 1, hi) - c0[a] + 1;  // through hi - 1 + 1, clamped
     }
     qsm_mg_opts o;
     qsm_mg_opts_default(&o);
@@ -1042,40 +1036,62 @@ int qsm_unwrap_laplacian(int dtype, const void *phas, const uint8_t *mask, const
     if ((rc = check_mg(dtype, cs, vsz, &o, L))) return rc;
     if ((rc = ensure_device())) return rc;
     const size_t n = cs[0] * cs[1] * cs[2];
-    std::vector<char> buf(n * es), out(n * es), d2(n * es);
-    std::vector<uint8_t> mc(n);
-    auto crop_copy = [&](const char *src, char *dst, bool to_crop) {
-        for (size_t k = 0; k < cs[2]; ++k)
-            for (size_t j = 0; j < cs[1]; ++j) {
-                const size_t lf = c0[0] + sz[0] * ((c0[1] + j) + sz[1] * (c0[2] + k));
-                const size_t lc = cs[0] * (j + cs[1] * k);
-                if (to_crop) std::memcpy(dst + lc * es, src + lf * es, cs[0] * es);
-                else std::memcpy(dst + lf * es, src + lc * es, cs[0] * es);
-            }
-    };
-    for (size_t k = 0; k < cs[2]; ++k)
-        for (size_t j = 0; j < cs[1]; ++j)
-            for (size_t i = 0; i < cs[0]; ++i)
-                mc[i + cs[0] * (j + cs[1] * k)] =
-                    mask[(c0[0] + i) + sz[0] * ((c0[1] + j) + sz[1] * (c0[2] + k))] ? 1 : 0;
-    DevBuf du, dm, dout, dd2, dw;
-    if ((rc = du.alloc(n * es)) || (rc = dm.alloc(n)) || (rc = dout.alloc(n * es))
-        || (rc = dd2.alloc(n * es)) || (rc = dw.alloc(qsm_mgpcg_work_bytes(dtype, cs, L))))
+    // the crop and the uncrop run on the device: dP and dD hold a whole
+    // echo (phase in, then the unwrapped phase; the Laplacian)
+    std::vector<int> ix;
+    for (int a = 0; a < 3; ++a)
+        for (size_t i = 0; i < cs[a]; ++i) ix.push_back(int(c0[a] + i));
+    DevBuf dP, dD, dM, dix, du, dm, dout, dd2, dw;
+    if ((rc = dP.alloc(N * es)) || (rc = dM.alloc(N)) || (d2phas && (rc = dD.alloc(N * es))) ||
+        (rc = dix.alloc(ix.size() * sizeof(int))) || (rc = du.alloc(n * es)) ||
+        (rc = dm.alloc(n)) || (rc = dout.alloc(n * es)) || (rc = dd2.alloc(n * es)) ||
+        (rc = dw.alloc(qsm_mgpcg_work_bytes(dtype, cs, L))))
         return rc;
-    QSM_HIP(hipMemcpy(dm.p, mc.data(), n, hipMemcpyHostToDevice));
+    QSM_HIP(hipMemcpy(dM.p, mask, N, hipMemcpyHostToDevice));
+    QSM_HIP(hipMemcpy(dix.p, ix.data(), ix.size() * sizeof(int), hipMemcpyHostToDevice));
+    const int *dixp = static_cast<const int *>(dix.p);
+    const CropMap cm{dixp, dixp + cs[0], dixp + cs[0] + cs[1], int(cs[0]), int(cs[1]),
+                     int(cs[2]), sz[0], sz[1], N, n};
+    crop_gather_kernel<uint8_t, true><<<crop_blocks(cm, 1), 256>>>(
+        static_cast<uint8_t *>(dm.p), static_cast<const uint8_t *>(dM.p), cm, 1);
+    QSM_HIP(hipGetLastError());
+    // one echo's crop in and out (es bytes per sample)
+    auto gather = [&](void *dst, const void *src) {
+        if (dtype == QSM_F32)
+            crop_gather_kernel<float><<<crop_blocks(cm, 1), 256>>>(
+                static_cast<float *>(dst), static_cast<const float *>(src), cm, 1);
+        else
+            crop_gather_kernel<double><<<crop_blocks(cm, 1), 256>>>(
+                static_cast<double *>(dst), static_cast<const double *>(src), cm, 1);
+    };
+    auto scatter = [&](void *dst, const void *src) {
+        if (dtype == QSM_F32)
+            crop_scatter_kernel<float><<<crop_blocks(cm, 1), 256>>>(
+                static_cast<float *>(dst), static_cast<const float *>(src), cm, 1);
+        else
+            crop_scatter_kernel<double><<<crop_blocks(cm, 1), 256>>>(
+                static_cast<double *>(dst), static_cast<const double *>(src), cm, 1);
+    };
     for (size_t e = 0; e < nechoes; ++e) {
-        crop_copy(static_cast<const char *>(phas) + e * N * es, buf.data(), true);
-        QSM_HIP(hipMemcpy(du.p, buf.data(), n * es, hipMemcpyHostToDevice));
+        QSM_HIP(hipMemcpy(dP.p, static_cast<const char *>(phas) + e * N * es, N * es,
+                          hipMemcpyHostToDevice));
+        gather(du.p, dP.p);
+        QSM_HIP(hipGetLastError());
         if ((rc = qsm_unwrap_laplacian_dev(dtype, du.p, static_cast<const uint8_t *>(dm.p), cs,
                                            vsz, dout.p, dd2.p, dw.p, nullptr, nullptr)))
             return rc;
-        QSM_HIP(hipStreamSynchronize(nullptr));
-        QSM_HIP(hipMemcpy(out.data(), dout.p, n * es, hipMemcpyDeviceToHost));
-        crop_copy(out.data(), static_cast<char *>(uphas) + e * N * es, false);
+        QSM_HIP(hipMemsetAsync(dP.p, 0, N * es, nullptr));
+        scatter(dP.p, dout.p);
         if (d2phas) {
-            QSM_HIP(hipMemcpy(d2.data(), dd2.p, n * es, hipMemcpyDeviceToHost));
-            crop_copy(d2.data(), static_cast<char *>(d2phas) + e * N * es, false);
+            QSM_HIP(hipMemsetAsync(dD.p, 0, N * es, nullptr));
+            scatter(dD.p, dd2.p);
         }
+        QSM_HIP(hipGetLastError());
+        QSM_HIP(hipMemcpy(static_cast<char *>(uphas) + e * N * es, dP.p, N * es,
+                          hipMemcpyDeviceToHost));
+        if (d2phas)
+            QSM_HIP(hipMemcpy(static_cast<char *>(d2phas) + e * N * es, dD.p, N * es,
+                              hipMemcpyDeviceToHost));
     }
     return QSM_OK;
 }
