@@ -77,6 +77,19 @@ int qsm_pd_tv(int dtype, const void *d2f, const uint8_t *mask,
               double lam, double tol, uint32_t maxiter, uint32_t verbose,
               void *chi, qsm_print_fn print, void *print_ctx);
 
+/* Replaces the per-echo body of sstgv.m:54-89 (sstgv_): crop phas (nechoes
+ * volumes of sz, column-major) and mask to the mask's bounding box +/-2
+ * voxels (sstgv.m:57-66), d2u = lapw(u, vsz) on the crop, then
+ * pd_tgv_mex(-d2u, mask, vsz, lam, alpha, tol, maxit, verbose) when alpha >
+ * eps, pd_tv_mex(-d2u, mask, vsz, lam, tol, maxit, verbose) otherwise
+ * (sstgv.m:76-87), uncropped into zeros: x holds nechoes volumes of sz.  The
+ * argument checks of sstgv.m:92-136 stay with the caller.  Every step after
+ * the bounding box runs on the device; the same chi as those calls. */
+int qsm_sstgv(int dtype, const void *phas, const uint8_t *mask, const size_t sz[3],
+              size_t nechoes, const double vsz[3], double lam, double alpha,
+              double tol, uint32_t maxit, uint32_t verbose, void *x,
+              qsm_print_fn print, void *print_ctx);
+
 /* ------------------------------------------------------------------------
  * Resident solver handle: the state of one inversion kept in HBM between
  * calls (used by the drop-in entry points above, by the benchmark, and by the

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <mutex>
 #include <new>
 #include <vector>
@@ -789,6 +790,100 @@ int qsm_pd_tgv(int dtype, const void *d2f, const uint8_t *mask,
 {
     return pd_solve(QSM_TGV, dtype, d2f, mask, sz, h, lam, alpha, tol, maxiter,
                     verbose, chi, print, print_ctx);
+}
+
+// sstgv.m:54-89 on host arrays, echo by echo, with every step after the
+// mask's bounding box on the device: one upload of the echo and the mask,
+// the crop (box +/-2, sstgv.m:57-66), d2u = lapw(u) on the crop, the solve
+// of pd_tgv_mex(-d2u, ...) (alpha > eps) or pd_tv_mex(-d2u, ...) (sstgv.m:
+// 76-87) with the same progress output, the uncrop into zeros, one
+// download.
+int qsm_sstgv(int dtype, const void *phas, const uint8_t *mask, const size_t sz[3],
+              size_t nechoes, const double vsz[3], double lam, double alpha, double tol,
+              uint32_t maxit, uint32_t verbose, void *x, qsm_print_fn print, void *print_ctx)
+{
+    QSM_ARG(dtype == QSM_F64 || dtype == QSM_F32, "bad dtype");
+    QSM_ARG(phas && mask && x && nechoes >= 1, "null pointer or no echo");
+    QSM_ARG(maxit > 0, "maxit must be > 0");
+    int rc = check_grid(sz, vsz, 1);
+    if (rc) return rc;
+    const size_t es = dtype_size(dtype), N = sz[0] * sz[1] * sz[2];
+    std::vector<char> any[3];
+    axis_any(mask, sz, any);
+    size_t c0[3], cs[3];
+    for (int a = 0; a < 3; ++a) {
+        size_t lo = 0, hi = sz[a];
+        while (lo < sz[a] && !any[a][lo]) ++lo;
+        while (hi > lo && !any[a][hi - 1]) --hi;
+        if (lo == hi) {  // empty mask: chi = 0
+            std::memset(x, 0, N * nechoes * es);
+            return QSM_OK;
+        }
+        c0[a] = lo >= 2 ? lo - 2 : 0;
+        cs[a] = std::min(sz[a], hi + 2) - c0[a];
+    }
+    if ((rc = check_grid(cs, vsz, 1))) return rc;
+    if ((rc = ensure_device())) return rc;
+    const size_t n = cs[0] * cs[1] * cs[2];
+    std::vector<int> ix;
+    for (int a = 0; a < 3; ++a)
+        for (size_t i = 0; i < cs[a]; ++i) ix.push_back(int(c0[a] + i));
+    DevBuf dP, dM, dix, du, dm, dd, dchi;
+    if ((rc = dP.alloc(N * es)) || (rc = dM.alloc(N)) ||
+        (rc = dix.alloc(ix.size() * sizeof(int))) || (rc = du.alloc(n * es)) ||
+        (rc = dm.alloc(n)) || (rc = dd.alloc(n * es)) || (rc = dchi.alloc(n * es)))
+        return rc;
+    QSM_HIP(hipMemcpy(dM.p, mask, N, hipMemcpyHostToDevice));
+    QSM_HIP(hipMemcpy(dix.p, ix.data(), ix.size() * sizeof(int), hipMemcpyHostToDevice));
+    const int *dixp = static_cast<const int *>(dix.p);
+    const CropMap cm{dixp, dixp + cs[0], dixp + cs[0] + cs[1], int(cs[0]), int(cs[1]),
+                     int(cs[2]), sz[0], sz[1], N, n};
+    crop_gather_kernel<uint8_t, true><<<crop_blocks(cm, 1), 256>>>(
+        static_cast<uint8_t *>(dm.p), static_cast<const uint8_t *>(dM.p), cm, 1);
+    QSM_HIP(hipGetLastError());
+    const bool tgv = alpha > std::numeric_limits<double>::epsilon();
+    for (size_t e = 0; e < nechoes; ++e) {
+        QSM_HIP(hipMemcpy(dP.p, static_cast<const char *>(phas) + e * N * es, N * es,
+                          hipMemcpyHostToDevice));
+        // d2f = -lapw(u) on the crop (lapw writes the interior; faces 0)
+        QSM_HIP(hipMemsetAsync(dd.p, 0, n * es, nullptr));
+        if (dtype == QSM_F32)
+            crop_gather_kernel<float><<<crop_blocks(cm, 1), 256>>>(
+                static_cast<float *>(du.p), static_cast<const float *>(dP.p), cm, 1);
+        else
+            crop_gather_kernel<double><<<crop_blocks(cm, 1), 256>>>(
+                static_cast<double *>(du.p), static_cast<const double *>(dP.p), cm, 1);
+        QSM_HIP(hipGetLastError());
+        if ((rc = qsm_lapw_dev(dtype, dd.p, du.p, cs, vsz, nullptr))) return rc;
+        if (dtype == QSM_F32)
+            QSM_HIP(launch_scale<float>(static_cast<float *>(dd.p),
+                                        static_cast<const float *>(dd.p), -1.0f, n, nullptr));
+        else
+            QSM_HIP(launch_scale<double>(static_cast<double *>(dd.p),
+                                         static_cast<const double *>(dd.p), -1.0, n, nullptr));
+        // the solve of pd_solve, from and to device arrays
+        qsm_pd *h = nullptr;
+        if ((rc = create_impl(&h, tgv ? QSM_TGV : QSM_TV, dtype, cs, vsz, lam,
+                              tgv ? alpha : 0.0, nullptr, int(maxit / 10) + 1)))
+            return rc;
+        if (!(rc = qsm_pd_load(h, dd.p, static_cast<const uint8_t *>(dm.p), 1)) &&
+            !(rc = qsm_pd_iterate(h, maxit, tol)) && !(rc = qsm_pd_get_chi(h, dchi.p, 1)))
+            print_progress(h, maxit, verbose, print, print_ctx);
+        qsm_pd_destroy(h);
+        if (rc) return rc;
+        // uncrop into zeros
+        QSM_HIP(hipMemsetAsync(dP.p, 0, N * es, nullptr));
+        if (dtype == QSM_F32)
+            crop_scatter_kernel<float><<<crop_blocks(cm, 1), 256>>>(
+                static_cast<float *>(dP.p), static_cast<const float *>(dchi.p), cm, 1);
+        else
+            crop_scatter_kernel<double><<<crop_blocks(cm, 1), 256>>>(
+                static_cast<double *>(dP.p), static_cast<const double *>(dchi.p), cm, 1);
+        QSM_HIP(hipGetLastError());
+        QSM_HIP(hipMemcpy(static_cast<char *>(x) + e * N * es, dP.p, N * es,
+                          hipMemcpyDeviceToHost));
+    }
+    return QSM_OK;
 }
 
 int qsm_pd_tv(int dtype, const void *d2f, const uint8_t *mask,

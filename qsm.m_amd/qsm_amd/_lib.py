@@ -48,6 +48,9 @@ _SIGS = {
                              PRINT_FN, _p]),
     "qsm_pd_tv": (C.c_int, [C.c_int, _p, _u8p, C.POINTER(_sz_t), _dp, C.c_double,
                             C.c_double, C.c_uint32, C.c_uint32, _p, PRINT_FN, _p]),
+    "qsm_sstgv": (C.c_int, [C.c_int, _p, _u8p, C.POINTER(_sz_t), C.c_size_t, _dp,
+                            C.c_double, C.c_double, C.c_double, C.c_uint32, C.c_uint32, _p,
+                            PRINT_FN, _p]),
     "qsm_pd_create": (C.c_int, [C.POINTER(_p), C.c_int, C.c_int, C.POINTER(_sz_t),
                                 _dp, C.c_double, C.c_double, _p]),
     "qsm_pd_footprint": (_sz_t, [C.c_int, C.c_int, C.POINTER(_sz_t)]),

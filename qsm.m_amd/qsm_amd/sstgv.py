@@ -4,17 +4,16 @@ Same arguments, defaults and flow as the reference: validate, crop phase
 and mask to the mask's bounding box +/-2 voxels (sstgv.m:57-66), and per
 echo compute d2u = lapw(u, vsz) and call pd_tgv_mex(-d2u, ...) when
 alpha > eps, pd_tv_mex(-d2u, ...) otherwise (sstgv.m:76-87), then uncrop
-into zeros.  lapw and the inversion run on the GPU.
+into zeros.  The validation runs here; everything after it is one library
+call (qsm_sstgv), which crops, runs lapw and the inversion and uncrops on
+the GPU.
 """
 from __future__ import annotations
 
 import numpy as np
 
-from . import fd, pd
-from .fastfft import crop_indices
-
-EPS = np.finfo(np.float64).eps
-
+from ._lib import check, dtype_code, fortran, h3, lib, ptr, sz3
+from .pd import _printer
 
 def _validate(phas, mask, vsz, lam, alpha, tol, maxit, verbose):
     phas = np.asarray(phas)
@@ -49,26 +48,14 @@ def sstgv(phas, mask, vsz, lam=1e-4, alpha=3.0, tol=3e-4, maxit=5000, verbose=Tr
     alpha1 = lambda, alpha0 = alpha*lambda; alpha < eps selects TV.
     """
     _validate(phas, mask, vsz, lam, alpha, tol, maxit, verbose)
-    phas = np.asarray(phas)
-    mask = np.asarray(mask) != 0
+    phas = fortran(np.asarray(phas))
+    m = fortran(np.asarray(mask) != 0, np.uint8)
     vsz = np.asarray(vsz, dtype=np.float64).reshape(3)
-    x = np.zeros(phas.shape, dtype=phas.dtype, order="F")
-    ix, iy, iz = crop_indices(mask)
-    if ix.size == 0:
-        return x
-    sl = tuple(slice(max(0, i[0] - 2), min(n, i[-1] + 3)) for i, n in zip((ix, iy, iz), mask.shape))
-    m = np.asfortranarray(mask[sl])
+    x = np.empty(phas.shape, dtype=phas.dtype, order="F")  # written whole by the call
     echoes = 1 if phas.ndim == 3 else phas.shape[3]
-    for t in range(echoes):
-        u = phas[sl] if phas.ndim == 3 else phas[sl + (t,)]
-        d2u = fd.lapw(np.asfortranarray(u), vsz)
-        vb = int(verbose)
-        if alpha > EPS:
-            chi = pd.pd_tgv_mex(-d2u, m, vsz, lam, alpha, tol, maxit, vb)
-        else:
-            chi = pd.pd_tv_mex(-d2u, m, vsz, lam, tol, maxit, vb)
-        if phas.ndim == 3:
-            x[sl] = chi
-        else:
-            x[sl + (t,)] = chi
+    cb, keep = _printer(int(verbose))
+    check(lib().qsm_sstgv(dtype_code(phas.dtype), ptr(phas), ptr(m), sz3(phas.shape[:3]), echoes,
+                          h3(vsz), float(lam), float(alpha), float(tol), int(maxit),
+                          int(verbose), ptr(x), cb, None))
+    del keep
     return x
