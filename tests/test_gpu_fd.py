@@ -161,8 +161,9 @@ def test_device_entry_points(cuda, dt):
     _dev_call("qsm_gradf_dev", dt, (dx, dy, dz), (U,), shape, stream=st)
     for a, b in zip(grab(dx, dy, dz), orc.gradf(u, H)):
         _same(a, b)
-    for t in (dx, dy, dz):
-        t.zero_()
+    with torch.cuda.stream(st):  # ordered before the entry point on st
+        for t in (dx, dy, dz):
+            t.zero_()
     _dev_call("qsm_gradfm_dev", dt, (dx, dy, dz), (U,), shape, Gd, stream=st)
     for a, b in zip(grab(dx, dy, dz), orc.gradfm(u, G, H)):
         _same(a, b)
@@ -170,10 +171,12 @@ def test_device_entry_points(cuda, dt):
     _same(grab(du)[0], orc.gradfp_adj(x, y, z, H))
     _dev_call("qsm_gradf_adj_dev", dt, (du,), (X, Y, Z), shape, stream=st)
     _same(grab(du)[0], orc.gradf_adj(x, y, z, H))
-    du.zero_()
+    with torch.cuda.stream(st):
+        du.zero_()
     _dev_call("qsm_gradfm_adj_dev", dt, (du,), (X, Y, Z), shape, Gd, stream=st)
     _same(grab(du)[0], orc.gradfm_adj(x, y, z, G, H))
-    du.zero_()
+    with torch.cuda.stream(st):
+        du.zero_()
     _dev_call("qsm_lapw_dev", dt, (du,), (U,), shape, stream=st)
     _same(grab(du)[0], orc.lapw(u, H))
 
