@@ -172,3 +172,13 @@ def fortran(a, dtype=None) -> np.ndarray:
     """Column-major (MATLAB) contiguous copy-or-view of a."""
     a = np.asarray(a, dtype=dtype)
     return np.asfortranarray(a)
+
+
+def mask_u8(mask) -> np.ndarray:
+    """Column-major uint8 0/1 array of mask != 0 for the library's mask
+    arguments: a view of a column-major bool mask, else one converted
+    copy."""
+    m = np.asarray(mask)
+    if m.dtype != np.bool_:
+        m = m != 0
+    return np.asfortranarray(m).view(np.uint8)

@@ -10,7 +10,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import check, dtype_code, fortran, h3, lib, ptr, sz3
+from ._lib import check, dtype_code, fortran, h3, lib, mask_u8, ptr, sz3
 
 _p = C.c_void_p
 _sz = C.c_size_t
@@ -84,7 +84,7 @@ class Batch:
                                       int(kz), C.byref(slot)))
         else:
             f = fortran(np.asarray(d2f), self.dtype)
-            m = fortran(np.asarray(mask) != 0, np.uint8)
+            m = mask_u8(mask)
             shape = f.shape
             check(lib().qsm_batch_add(self._h, kind, sz3(shape), h3(vsz), float(lam), float(alpha),
                                       ptr(f), ptr(m), 0, int(maxiter), float(tol), int(kz),

@@ -15,7 +15,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import check, dtype_code, fortran, h3, lib, ptr
+from ._lib import check, dtype_code, fortran, h3, lib, mask_u8, ptr
 
 _sz = C.c_size_t
 _p = C.c_void_p
@@ -42,7 +42,7 @@ def _prep(f, mask):
     if m.shape != f.shape[:3]:
         raise ValueError("mask must match the first three dimensions of f")
     ne = 1 if f.ndim == 3 else f.shape[3]
-    return fortran(f, f.dtype), fortran(m != 0, np.uint8), ne
+    return fortran(f, f.dtype), mask_u8(m), ne
 
 
 def default_radii(vsz):

@@ -10,7 +10,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from ._lib import check, dtype_code, fortran, h3, lib, ptr, sz3
+from ._lib import check, dtype_code, fortran, h3, lib, mask_u8, ptr, sz3
 
 
 def _out_ok(a, ref):
@@ -27,7 +27,7 @@ def _vec_op(name, outs, u, h, G=None):
         _out_ok(o, u)
     args = [dtype_code(u.dtype), *(ptr(o) for o in outs), ptr(u)]
     if G is not None:
-        g = fortran(np.asarray(G) != 0, np.uint8)
+        g = mask_u8(G)
         args.append(ptr(g))
     check(getattr(lib(), name)(*args, sz3(u.shape), h3(h)))
 
@@ -37,7 +37,7 @@ def _adj_op(name, du, x, y, z, h, G=None):
     _out_ok(du, x)
     args = [dtype_code(du.dtype), ptr(du), ptr(x), ptr(y), ptr(z)]
     if G is not None:
-        g = fortran(np.asarray(G) != 0, np.uint8)
+        g = mask_u8(G)
         args.append(ptr(g))
     check(getattr(lib(), name)(*args, sz3(du.shape), h3(h)))
 
@@ -125,7 +125,7 @@ def lap1_mex(d2u, u, G, h):
     if u.dtype not in (np.float32, np.float64):
         raise TypeError("u must be single or double")
     _out_ok(d2u, u)
-    g = fortran(np.asarray(G) != 0, np.uint8)
+    g = mask_u8(G)
     if g.shape != u.shape:
         raise ValueError("G must have the shape of u")
     check(lib().qsm_lap1(dtype_code(u.dtype), ptr(d2u), ptr(u), ptr(g), sz3(u.shape), h3(h)))

@@ -16,7 +16,7 @@ import sys
 import numpy as np
 
 from . import _lib
-from ._lib import check, dtype_code, fortran, h3, lib, ptr, sz3
+from ._lib import check, dtype_code, fortran, h3, lib, mask_u8, ptr, sz3
 
 
 def _printer(verbose):
@@ -35,7 +35,7 @@ def _inputs(d2f, mask):
     if f.dtype not in (np.float32, np.float64):
         f = f.astype(np.float64)
     f = fortran(f)
-    m = fortran(np.asarray(mask) != 0, np.uint8)
+    m = mask_u8(mask)
     if m.shape != f.shape:
         raise ValueError(f"mask shape {m.shape} != d2f shape {f.shape}")
     return f, m

@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from ._lib import check, dtype_code, fortran, h3, lib, ptr, sz3
+from ._lib import check, dtype_code, fortran, h3, lib, mask_u8, ptr, sz3
 from .pd import _printer
 
 def _validate(phas, mask, vsz, lam, alpha, tol, maxit, verbose):
@@ -49,7 +49,7 @@ def sstgv(phas, mask, vsz, lam=1e-4, alpha=3.0, tol=3e-4, maxit=5000, verbose=Tr
     """
     _validate(phas, mask, vsz, lam, alpha, tol, maxit, verbose)
     phas = fortran(np.asarray(phas))
-    m = fortran(np.asarray(mask) != 0, np.uint8)
+    m = mask_u8(mask)
     vsz = np.asarray(vsz, dtype=np.float64).reshape(3)
     x = np.empty(phas.shape, dtype=phas.dtype, order="F")  # written whole by the call
     echoes = 1 if phas.ndim == 3 else phas.shape[3]

@@ -16,7 +16,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import check, dtype_code, fortran, h3, lib, ptr, sz3
+from ._lib import check, dtype_code, fortran, h3, lib, mask_u8, ptr, sz3
 
 _sz = C.c_size_t
 _p = C.c_void_p
@@ -80,7 +80,7 @@ def tv(f, mask, vsz, bdir=None, opts=None, return_trace=False):
     if bdir is None or np.size(bdir) == 0:
         bdir = (0.0, 0.0, 1.0)
     o = make_opts(opts)
-    m8 = fortran(m != 0, np.uint8)
+    m8 = mask_u8(m)
     code = dtype_code(f.dtype)
     echoes = 1 if f.ndim == 3 else f.shape[3]
     x = np.zeros(f.shape, dtype=f.dtype, order="F")

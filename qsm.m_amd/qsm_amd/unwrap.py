@@ -13,7 +13,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import check, dtype_code, fortran, h3, lib, ptr, sz3
+from ._lib import check, dtype_code, fortran, h3, lib, mask_u8, ptr, sz3
 
 _sz = C.c_size_t
 _p = C.c_void_p
@@ -52,7 +52,7 @@ def mgpcg(f, mask, vsz, tol=None, maxit=None, mgopts=None, return_info=False):
     if f.dtype not in (np.float32, np.float64):
         raise TypeError("f must be single or double")
     f = fortran(f, f.dtype)
-    m = fortran(np.asarray(mask) != 0, np.uint8)
+    m = mask_u8(mask)
     if m.shape != f.shape or f.ndim != 3:
         raise ValueError("f and mask must be 3-D of the same size")
     if tol is None or np.size(tol) == 0:
@@ -87,7 +87,7 @@ def unwrap_laplacian(phas, mask, vsz):
     if v.size != 3 or not np.all(np.isfinite(v)) or np.any(v <= 0):
         raise ValueError("vsz must be 3 finite values > 0")
     p = fortran(p, p.dtype)
-    m8 = fortran(m != 0, np.uint8)
+    m8 = mask_u8(m)
     ne = 1 if p.ndim == 3 else p.shape[3]
     up = np.zeros(p.shape, dtype=p.dtype, order="F")
     d2 = np.zeros(p.shape, dtype=p.dtype, order="F")

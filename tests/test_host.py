@@ -178,3 +178,18 @@ def test_sstgv_rejects_bad_arguments_before_any_device_work():
     # an empty mask returns zeros without touching the device
     out = sstgv(ph, np.zeros((6, 6, 6), bool), (1, 1, 1))
     assert out.shape == ph.shape and not out.any()
+
+
+def test_mask_argument_is_a_zero_copy_view_when_it_can_be():
+    """Mask arguments reach the library as column-major uint8 0/1: a
+    column-major bool mask by a view of its memory, anything else by one
+    converted copy (row-major, or nonzero values other than 1)."""
+    from qsm_amd._lib import mask_u8
+
+    m = np.asfortranarray(np.random.default_rng(0).random((5, 4, 3)) < 0.5)
+    v = mask_u8(m)
+    assert v.dtype == np.uint8 and v.flags.f_contiguous and np.shares_memory(v, m)
+    w = mask_u8(np.ascontiguousarray(m))
+    assert w.flags.f_contiguous and np.array_equal(w, m.astype(np.uint8))
+    z = mask_u8((m * 255).astype(np.int32))
+    assert z.dtype == np.uint8 and np.array_equal(z, m.astype(np.uint8))
