@@ -60,7 +60,7 @@ def sharp(f, mask, vsz, r=9, thr=0.05):
     mo = np.zeros(m.shape, dtype=np.uint8, order="F")
     check(lib().qsm_sharp(dtype_code(f.dtype), ptr(f), ptr(m), (_sz * 3)(*f.shape[:3]), ne,
                           h3(vsz), float(r), float(thr), ptr(fl), ptr(mo)))
-    return fl, mo.astype(bool)
+    return fl, mo.view(bool)  # 0/1 bytes
 
 
 def vsharp(f, mask, vsz, rs=None, thr=0.05):
@@ -74,7 +74,7 @@ def vsharp(f, mask, vsz, rs=None, thr=0.05):
     check(lib().qsm_vsharp(dtype_code(f.dtype), ptr(f), ptr(m), (_sz * 3)(*f.shape[:3]), ne,
                            h3(vsz), rs.ctypes.data_as(_dp), rs.size, float(thr), ptr(fl),
                            ptr(mo)))
-    return fl, mo.astype(bool)
+    return fl, mo.view(bool)  # 0/1 bytes
 
 
 def _code(dtype):

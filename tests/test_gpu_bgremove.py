@@ -124,3 +124,17 @@ def test_vsharp_register_pass_lengths(cuda, dt):
     fl, mo = gbg.vsharp(f4, m, vsz, rs)
     np.testing.assert_array_equal(fl[..., 0], gf)
     np.testing.assert_array_equal(mo, gm)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_empty_mask_gives_zeros(cuda, dt):
+    """An empty mask: no slice to crop, the local field and the eroded mask
+    are all zeros (sharp.m:51-56 crops to nothing), one echo or two."""
+    shape = (20, 16, 12)
+    f = np.random.default_rng(4).standard_normal(shape + (2,)).astype(dt)
+    m = np.zeros(shape, bool)
+    for fn in (lambda a: gbg.sharp(a, m, (1, 1, 1)), lambda a: gbg.vsharp(a, m, (1, 1, 1))):
+        for a in (f[..., 0], f):
+            fl, mo = fn(np.asfortranarray(a))
+            assert fl.shape == a.shape and fl.dtype == a.dtype and not fl.any()
+            assert mo.shape == shape and not mo.any()

@@ -49,3 +49,30 @@ def test_sstgv_matches_oracle_composition(cuda, dt, alpha):
     box = np.zeros(shape, bool)
     box[tuple(slice(max(0, i[0] - 2), min(n, i[-1] + 3)) for i, n in zip(idx, shape))] = True
     assert got[~box].size > 0 and not got[~box].any()
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_sstgv_mask_on_the_faces_and_single_echo(cuda, dt):
+    """A mask that reaches the volume's low faces on x and y and its high
+    face on z: the box +/- 2 is clamped to the volume (sstgv.m:57-66).  A
+    3-D phase is one echo."""
+    rng = np.random.default_rng(5)
+    shape = (21, 18, 16)
+    mask = np.zeros(shape, bool)
+    mask[0:12, 0:15, 5:16] = rng.random((12, 15, 11)) < 0.9
+    mask[0, 0, 15] = mask[11, 14, 5] = True
+    phas = np.asfortranarray(np.angle(np.exp(1j * rng.uniform(-4, 4, shape))).astype(dt))
+    vsz = (1.0, 1.1, 0.9)
+    got = sstgv(phas, mask, vsz, 3e-3, 2.0, 0.0, 40, False)
+    want = _reference(phas[..., None], mask, vsz, 3e-3, 2.0, 0.0, 40)[..., 0]
+    assert got.shape == shape and got.dtype == phas.dtype
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_sstgv_empty_mask_is_zero(cuda, dt):
+    """An empty mask: nothing to invert, chi = 0 (sstgv.m's crop is empty)."""
+    shape = (12, 10, 8)
+    phas = np.asfortranarray(np.random.default_rng(2).uniform(-3, 3, shape + (2,)).astype(dt))
+    got = sstgv(phas, np.zeros(shape, bool), (1, 1, 1), 3e-3, 2.0, 0.0, 10, False)
+    assert got.shape == phas.shape and got.dtype == phas.dtype and not got.any()
