@@ -560,7 +560,9 @@ def admm_leg(cfg, field, mask, iters=20, cpu_iters=2):
     precisions, `iters` ADMM iterations (the default maxit) with the
     stopping tolerances at 0 so that every run does them all; one warm-up
     solve, then one solve timed with HIP events on its stream (the loop is
-    decided on the device: one synchronisation per solve).
+    decided on the device: one synchronisation per solve).  default_opts:
+    one more timed solve with the method's default tolerances, which stop
+    the loop early; the launches enqueued after the stop return at once.
 
     hbm_bytes per ADMM iteration: right-hand side 7 N s (v, l read with -1
     neighbours, r written), the half-spectrum solve 2 N s + 9 S (S = H ny nz
