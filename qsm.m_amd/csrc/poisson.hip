@@ -468,32 +468,12 @@ __global__ __launch_bounds__(NT) void zero_kernel(T *a, size_t n)
     if (l < n) a[l] = T(0);
 }
 
-// p = q + beta p
-template <typename T>
-__global__ __launch_bounds__(NT) void xpby_kernel(T *p, const T *q, T beta, size_t n)
-{
-    const size_t l = size_t(blockIdx.x) * NT + threadIdx.x;
-    if (l < n) p[l] = beta * p[l] + q[l];
-}
-
 // r -= q
 template <typename T>
 __global__ __launch_bounds__(NT) void sub_kernel(T *r, const T *q, size_t n)
 {
     const size_t l = size_t(blockIdx.x) * NT + threadIdx.x;
     if (l < n) r[l] = r[l] - q[l];
-}
-
-// x += a p; r -= a q
-template <typename T>
-__global__ __launch_bounds__(NT) void update_kernel(T *x, T *r, const T *p, const T *q, T a,
-                                                    size_t n)
-{
-    const size_t l = size_t(blockIdx.x) * NT + threadIdx.x;
-    if (l < n) {
-        x[l] = x[l] + a * p[l];
-        r[l] = r[l] - a * q[l];
-    }
 }
 
 // partial sums of a[l] * b[l] over a fixed grid of DOT_BLOCKS blocks (each
@@ -528,6 +508,67 @@ __global__ __launch_bounds__(NT) void sum_kernel(const double *part, int nb, dou
     if (threadIdx.x == 0) *out = red[0];
 }
 
+// The PCG loop's scalars on the device (mgpcg_mex.c: mx_pcg), so that an
+// iteration reads back only ||r||^2 and the breakdown flag.
+struct Pcg {
+    double rho, prevrho, pq, rr;
+    int brk;  // rho or pq not positive: the reference's loop breaks there
+    int pad;
+};
+
+// sum_kernel's reduction into the PCG scalars.  WHAT 0: rr; 1: rho, the
+// previous rho kept as prevrho, breakdown when !(rho > 0); 2: pq,
+// breakdown when !(pq > 0).
+template <int WHAT>
+__global__ __launch_bounds__(NT) void pcg_sum_kernel(const double *part, int nb, Pcg *g)
+{
+    __shared__ double red[NT];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += NT) s += part[b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = NT / 2; k > 0; k >>= 1) {
+        if (int(threadIdx.x) < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double v = red[0];
+    if (WHAT == 0) {
+        g->rr = v;
+    } else if (WHAT == 1) {
+        g->prevrho = g->rho;
+        g->rho = v;
+        if (!(v > 0.0)) g->brk = 1;
+    } else {
+        g->pq = v;
+        if (!(v > 0.0)) g->brk = 1;
+    }
+}
+
+// p = q + beta p with beta = rho / prevrho (skipped after a breakdown)
+template <typename T>
+__global__ __launch_bounds__(NT) void xpby_pcg_kernel(T *p, const T *q, const Pcg *g, size_t n)
+{
+    if (g->brk) return;
+    const T beta = T(g->rho / g->prevrho);
+    const size_t l = size_t(blockIdx.x) * NT + threadIdx.x;
+    if (l < n) p[l] = beta * p[l] + q[l];
+}
+
+// x += a p; r -= a q with a = rho / pq (skipped after a breakdown)
+template <typename T>
+__global__ __launch_bounds__(NT) void update_pcg_kernel(T *x, T *r, const T *p, const T *q,
+                                                        const Pcg *g, size_t n)
+{
+    if (g->brk) return;
+    const T a = T(g->rho / g->pq);
+    const size_t l = size_t(blockIdx.x) * NT + threadIdx.x;
+    if (l < n) {
+        x[l] = x[l] + a * p[l];
+        r[l] = r[l] - a * q[l];
+    }
+}
+
 unsigned g1(size_t n) { return unsigned((n + NT - 1) / NT); }
 
 size_t al(size_t n) { return (n + 255) / 256 * 256; }
@@ -553,6 +594,7 @@ struct Solver {
     uint8_t *tmp;       // band seed scratch (level-0 size)
     int *rowcnt, *btot; // band list construction: per-row counts, totals per level
     double *part, *sum; // reductions
+    Pcg *pcg;           // the PCG scalars
 };
 
 // Level sizes: the padded grid, then ((n - 2 + 1) >> 1) + 2 per level
@@ -592,6 +634,7 @@ size_t solver_layout(const size_t sz[3], const double vsz[3], int L, Solver<T> *
     const size_t nc = d[L - 1].n();
     char *pcp = take(nc * sizeof(T)), *pcq = take(nc * sizeof(T));
     char *ppart = take(DOT_BLOCKS * sizeof(double)), *psum = take(sizeof(double));
+    char *ppcg = take(sizeof(Pcg));
     if (s) {
         s->L = L;
         s->x = reinterpret_cast<T *>(px);
@@ -603,6 +646,7 @@ size_t solver_layout(const size_t sz[3], const double vsz[3], int L, Solver<T> *
         s->btot = reinterpret_cast<int *>(pbt);
         s->part = reinterpret_cast<double *>(ppart);
         s->sum = reinterpret_cast<double *>(psum);
+        s->pcg = reinterpret_cast<Pcg *>(ppcg);
     }
     return off;
 }
@@ -803,11 +847,13 @@ int mgpcg_t(const T *src, bool unwrap, const uint8_t *mask, const size_t sz[3],
         zero_kernel<T><<<g1(n0), NT, 0, st>>>(s.x, n0);
         QSM_HIP(hipGetLastError());
     }
-    double rho = 1.0, prevrho;
+    // rho, prevrho, pq and ||r||^2 stay on the device; each iteration reads
+    // back ||r||^2 and the breakdown flag once, at its end
+    const Pcg g0{1.0, 1.0, 0.0, 0.0, 0, 0};  // rho = 1 before the first iteration
+    QSM_HIP(hipMemcpyAsync(s.pcg, &g0, sizeof(Pcg), hipMemcpyHostToDevice, st));
     int it = 0;
     for (; it < maxit; ++it) {
         if (res <= reltol) break;
-        prevrho = rho;
         // q = M \ r: m.cycles V-cycles from q = 0 (mgpcg_mex.c: mx_mg; with
         // mgopts.tol > 0 they stop once ||r - A q|| < mgopts.tol ||r||)
         zero_kernel<T><<<g1(n0), NT, 0, st>>>(q, n0);
@@ -823,23 +869,27 @@ int mgpcg_t(const T *src, bool unwrap, const uint8_t *mask, const size_t sz[3],
             }
             if ((rc = cycle(s, 0, m, st))) return rc;
         }
-        if ((rc = dot(r, q, n0, s, rho, st))) return rc;
-        // breakdown: the residual is exactly zero on the mask (the rule
-        // above never stops a right-hand side that is nonzero off the mask,
-        // since ||r|| counts the whole array), so no direction is left
-        if (!(rho > 0.0)) break;
-        const double beta = rho / prevrho;
-        xpby_kernel<T><<<g1(n0), NT, 0, st>>>(s.p, q, T(beta), n0);
+        // rho = r.q (prevrho = the last one); breakdown when !(rho > 0): the
+        // residual is exactly zero on the mask (the rule above never stops
+        // a right-hand side that is nonzero off the mask, since ||r|| counts
+        // the whole array), so no direction is left
+        dot_kernel<T><<<DOT_BLOCKS, NT, 0, st>>>(r, q, n0, s.part);
+        pcg_sum_kernel<1><<<1, NT, 0, st>>>(s.part, DOT_BLOCKS, s.pcg);
+        // p = q + (rho / prevrho) p; q = A p; pq = p.q (breakdown: !(pq > 0))
+        xpby_pcg_kernel<T><<<g1(n0), NT, 0, st>>>(s.p, q, s.pcg, n0);
         apply_kernel<T><<<grid3(L0.d), b, 0, st>>>(q, s.p, L0.G, L0.d, L0.c);
+        dot_kernel<T><<<DOT_BLOCKS, NT, 0, st>>>(s.p, q, n0, s.part);
+        pcg_sum_kernel<2><<<1, NT, 0, st>>>(s.part, DOT_BLOCKS, s.pcg);
+        // x += (rho / pq) p; r -= (rho / pq) q; ||r||^2
+        update_pcg_kernel<T><<<g1(n0), NT, 0, st>>>(s.x, r, s.p, q, s.pcg, n0);
+        dot_kernel<T><<<DOT_BLOCKS, NT, 0, st>>>(r, r, n0, s.part);
+        pcg_sum_kernel<0><<<1, NT, 0, st>>>(s.part, DOT_BLOCKS, s.pcg);
         QSM_HIP(hipGetLastError());
-        double pq;
-        if ((rc = dot(s.p, q, n0, s, pq, st))) return rc;
-        if (!(pq > 0.0)) break;
-        const double a = rho / pq;
-        update_kernel<T><<<g1(n0), NT, 0, st>>>(s.x, r, s.p, q, T(a), n0);
-        QSM_HIP(hipGetLastError());
-        if ((rc = dot(r, r, n0, s, res, st))) return rc;
-        res = std::sqrt(res);
+        Pcg g;
+        QSM_HIP(hipMemcpyAsync(&g, s.pcg, sizeof(Pcg), hipMemcpyDeviceToHost, st));
+        QSM_HIP(hipStreamSynchronize(st));
+        if (g.brk) break;  // x and r are as the reference leaves them
+        res = std::sqrt(g.rr);
         if (resnorm) resnorm[it] = res;
     }
     if (iters) *iters = it;
