@@ -182,3 +182,28 @@ def test_sharp_work_bytes_refuse_empty_crop_and_no_echoes():
     assert _lib.lib().qsm_sharp_work_bytes(_lib.F64, sz0, h, 9.0) == 0
     assert _lib.lib().qsm_vsharp_work_bytes(_lib.F64, sz, h, 9.0, 0) == 0
     assert _lib.lib().qsm_sharp_work_bytes(7, sz, h, 9.0) == 0
+
+
+def test_sstgv_refuses_bad_arguments_before_device_work():
+    """qsm_sstgv checks its arguments before it looks for a device: a bad
+    class, a null pointer, no echo, maxit 0 and a non-positive voxel size
+    are QSM_EARG (with the reason as the last error)."""
+    import numpy as np
+
+    lib = _lib.lib()
+    f = np.zeros((4, 4, 4), np.float64, order="F")
+    m = np.ones((4, 4, 4), np.uint8, order="F")
+    x = np.zeros_like(f)
+    sz = (C.c_size_t * 3)(4, 4, 4)
+    good = (C.c_double * 3)(1.0, 1.0, 1.0)
+    bad = (C.c_double * 3)(1.0, 0.0, 1.0)
+    noprint = _lib.PRINT_FN()
+
+    def call(dtype=_lib.F64, fp=f.ctypes.data, ne=1, vsz=good, maxit=10):
+        return lib.qsm_sstgv(dtype, fp, m.ctypes.data, sz, ne, vsz, 1e-3, 2.0, 0.0, maxit, 0,
+                             x.ctypes.data, noprint, None)
+
+    for kw, why in ((dict(dtype=7), "dtype"), (dict(fp=None), "null"), (dict(ne=0), "echo"),
+                    (dict(maxit=0), "maxit"), (dict(vsz=bad), "voxel size")):
+        assert call(**kw) == _lib.QSM_EARG, kw
+        assert why in lib.qsm_last_error().decode(), (kw, lib.qsm_last_error())
